@@ -1,0 +1,87 @@
+"""ctypes binding of libalamo_amd.so (the C ABI declared in include/alamo_amd.h).
+
+The library is built in-tree by alamo_amd/build.py.  Loading fails loudly when it
+is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libalamo_amd.so")
+
+c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+P = ctypes.POINTER
+
+
+class MGDesc(ctypes.Structure):
+    _fields_ = [("n_cell", c_int * 3), ("dx", c_double * 3), ("sym", c_int),
+                ("bc_type", (c_int * 3) * 26), ("bc_val", (c_double * 3) * 26),
+                ("max_coarsening_level", c_int), ("precision", c_int), ("cycle", c_int),
+                ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
+                ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
+                ("bottom_tol_rel", c_double), ("verbose", c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("converged", c_int), ("initial_res_inf", c_double),
+                ("final_res_inf", c_double), ("final_res_l2", c_double), ("rhs_inf", c_double),
+                ("target", c_double), ("seconds", c_double), ("fallback_events", c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "alamo_last_error": (ctypes.c_char_p, []),
+    "alamo_version": (c_int, []),
+    "alamo_mg_desc_default": (c_int, [P(MGDesc)]),
+    "alamo_mg_create": (c_int, [P(MGDesc), P(c_void_p)]),
+    "alamo_mg_destroy": (c_int, [c_void_p]),
+    "alamo_mg_set_stream": (c_int, [c_void_p, c_void_p]),
+    "alamo_mg_num_levels": (c_int, [c_void_p, P(c_int)]),
+    "alamo_mg_level_ncell": (c_int, [c_void_p, c_int, P(c_int)]),
+    "alamo_mg_set_model": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "alamo_mg_set_model_host": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "alamo_mg_set_model_grains_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "alamo_mg_prepare": (c_int, [c_void_p]),
+    "alamo_mg_get_coeff": (c_int, [c_void_p, c_int, c_void_p]),
+    "alamo_mg_fapply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_diagonal": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "alamo_mg_fsmooth": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_residual": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_restrict": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_interpolate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_norms": (c_int, [c_void_p, c_int, c_void_p, P(c_double), P(c_double), c_void_p]),
+    "alamo_bc_init": (c_int, [c_void_p, c_void_p, c_int]),
+    "alamo_newton_rhs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_linear_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(SolveStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libalamo_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class AlamoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("alamo error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != 0:
+        raise AlamoError(rc, lib().alamo_last_error().decode())
+    return rc

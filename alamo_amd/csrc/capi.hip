@@ -1,0 +1,739 @@
+// C ABI, multigrid hierarchy and solve driver of libalamo_amd.so.
+//
+// The driver restates amrex::MLMG (AMReX 26.06, which the reference drives through
+// Solver/Nonlocal/Linear.H:255-290) from its documented algorithm: F-cycle every
+// iteration (setMaxFmgIter(1e8), Linear.H:261) or V-cycles, pre/post smoothing by
+// Operator<Node>::Fsmooth, restriction of the correction residual, a
+// Jacobi-preconditioned BiCGStab bottom solve and the max-norm stopping test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/alamo_amd.h"
+#include "kernels_generic.h"
+#include "kernels_iso.h"
+
+using namespace alamo;
+
+static thread_local std::string g_err;
+
+#define FAIL(code, msg)            \
+    do {                           \
+        g_err = (msg);             \
+        return (code);             \
+    } while (0)
+
+#define HIPCHK(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            g_err = std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" +        \
+                    std::to_string(__LINE__) + " (" #expr ")";                                            \
+            return ALAMO_ERR_HIP;                                                                         \
+        }                                                                                                 \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != ALAMO_OK) return r_; \
+    } while (0)
+
+namespace {
+
+struct LevelBuf {
+    Geom g{};
+    double* C = nullptr;   // FP64 stiffness, ncoef SoA planes
+    double* D = nullptr;   // FP64 diagonal, 3 planes
+    double *cor = nullptr, *res = nullptr, *rescor = nullptr, *tmp = nullptr, *hold = nullptr;
+    float* Cf = nullptr;
+    float* Df = nullptr;
+    float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
+};
+
+}  // namespace
+
+struct alamo_mg {
+    alamo_mg_desc d{};
+    BCTable bc{};
+    int sym = 0, nc = 2;
+    hipStream_t stream = nullptr;
+    std::vector<LevelBuf> L;
+    double* partial = nullptr;      // reduction partials
+    double* red = nullptr;          // device reduction result (4 doubles)
+    double* red_host = nullptr;     // pinned host mirror
+    double* bottom_work = nullptr;  // 8 x 3 x tot(bottom)
+    int* bottom_iters = nullptr;
+    double* sol_res = nullptr;      // level-0 FP64 residual for the mixed-precision outer loop
+    bool model_set = false, prepared = false;
+    int nblk_red = 1024;
+    std::vector<void*> allocs;
+};
+
+static Geom make_geom(const int* n, const double* dx) {
+    Geom g{};
+    for (int d = 0; d < 3; d++) {
+        g.n[d] = n[d];
+        g.N[d] = n[d] + 1;
+        g.dx[d] = dx[d];
+    }
+    g.sy = g.N[0];
+    g.sz = (int64_t)g.N[0] * g.N[1];
+    g.tot = g.sz * g.N[2];
+    return g;
+}
+
+static inline unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned gs_blocks(int64_t n) { return (unsigned)std::min<int64_t>(4096, (n + 255) / 256); }
+
+template <typename T>
+static int dalloc(alamo_mg* h, T** p, int64_t n) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, std::max<int64_t>(1, n) * sizeof(T)));
+    h->allocs.push_back(q);
+    *p = (T*)q;
+    return ALAMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch
+// ---------------------------------------------------------------------------
+template <typename T, int SYM>
+static void launch_node_sym(int mode, const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b, T* out,
+                            T omega, hipStream_t s) {
+    dim3 grid(nblocks(g.tot)), blk(256);
+    switch (mode) {
+        case APPLY: k_node<T, SYM, APPLY><<<grid, blk, 0, s>>>(g, bc, C, u, b, out, omega); break;
+        case RESID: k_node<T, SYM, RESID><<<grid, blk, 0, s>>>(g, bc, C, u, b, out, omega); break;
+        case JACOBI: k_node<T, SYM, JACOBI><<<grid, blk, 0, s>>>(g, bc, C, u, b, out, omega); break;
+        default: k_node<T, SYM, DIAG><<<grid, blk, 0, s>>>(g, bc, C, u, b, out, omega); break;
+    }
+}
+
+template <typename T>
+static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b,
+                        T* out, T omega, hipStream_t s) {
+    if (sym == ISO) {
+        // large isotropic levels take the LDS-tiled interior kernel plus the boundary kernel
+        if (mode != DIAG && iso_tiled_ok(g)) {
+            launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
+            return;
+        }
+        launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);
+    } else if (sym == MAJORMINOR) {
+        launch_node_sym<T, MAJORMINOR>(mode, g, bc, C, u, b, out, omega, s);
+    } else {
+        launch_node_sym<T, DIAGONAL>(mode, g, bc, C, u, b, out, omega, s);
+    }
+}
+
+template <typename T>
+static void launch_bottom(int sym, const Geom& g, const BCTable& bc, const T* C, const T* D, T* x, const T* b, T* w,
+                          double rtol, int maxit, int* it, hipStream_t s) {
+    if (sym == ISO) k_bottom_bicgstab<T, ISO><<<1, BOTTOM_BLOCK, 0, s>>>(g, bc, C, D, x, b, w, rtol, maxit, it);
+    else if (sym == MAJORMINOR)
+        k_bottom_bicgstab<T, MAJORMINOR><<<1, BOTTOM_BLOCK, 0, s>>>(g, bc, C, D, x, b, w, rtol, maxit, it);
+    else k_bottom_bicgstab<T, DIAGONAL><<<1, BOTTOM_BLOCK, 0, s>>>(g, bc, C, D, x, b, w, rtol, maxit, it);
+}
+
+template <typename T>
+static void vfill(T* x, int64_t n, T v, hipStream_t s) {
+    k_fill<T><<<gs_blocks(n), 256, 0, s>>>(x, n, v);
+}
+
+template <typename T>
+static int norms_dev(alamo_mg* h, const T* x, int64_t n, double* ninf, double* nl2) {
+    const int nb = (int)std::min<int64_t>(h->nblk_red, (n + RED_BLOCK - 1) / RED_BLOCK);
+    k_norm_partial<T><<<nb, RED_BLOCK, 0, h->stream>>>(x, n, h->partial);
+    k_norm_final<<<1, RED_BLOCK, 0, h->stream>>>(h->partial, nb, h->red);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->red, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (ninf) *ninf = h->red_host[0];
+    if (nl2) *nl2 = std::sqrt(h->red_host[1]);
+    return ALAMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid cycles (templated on the cycle precision T)
+// ---------------------------------------------------------------------------
+template <typename T> struct LB;
+template <> struct LB<double> {
+    static double*& C(LevelBuf& l) { return l.C; }
+    static double*& D(LevelBuf& l) { return l.D; }
+    static double*& cor(LevelBuf& l) { return l.cor; }
+    static double*& res(LevelBuf& l) { return l.res; }
+    static double*& rescor(LevelBuf& l) { return l.rescor; }
+    static double*& tmp(LevelBuf& l) { return l.tmp; }
+    static double*& hold(LevelBuf& l) { return l.hold; }
+};
+template <> struct LB<float> {
+    static float*& C(LevelBuf& l) { return l.Cf; }
+    static float*& D(LevelBuf& l) { return l.Df; }
+    static float*& cor(LevelBuf& l) { return l.corf; }
+    static float*& res(LevelBuf& l) { return l.resf; }
+    static float*& rescor(LevelBuf& l) { return l.rescorf; }
+    static float*& tmp(LevelBuf& l) { return l.tmpf; }
+    static float*& hold(LevelBuf& l) { return l.holdf; }
+};
+
+template <typename T>
+static void smooth(alamo_mg* h, int l, int ncalls) {
+    LevelBuf& lb = h->L[l];
+    for (int c = 0; c < ncalls; c++)
+        for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
+            launch_node<T>(h->sym, JACOBI, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::tmp(lb),
+                           T(h->d.omega), h->stream);
+            std::swap(LB<T>::cor(lb), LB<T>::tmp(lb));
+        }
+}
+
+template <typename T>
+static void bottom(alamo_mg* h) {
+    const int bot = (int)h->L.size() - 1;
+    LevelBuf& lb = h->L[bot];
+    const int64_t n = 3 * lb.g.tot;
+    // the bottom solve always runs in FP64
+    if constexpr (std::is_same<T, float>::value)
+        k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(lb.resf, lb.res, n);
+    vfill<double>(lb.cor, n, 0.0, h->stream);
+    launch_bottom<double>(h->sym, lb.g, h->bc, lb.C, lb.D, lb.cor, lb.res, h->bottom_work, h->d.bottom_tol_rel,
+                          h->d.bottom_max_iter, h->bottom_iters, h->stream);
+    if constexpr (std::is_same<T, float>::value)
+        k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(lb.cor, lb.corf, n);
+}
+
+template <typename T>
+static void vcycle(alamo_mg* h, int top) {
+    const int bot = (int)h->L.size() - 1;
+    for (int l = top; l < bot; l++) {
+        LevelBuf& lb = h->L[l];
+        vfill<T>(LB<T>::cor(lb), 3 * lb.g.tot, T(0), h->stream);
+        smooth<T>(h, l, h->d.pre_smooth);
+        launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
+                       T(0), h->stream);
+        LevelBuf& lc = h->L[l + 1];
+        k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, 3, LB<T>::rescor(lb), LB<T>::res(lc));
+    }
+    bottom<T>(h);
+    for (int l = bot - 1; l >= top; l--) {
+        LevelBuf& lb = h->L[l];
+        LevelBuf& lc = h->L[l + 1];
+        k_interp_add<T><<<nblocks(lb.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        smooth<T>(h, l, h->d.post_smooth);
+    }
+}
+
+template <typename T>
+static void fcycle(alamo_mg* h) {
+    const int bot = (int)h->L.size() - 1;
+    for (int l = 1; l <= bot; l++) {
+        LevelBuf& lf = h->L[l - 1];
+        LevelBuf& lc = h->L[l];
+        k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lf.g, 3, LB<T>::res(lf), LB<T>::res(lc));
+    }
+    bottom<T>(h);
+    for (int l = bot - 1; l >= 0; l--) {
+        LevelBuf& lb = h->L[l];
+        LevelBuf& lc = h->L[l + 1];
+        const int64_t n = 3 * lb.g.tot;
+        vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
+        k_interp_add<T><<<nblocks(lb.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        // res = res - L(cor)
+        launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
+                       T(0), h->stream);
+        std::swap(LB<T>::res(lb), LB<T>::rescor(lb));
+        std::swap(LB<T>::hold(lb), LB<T>::cor(lb));
+        vcycle<T>(h, l);
+        k_add<T><<<gs_blocks(n), 256, 0, h->stream>>>(LB<T>::cor(lb), LB<T>::cor(lb), LB<T>::hold(lb), n);
+    }
+}
+
+template <typename T>
+static void one_cycle(alamo_mg* h) {
+    if (h->d.cycle == ALAMO_CYCLE_F) fcycle<T>(h);
+    else vcycle<T>(h, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Setup
+// ---------------------------------------------------------------------------
+static int prepare(alamo_mg* h) {
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "alamo_mg_prepare: SetModel has not been called");
+    const int nl = (int)h->L.size();
+    for (int l = 1; l < nl; l++) {
+        LevelBuf& lf = h->L[l - 1];
+        LevelBuf& lc = h->L[l];
+        k_restrict<double><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lf.g, h->nc, lf.C, lc.C);
+    }
+    for (int l = 0; l < nl; l++) {
+        LevelBuf& lb = h->L[l];
+        launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, lb.D, 0.0, h->stream);
+        if (h->d.precision == ALAMO_PREC_MIXED) {
+            k_convert<double, float><<<gs_blocks(h->nc * lb.g.tot), 256, 0, h->stream>>>(lb.C, lb.Cf,
+                                                                                         h->nc * lb.g.tot);
+            k_convert<double, float><<<gs_blocks(3 * lb.g.tot), 256, 0, h->stream>>>(lb.D, lb.Df, 3 * lb.g.tot);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    h->prepared = true;
+    return ALAMO_OK;
+}
+
+static int check_level(alamo_mg* h, int amrlev, int mglev) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
+    if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    if (!h->prepared) TRY(prepare(h));
+    return ALAMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Solve
+// ---------------------------------------------------------------------------
+static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                      alamo_solve_stats* st) {
+    if (h->L.back().g.tot > 17 * 17 * 17)
+        FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
+    if (!h->prepared) TRY(prepare(h));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    const bool mixed = h->d.precision == ALAMO_PREC_MIXED;
+    double* res0 = mixed ? h->sol_res : l0.res;
+    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0, 0.0, h->stream);
+    double r0 = 0, bn = 0, rl2 = 0;
+    TRY(norms_dev<double>(h, res0, n, &r0, &rl2));
+    TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
+    double rn = r0;
+    int it = 0;
+    if (h->d.verbose) fprintf(stderr, "alamo: iter 0 |r|inf %.6e target %.3e\n", r0, target);
+    while (rn > target && it < h->d.max_iter && std::isfinite(rn)) {
+        if (mixed) {
+            k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(res0, l0.resf, n);
+            one_cycle<float>(h);
+            k_axpy<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, l0.corf, n, 1.0);
+        } else {
+            one_cycle<double>(h);
+            k_axpy<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, l0.cor, n, 1.0);
+        }
+        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0, 0.0, h->stream);
+        TRY(norms_dev<double>(h, res0, n, &rn, &rl2));
+        it++;
+        if (h->d.verbose) fprintf(stderr, "alamo: iter %d |r|inf %.6e\n", it, rn);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    const bool conv = rn <= target;
+    if (st) {
+        st->iterations = it;
+        st->converged = conv ? 1 : 0;
+        st->initial_res_inf = r0;
+        st->final_res_inf = rn;
+        st->final_res_l2 = rl2;
+        st->rhs_inf = bn;
+        st->target = target;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->fallback_events = 0;
+    }
+    if (!std::isfinite(rn)) FAIL(ALAMO_ERR_NAN, "alamo_linear_solve: residual is not finite (diverged)");
+    if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_linear_solve: max_iter reached before the tolerance");
+    return ALAMO_OK;
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* alamo_last_error(void) { return g_err.c_str(); }
+int alamo_version(void) { return 1; }
+
+int alamo_mg_desc_default(alamo_mg_desc* d) {
+    if (!d) FAIL(ALAMO_ERR_ARG, "null desc");
+    memset(d, 0, sizeof(*d));
+    for (int f = 0; f < 26; f++)
+        for (int c = 0; c < 3; c++) {
+            d->bc_type[f][c] = ALAMO_BC_DISPLACEMENT;
+            d->bc_val[f][c] = 0.0;
+        }
+    d->sym = ALAMO_SYM_ISOTROPIC;
+    d->max_coarsening_level = -1;
+    d->precision = ALAMO_PREC_FP64;
+    d->cycle = ALAMO_CYCLE_F;
+    d->pre_smooth = 2;
+    d->post_smooth = 2;
+    d->sweeps_per_smooth = 2;
+    d->omega = 2.0 / 3.0;
+    d->max_iter = 200;
+    d->bottom_max_iter = 200;
+    d->bottom_tol_rel = 1e-4;
+    d->verbose = 0;
+    return ALAMO_OK;
+}
+
+int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
+    if (!d || !out) FAIL(ALAMO_ERR_ARG, "null argument");
+    for (int q = 0; q < 3; q++) {
+        if (d->n_cell[q] < 2) FAIL(ALAMO_ERR_ARG, "n_cell must be >= 2 in every direction");
+        if (!(d->dx[q] > 0)) FAIL(ALAMO_ERR_ARG, "dx must be positive");
+    }
+    if (d->sym < 0 || d->sym > 2) FAIL(ALAMO_ERR_ARG, "unknown sym");
+    for (int f = 0; f < 26; f++)
+        for (int c = 0; c < 3; c++) {
+            int t = d->bc_type[f][c];
+            if (t < 0 || t > 4) FAIL(ALAMO_ERR_ARG, "unknown bc type");
+            if (t == ALAMO_BC_PERIODIC) FAIL(ALAMO_ERR_ARG, "periodic boundaries are not supported by this handle");
+        }
+    if (d->sweeps_per_smooth < 1 || d->pre_smooth < 0 || d->post_smooth < 0) FAIL(ALAMO_ERR_ARG, "bad smoothing");
+    alamo_mg* h = new alamo_mg();
+    h->d = *d;
+    h->sym = d->sym;
+    h->nc = ncoef_of(d->sym);
+    for (int f = 0; f < 26; f++)
+        for (int c = 0; c < 3; c++) h->bc.t[f][c] = (int8_t)d->bc_type[f][c];
+    // hierarchy: coarsen by 2 while every direction keeps >= 2 cells (AMReX mg_domain_min_width)
+    int n[3] = {d->n_cell[0], d->n_cell[1], d->n_cell[2]};
+    double dx[3] = {d->dx[0], d->dx[1], d->dx[2]};
+    std::vector<Geom> gs;
+    gs.push_back(make_geom(n, dx));
+    while ((d->max_coarsening_level < 0 || (int)gs.size() <= d->max_coarsening_level) && n[0] % 2 == 0 &&
+           n[1] % 2 == 0 && n[2] % 2 == 0 && n[0] / 2 >= 2 && n[1] / 2 >= 2 && n[2] / 2 >= 2) {
+        for (int q = 0; q < 3; q++) { n[q] /= 2; dx[q] *= 2; }
+        gs.push_back(make_geom(n, dx));
+    }
+    const bool mixed = d->precision == ALAMO_PREC_MIXED;
+    h->L.resize(gs.size());
+    int rc = ALAMO_OK;
+    for (size_t l = 0; l < gs.size() && rc == ALAMO_OK; l++) {
+        LevelBuf& lb = h->L[l];
+        lb.g = gs[l];
+        const int64_t v = 3 * lb.g.tot;
+        rc |= dalloc(h, &lb.C, h->nc * lb.g.tot);
+        rc |= dalloc(h, &lb.D, v);
+        // FP64 cycle buffers: all levels for FP64 solves; the bottom level always (FP64 bottom solve)
+        if (!mixed || l == gs.size() - 1 || l == 0) {
+            rc |= dalloc(h, &lb.cor, v);
+            rc |= dalloc(h, &lb.res, v);
+            if (!mixed) {
+                rc |= dalloc(h, &lb.rescor, v);
+                rc |= dalloc(h, &lb.tmp, v);
+                rc |= dalloc(h, &lb.hold, v);
+            }
+        }
+        if (mixed) {
+            rc |= dalloc(h, &lb.Cf, h->nc * lb.g.tot);
+            rc |= dalloc(h, &lb.Df, v);
+            rc |= dalloc(h, &lb.corf, v);
+            rc |= dalloc(h, &lb.resf, v);
+            rc |= dalloc(h, &lb.rescorf, v);
+            rc |= dalloc(h, &lb.tmpf, v);
+            rc |= dalloc(h, &lb.holdf, v);
+        }
+    }
+    rc |= dalloc(h, &h->partial, 2 * h->nblk_red);
+    rc |= dalloc(h, &h->red, 4);
+    rc |= dalloc(h, &h->bottom_work, 8 * 3 * gs.back().tot);
+    rc |= dalloc(h, &h->bottom_iters, 1);
+    if (mixed) rc |= dalloc(h, &h->sol_res, 3 * gs[0].tot);
+    if (rc == ALAMO_OK && hipHostMalloc((void**)&h->red_host, 4 * sizeof(double)) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        rc = ALAMO_ERR_HIP;
+    }
+    if (rc != ALAMO_OK) {
+        std::string e = g_err;
+        alamo_mg_destroy(h);
+        g_err = e;
+        return ALAMO_ERR_HIP;
+    }
+    *out = h;
+    return ALAMO_OK;
+}
+
+int alamo_mg_destroy(alamo_mg* h) {
+    if (!h) return ALAMO_OK;
+    (void)hipDeviceSynchronize();
+    for (void* p : h->allocs) (void)hipFree(p);
+    if (h->red_host) (void)hipHostFree(h->red_host);
+    delete h;
+    return ALAMO_OK;
+}
+
+int alamo_mg_set_stream(alamo_mg* h, void* stream) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    h->stream = (hipStream_t)stream;
+    return ALAMO_OK;
+}
+
+int alamo_mg_num_levels(alamo_mg* h, int* nlev) {
+    if (!h || !nlev) FAIL(ALAMO_ERR_ARG, "null argument");
+    *nlev = (int)h->L.size();
+    return ALAMO_OK;
+}
+
+int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3) {
+    if (!h || !ncell3) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
+    for (int d = 0; d < 3; d++) ncell3[d] = h->L[mglev].g.n[d];
+    return ALAMO_OK;
+}
+
+static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, bool host) {
+    if (!h || !C) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
+    if (layout != ALAMO_LAYOUT_AOS && layout != ALAMO_LAYOUT_SOA) FAIL(ALAMO_ERR_ARG, "unknown layout");
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = h->nc * l0.g.tot;
+    const double* src = C;
+    double* staged = nullptr;
+    if (host) {
+        HIPCHK(hipMalloc(&staged, n * sizeof(double)));
+        HIPCHK(hipMemcpy(staged, C, n * sizeof(double), hipMemcpyHostToDevice));
+        src = staged;
+    }
+    if (layout == ALAMO_LAYOUT_SOA) {
+        HIPCHK(hipMemcpyAsync(l0.C, src, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        k_aos_to_soa<double><<<gs_blocks(l0.g.tot), 256, 0, h->stream>>>(src, l0.C, l0.g.tot, h->nc);
+    }
+    HIPCHK(hipGetLastError());
+    if (staged) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipFree(staged));
+    }
+    h->model_set = true;
+    h->prepared = false;
+    return ALAMO_OK;
+}
+
+int alamo_mg_set_model(alamo_mg* h, int amrlev, const double* C, int layout) {
+    return set_model_impl(h, amrlev, C, layout, false);
+}
+int alamo_mg_set_model_host(alamo_mg* h, int amrlev, const double* C, int layout) {
+    return set_model_impl(h, amrlev, C, layout, true);
+}
+
+int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const double* table, int ngrains) {
+    if (!h || !gid || !table || ngrains <= 0) FAIL(ALAMO_ERR_ARG, "bad argument");
+    if (h->sym != ALAMO_SYM_MAJORMINOR) FAIL(ALAMO_ERR_ARG, "grain tables need sym MAJORMINOR");
+    const int64_t tot = h->L[0].g.tot;
+    std::vector<double> soa((size_t)21 * tot);
+    for (int64_t m = 0; m < tot; m++) {
+        const int gi = gid[m];
+        if (gi >= ngrains) FAIL(ALAMO_ERR_ARG, "grain id out of range");
+        for (int q = 0; q < 21; q++) soa[(size_t)q * tot + m] = table[21 * gi + q];
+    }
+    return set_model_impl(h, 0, soa.data(), ALAMO_LAYOUT_SOA, true);
+}
+
+int alamo_mg_prepare(alamo_mg* h) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    return prepare(h);
+}
+
+int alamo_mg_get_coeff(alamo_mg* h, int mglev, double* C_soa) {
+    TRY(check_level(h, 0, mglev));
+    LevelBuf& lb = h->L[mglev];
+    HIPCHK(hipMemcpyAsync(C_soa, lb.C, h->nc * lb.g.tot * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ALAMO_OK;
+}
+
+int alamo_mg_fapply(alamo_mg* h, int amrlev, int mglev, double* out, const double* in, void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    LevelBuf& lb = h->L[mglev];
+    launch_node<double>(h->sym, APPLY, lb.g, h->bc, lb.C, in, nullptr, out, 0.0, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    LevelBuf& lb = h->L[mglev];
+    launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, diag, 0.0, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    LevelBuf& lb = h->L[mglev];
+    double* tmp = nullptr;
+    HIPCHK(hipMallocAsync((void**)&tmp, 3 * lb.g.tot * sizeof(double), (hipStream_t)stream));
+    double* a = x;
+    double* c = tmp;
+    for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
+        launch_node<double>(h->sym, JACOBI, lb.g, h->bc, lb.C, a, b, c, h->d.omega, (hipStream_t)stream);
+        std::swap(a, c);
+    }
+    if (a != x) HIPCHK(hipMemcpyAsync(x, a, 3 * lb.g.tot * sizeof(double), hipMemcpyDeviceToDevice,
+                                      (hipStream_t)stream));
+    HIPCHK(hipFreeAsync(tmp, (hipStream_t)stream));
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_residual(alamo_mg* h, int amrlev, int mglev, double* r, const double* x, const double* b,
+                      void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    LevelBuf& lb = h->L[mglev];
+    launch_node<double>(h->sym, RESID, lb.g, h->bc, lb.C, x, b, r, 0.0, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_restrict(alamo_mg* h, int amrlev, int cmglev, double* crse, const double* fine, void* stream) {
+    TRY(check_level(h, amrlev, cmglev));
+    if (cmglev < 1) FAIL(ALAMO_ERR_ARG, "cmglev must be >= 1");
+    LevelBuf& lc = h->L[cmglev];
+    LevelBuf& lf = h->L[cmglev - 1];
+    k_restrict<double><<<nblocks(lc.g.tot), 256, 0, (hipStream_t)stream>>>(lc.g, lf.g, 3, fine, crse);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_interpolate(alamo_mg* h, int amrlev, int fmglev, double* fine, const double* crse, void* stream) {
+    TRY(check_level(h, amrlev, fmglev));
+    if (fmglev >= (int)h->L.size() - 1) FAIL(ALAMO_ERR_ARG, "fmglev must have a coarser level");
+    LevelBuf& lc = h->L[fmglev + 1];
+    LevelBuf& lf = h->L[fmglev];
+    k_interp_add<double><<<nblocks(lf.g.tot), 256, 0, (hipStream_t)stream>>>(lc.g, lf.g, crse, fine);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_norms(alamo_mg* h, int mglev, const double* x, double* norm_inf, double* norm_l2, void* stream) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
+    hipStream_t keep = h->stream;
+    h->stream = (hipStream_t)stream;
+    int rc = norms_dev<double>(h, x, 3 * h->L[mglev].g.tot, norm_inf, norm_l2);
+    h->stream = keep;
+    return rc;
+}
+
+int alamo_bc_init(alamo_mg* h, double* b, int homogeneous);
+int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs);
+
+int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       alamo_solve_stats* stats) {
+    if (!h || !sol || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    return solve_impl(h, sol, rhs, tol_rel, tol_abs, stats);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Boundary values and the Newton residual-form right-hand side.
+// ---------------------------------------------------------------------------
+struct BCVals {
+    double v[26][3];
+};
+
+__global__ void k_bc_init(Geom g, BCTable bc, BCVals vals, int homogeneous, double* __restrict__ b) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int f = classify(i, j, k, hi);
+    if (f == INT) return;
+    for (int c = 0; c < 3; c++) {
+        const int t = bc.t[f][c];
+        if (t == PERIODIC) continue;
+        b[c * g.tot + m] = (homogeneous && t == DISPLACEMENT) ? 0.0 : vals.v[f][c];
+    }
+}
+
+// DW = C : (grad u - F0) with GetStencil gradients (Newton.H:157-184)
+template <int SYM>
+__global__ void k_newton_dw(Geom g, const double* __restrict__ C, const double* __restrict__ F0,
+                            const double* __restrict__ u, double* __restrict__ DW) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int idx[3] = {i, j, k};
+    const int64_t st[3] = {1, g.sy, g.sz};
+    double G[3][3], S[3][3], c[NCoef<SYM>::value];
+    for (int p = 0; p < 3; p++)
+        for (int d = 0; d < 3; d++) {
+            G[p][d] = d1(u + p * g.tot, m, st[d], idx[d], hi[d], g.dx[d]);
+            if (F0) G[p][d] -= F0[9 * m + 3 * p + d];
+        }
+    load_coef<double, SYM>(C, g.tot, m, c);
+    c_dot<SYM>(c, G, S);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) DW[(3 * a + b) * g.tot + m] = S[a][b];
+}
+
+// rhs = b - div(DW) inside, b - BC(u, grad u, DW) on the boundary (Newton.H:199-231)
+__global__ void k_newton_rhs(Geom g, BCTable bc, const double* __restrict__ DW, const double* __restrict__ u,
+                             const double* __restrict__ b, double* __restrict__ rhs) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int idx[3] = {i, j, k};
+    const int64_t st[3] = {1, g.sy, g.sz};
+    double f[3];
+    if (i == 0 || j == 0 || k == 0 || i == hi[0] || j == hi[1] || k == hi[2]) {
+        double G[3][3], S[3][3], uu[3];
+        for (int p = 0; p < 3; p++) {
+            uu[p] = u[p * g.tot + m];
+            for (int d = 0; d < 3; d++) G[p][d] = d1(u + p * g.tot, m, st[d], idx[d], hi[d], g.dx[d]);
+        }
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) S[a][c] = DW[(3 * a + c) * g.tot + m];
+        bc_row(bc, classify(i, j, k, hi), uu, G, S, f);
+    } else {
+        for (int a = 0; a < 3; a++) {
+            double s = 0.0;
+            for (int d = 0; d < 3; d++) {
+                const double* P = DW + (3 * a + d) * g.tot;
+                s += (P[m + st[d]] - P[m - st[d]]) / 2. / g.dx[d];
+            }
+            f[a] = s;
+        }
+    }
+    for (int p = 0; p < 3; p++) rhs[p * g.tot + m] = b[p * g.tot + m] - f[p];
+}
+
+extern "C" int alamo_bc_init(alamo_mg* h, double* b, int homogeneous) {
+    if (!h || !b) FAIL(ALAMO_ERR_ARG, "null argument");
+    BCVals v;
+    for (int f = 0; f < 26; f++)
+        for (int c = 0; c < 3; c++) v.v[f][c] = h->d.bc_val[f][c];
+    const Geom& g = h->L[0].g;
+    k_bc_init<<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->bc, v, homogeneous, b);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+extern "C" int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs) {
+    if (!h || !u || !b || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    const Geom& g = h->L[0].g;
+    double* DW = nullptr;
+    HIPCHK(hipMallocAsync((void**)&DW, 9 * g.tot * sizeof(double), h->stream));
+    if (h->sym == ISO) k_newton_dw<ISO><<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->L[0].C, F0, u, DW);
+    else if (h->sym == MAJORMINOR)
+        k_newton_dw<MAJORMINOR><<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->L[0].C, F0, u, DW);
+    else k_newton_dw<DIAGONAL><<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->L[0].C, F0, u, DW);
+    k_newton_rhs<<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->bc, DW, u, b, rhs);
+    HIPCHK(hipFreeAsync(DW, h->stream));
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}

@@ -1,0 +1,394 @@
+// Node-parallel kernels: one work-item per node, any symmetry, FP64 or FP32.
+// They serve every multigrid level and are the reference the tiled kernels of
+// kernels_iso.h are checked against.
+#pragma once
+#include "node_ops.h"
+
+namespace alamo {
+
+enum Mode { APPLY = 0, RESID = 1, JACOBI = 2, DIAG = 3 };
+
+__device__ __forceinline__ void node_ijk(const Geom& g, int64_t m, int& i, int& j, int& k) {
+    k = (int)(m / g.sz);
+    int64_t r = m - (int64_t)k * g.sz;
+    j = (int)(r / g.sy);
+    i = (int)(r - (int64_t)j * g.sy);
+}
+
+// Row result of one node: Au (MODE APPLY/RESID/JACOBI) and/or the diagonal.
+template <typename T, int SYM, bool NEED_DIAG>
+__device__ __forceinline__ void node_row(const Geom& g, const BCTable& bc, const T* __restrict__ C,
+                                         const T* __restrict__ u, int i, int j, int k, T f[3], T D[3]) {
+    const bool bnd = i == 0 || j == 0 || k == 0 || i == g.n[0] || j == g.n[1] || k == g.n[2];
+    if (bnd) {
+        if (u) apply_boundary<T, SYM>(g, bc, C, u, i, j, k, f);
+        if (NEED_DIAG) diag_boundary<T, SYM>(g, bc, C, i, j, k, D);
+    } else {
+        const int64_t m = i + g.sy * j + g.sz * k;
+        T c[NCoef<SYM>::value];
+        if (u) apply_interior<T, SYM>(g, C, u, m, f, c);
+        else load_coef<T, SYM>(C, g.tot, m, c);
+        if (NEED_DIAG) diag_interior<T, SYM>(g, c, D);
+    }
+}
+
+// out = A u | b - A u | Jacobi(u) | D, over all nodes of the level.
+// Jacobi: x_new = (1-w) x + w (b - (A x - D x)) / D   (Operator/Operator.cpp:109-146)
+template <typename T, int SYM, int MODE>
+__global__ void __launch_bounds__(256) k_node(Geom g, BCTable bc, const T* __restrict__ C, const T* __restrict__ u,
+                                              const T* __restrict__ b, T* __restrict__ out, T omega) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    T f[3] = {T(0), T(0), T(0)}, D[3] = {T(1), T(1), T(1)};
+    node_row<T, SYM, MODE == JACOBI || MODE == DIAG>(g, bc, C, MODE == DIAG ? nullptr : u, i, j, k, f, D);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const int64_t q = p * g.tot + m;
+        if (MODE == APPLY) out[q] = f[p];
+        else if (MODE == RESID) out[q] = b[q] - f[p];
+        else if (MODE == DIAG) out[q] = D[p];
+        else {
+            const T x = u[q];
+            const T Rx = f[p] - D[p] * x;
+            out[q] = (T(1) - omega) * x + omega * (b[q] - Rx) / D[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: per-block partial (max |x|, sum x^2) or (sum a*b), wavefront
+// shuffles inside a 64-wide wavefront, LDS across the 4 wavefronts of a block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+constexpr int RED_BLOCK = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(RED_BLOCK) k_norm_partial(const T* __restrict__ x, int64_t n,
+                                                            double* __restrict__ partial) {
+    double mx = 0.0, ss = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)x[q];
+        // a NaN poisons the max so that a diverged solve cannot report a small norm
+        mx = (v != v) ? v : fmax(mx, fabs(v));
+        ss += v * v;
+    }
+    mx = wave_max(mx);
+    ss = wave_sum(ss);
+    __shared__ double smx[RED_BLOCK / 64], sss[RED_BLOCK / 64];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { smx[w] = mx; sss[w] = ss; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = smx[0], s = sss[0];
+        for (int q = 1; q < RED_BLOCK / 64; q++) { a = (smx[q] != smx[q]) ? smx[q] : fmax(a, smx[q]); s += sss[q]; }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = s;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_BLOCK) k_dot_partial(const T* __restrict__ a, const T* __restrict__ b,
+                                                           int64_t n, double* __restrict__ partial) {
+    double ss = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        ss += (double)a[q] * (double)b[q];
+    ss = wave_sum(ss);
+    __shared__ double sss[RED_BLOCK / 64];
+    if (threadIdx.x % 64 == 0) sss[threadIdx.x / 64] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < RED_BLOCK / 64; q++) s += sss[q];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// Final pass of the two-level norm reduction: out[0] = max, out[1] = sum of squares.
+__global__ void __launch_bounds__(RED_BLOCK) k_norm_final(const double* __restrict__ partial, int nb,
+                                                          double* __restrict__ out) {
+    double mx = 0.0, ss = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        const double a = partial[2 * q];
+        mx = (a != a) ? a : fmax(mx, a);
+        ss += partial[2 * q + 1];
+    }
+    mx = wave_max(mx);
+    ss = wave_sum(ss);
+    __shared__ double smx[RED_BLOCK / 64], sss[RED_BLOCK / 64];
+    if (threadIdx.x % 64 == 0) { smx[threadIdx.x / 64] = mx; sss[threadIdx.x / 64] = ss; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = smx[0], s = sss[0];
+        for (int q = 1; q < RED_BLOCK / 64; q++) { a = (smx[q] != smx[q]) ? smx[q] : fmax(a, smx[q]); s += sss[q]; }
+        out[0] = a;
+        out[1] = s;
+    }
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_sum_final(const double* __restrict__ partial, int nb,
+                                                         double* __restrict__ out) {
+    double ss = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) ss += partial[q];
+    ss = wave_sum(ss);
+    __shared__ double sss[RED_BLOCK / 64];
+    if (threadIdx.x % 64 == 0) sss[threadIdx.x / 64] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < RED_BLOCK / 64; q++) s += sss[q];
+        out[0] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid transfers.
+// ---------------------------------------------------------------------------
+// Full-weighting restriction, one work-item per coarse node, `np` SoA planes.
+// Operator/Operator.cpp:294-352 (residual) and Operator/Elastic.cpp:786-829 (stiffness).
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(Geom gc, Geom gf, int np, const T* __restrict__ fine,
+                                                  T* __restrict__ crse) {
+    const int64_t mc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (mc >= gc.tot) return;
+    int I, J, K;
+    node_ijk(gc, mc, I, J, K);
+    const bool bx = I == 0 || I == gc.n[0], by = J == 0 || J == gc.n[1], bz = K == 0 || K == gc.n[2];
+    const int64_t mf = 2 * I + gf.sy * (2 * J) + gf.sz * (2 * K);
+    const int64_t sx = 1, sy = gf.sy, sz = gf.sz;
+    for (int p = 0; p < np; p++) {
+        const T* F = fine + p * gf.tot + mf;
+        auto at = [&](int a, int b, int c) { return F[a * sx + b * sy + c * sz]; };
+        T v;
+        if (bx && by && bz) v = at(0, 0, 0);
+        else if (by && bz) v = T(0.25) * at(-1, 0, 0) + T(0.5) * at(0, 0, 0) + T(0.25) * at(1, 0, 0);
+        else if (bz && bx) v = T(0.25) * at(0, -1, 0) + T(0.5) * at(0, 0, 0) + T(0.25) * at(0, 1, 0);
+        else if (bx && by) v = T(0.25) * at(0, 0, -1) + T(0.5) * at(0, 0, 0) + T(0.25) * at(0, 0, 1);
+        else if (bx)
+            v = (at(0, -1, -1) + T(2) * at(0, 0, -1) + at(0, 1, -1) + T(2) * at(0, -1, 0) + T(4) * at(0, 0, 0) +
+                 T(2) * at(0, 1, 0) + at(0, -1, 1) + T(2) * at(0, 0, 1) + at(0, 1, 1)) / T(16);
+        else if (by)
+            v = (at(-1, 0, -1) + T(2) * at(-1, 0, 0) + at(-1, 0, 1) + T(2) * at(0, 0, -1) + T(4) * at(0, 0, 0) +
+                 T(2) * at(0, 0, 1) + at(1, 0, -1) + T(2) * at(1, 0, 0) + at(1, 0, 1)) / T(16);
+        else if (bz)
+            v = (at(-1, -1, 0) + T(2) * at(0, -1, 0) + at(1, -1, 0) + T(2) * at(-1, 0, 0) + T(4) * at(0, 0, 0) +
+                 T(2) * at(1, 0, 0) + at(-1, 1, 0) + T(2) * at(0, 1, 0) + at(1, 1, 0)) / T(16);
+        else {
+            T c8 = T(0), c12 = T(0), c6 = T(0);
+#pragma unroll
+            for (int a = -1; a <= 1; a++)
+#pragma unroll
+                for (int b = -1; b <= 1; b++)
+#pragma unroll
+                    for (int c = -1; c <= 1; c++) {
+                        const int nz = (a != 0) + (b != 0) + (c != 0);
+                        if (nz == 3) c8 += at(a, b, c);
+                        else if (nz == 2) c12 += at(a, b, c);
+                        else if (nz == 1) c6 += at(a, b, c);
+                    }
+            v = c8 / T(64) + c12 / T(32) + c6 / T(16) + at(0, 0, 0) / T(8);
+        }
+        crse[p * gc.tot + mc] = v;
+    }
+}
+
+// fine += I(crse), trilinear nodal prolongation by parity (Operator/Operator.cpp:399-428).
+template <typename T>
+__global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* __restrict__ crse,
+                                                    T* __restrict__ fine) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= gf.tot) return;
+    int i, j, k;
+    node_ijk(gf, m, i, j, k);
+    const int I = i / 2, J = j / 2, K = k / 2;
+    const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
+    const int64_t mc = I + gc.sy * J + gc.sz * K;
+    const int64_t sx = 1, sy = gc.sy, sz = gc.sz;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* Cp = crse + p * gc.tot + mc;
+        auto at = [&](int a, int b, int c) { return Cp[a * sx + b * sy + c * sz]; };
+        T v;
+        if (ei && ej && ek) v = at(0, 0, 0);
+        else if (ej && ek) v = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
+        else if (ek && ei) v = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
+        else if (ei && ej) v = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
+        else if (ei) v = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
+        else if (ej) v = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
+        else if (ek) v = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
+        else
+            v = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
+                            at(1, 1, 0) + at(1, 1, 1));
+        fine[p * gf.tot + m] += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, int64_t n, T v) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        x[q] = v;
+}
+template <typename Ti, typename To>
+__global__ void k_convert(const Ti* __restrict__ a, To* __restrict__ b, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        b[q] = (To)a[q];
+}
+// y += alpha * x   (x may be a different precision)
+template <typename Tx, typename Ty>
+__global__ void k_axpy(Ty* __restrict__ y, const Tx* __restrict__ x, int64_t n, double alpha) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        y[q] = (Ty)((double)y[q] + alpha * (double)x[q]);
+}
+// y = a + b
+template <typename T>
+__global__ void k_add(T* __restrict__ y, const T* __restrict__ a, const T* __restrict__ b, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        y[q] = a[q] + b[q];
+}
+// AoS (ncoef per node) -> SoA planes
+template <typename T>
+__global__ void k_aos_to_soa(const double* __restrict__ aos, T* __restrict__ soa, int64_t tot, int nc) {
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < tot; m += (int64_t)gridDim.x * blockDim.x)
+        for (int q = 0; q < nc; q++) soa[q * tot + m] = (T)aos[m * nc + q];
+}
+
+// ---------------------------------------------------------------------------
+// Bottom solve: Jacobi-preconditioned BiCGStab (the amrex::MLCGSolver choice the
+// reference forces, Solver/Nonlocal/Linear.H:258, preconditioned by
+// Operator<Node>::normalize, Operator/Operator.cpp:156-165), run entirely by one
+// workgroup on the coarsest level.  Work vectors live in `w` (8 x 3*tot).
+// Global-memory phases are separated by device-scope fences + barriers.
+// ---------------------------------------------------------------------------
+constexpr int BOTTOM_BLOCK = 1024;
+
+template <typename T, int SYM>
+__device__ void bottom_apply(const Geom& g, const BCTable& bc, const T* C, const T* x, T* y) {
+    for (int64_t m = threadIdx.x; m < g.tot; m += blockDim.x) {
+        int i, j, k;
+        node_ijk(g, m, i, j, k);
+        T f[3], D[3];
+        node_row<T, SYM, false>(g, bc, C, x, i, j, k, f, D);
+        for (int p = 0; p < 3; p++) y[p * g.tot + m] = f[p];
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64;
+    __syncthreads();
+    if (threadIdx.x % 64 == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / 64); q++) s += sh[q];
+    return s;
+}
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int w = threadIdx.x / 64;
+    __syncthreads();
+    if (threadIdx.x % 64 == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / 64); q++) s = (sh[q] != sh[q]) ? sh[q] : fmax(s, sh[q]);
+    return s;
+}
+
+__device__ __forceinline__ void phase_sync() {
+    __threadfence();
+    __syncthreads();
+}
+
+template <typename T, int SYM>
+__global__ void __launch_bounds__(BOTTOM_BLOCK) k_bottom_bicgstab(Geom g, BCTable bc, const T* __restrict__ C,
+                                                                   const T* __restrict__ Dg, T* x,
+                                                                   const T* __restrict__ b, T* w, double rtol,
+                                                                   int maxit, int* iters) {
+    __shared__ double sh[BOTTOM_BLOCK / 64];
+    const int64_t n = 3 * g.tot;
+    T *r = w, *rh = w + n, *p = w + 2 * n, *v = w + 3 * n, *s = w + 4 * n, *t = w + 5 * n, *ph = w + 6 * n,
+      *sh_ = w + 7 * n;
+    bottom_apply<T, SYM>(g, bc, C, x, r);
+    phase_sync();
+    double r0 = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+        r[q] = b[q] - r[q];
+        rh[q] = r[q];
+        p[q] = T(0);
+        v[q] = T(0);
+        r0 = fmax(r0, fabs((double)r[q]));
+    }
+    r0 = block_max(r0, sh);
+    phase_sync();
+    int it = 0;
+    if (r0 > 0.0) {
+        double rho_1 = 0.0, alpha = 0.0, omega = 0.0;
+        for (it = 1; it <= maxit; it++) {
+            double rho = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) rho += (double)rh[q] * (double)r[q];
+            rho = block_sum(rho, sh);
+            if (rho == 0.0) break;
+            const double beta = (it == 1) ? 0.0 : (rho / rho_1) * (alpha / omega);
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+                const T pn = (it == 1) ? r[q] : T(r[q] + beta * (p[q] - omega * v[q]));
+                p[q] = pn;
+                ph[q] = pn / Dg[q];
+            }
+            phase_sync();
+            bottom_apply<T, SYM>(g, bc, C, ph, v);
+            phase_sync();
+            double rv = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) rv += (double)rh[q] * (double)v[q];
+            rv = block_sum(rv, sh);
+            if (rv == 0.0) break;
+            alpha = rho / rv;
+            double smax = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+                x[q] = T(x[q] + alpha * ph[q]);
+                s[q] = T(r[q] - alpha * v[q]);
+                sh_[q] = s[q] / Dg[q];
+                smax = fmax(smax, fabs((double)s[q]));
+            }
+            smax = block_max(smax, sh);
+            if (smax <= rtol * r0) break;
+            phase_sync();
+            bottom_apply<T, SYM>(g, bc, C, sh_, t);
+            phase_sync();
+            double tt = 0.0, ts = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+                tt += (double)t[q] * (double)t[q];
+                ts += (double)t[q] * (double)s[q];
+            }
+            tt = block_sum(tt, sh);
+            ts = block_sum(ts, sh);
+            if (tt == 0.0) break;
+            omega = ts / tt;
+            double rmax = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+                x[q] = T(x[q] + omega * sh_[q]);
+                r[q] = T(s[q] - omega * t[q]);
+                rmax = fmax(rmax, fabs((double)r[q]));
+            }
+            rmax = block_max(rmax, sh);
+            phase_sync();
+            if (rmax <= rtol * r0 || omega == 0.0) break;
+            rho_1 = rho;
+        }
+    }
+    if (threadIdx.x == 0 && iters) *iters = it;
+}
+
+}  // namespace alamo

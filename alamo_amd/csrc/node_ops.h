@@ -1,0 +1,173 @@
+// Per-node arithmetic of Operator::Elastic<SYM> shared by every kernel.
+//
+//   interior rows   f_i = C_ijkl u_k,lj + (dC_idkl/dx_d) u_k,l      (Operator/Elastic.cpp:204-254)
+//   boundary rows   f   = BC(u, grad u, C : grad u)                 (Operator/Elastic.cpp:195-203,
+//                                                                   BC/Operator/Elastic/Constant.H:344-425)
+//   diagonal        Operator/Elastic.cpp:296-379 with Numeric/Stencil.H:906-963
+//
+// psi is unset (psi_avg = 1) and m_uniform is false, the configuration
+// Integrator/Base/Mechanics.H:193-198 builds.
+#pragma once
+#include "common.h"
+
+namespace alamo {
+
+template <typename T, int SYM>
+__device__ __forceinline__ void load_coef(const T* __restrict__ C, int64_t tot, int64_t m, T* c) {
+#pragma unroll
+    for (int q = 0; q < NCoef<SYM>::value; q++) c[q] = C[q * tot + m];
+}
+
+// First derivative with Numeric::GetStencil semantics (Numeric/Stencil.H:41-60, 70-191).
+template <typename T>
+__device__ __forceinline__ T d1(const T* __restrict__ f, int64_t m, int64_t s, int idx, int hi, T dx) {
+    if (idx == 0) return (f[m + s] - f[m]) / dx;
+    if (idx == hi) return (f[m] - f[m - s]) / dx;
+    return (f[m + s] - f[m - s]) * T(0.5) / dx;
+}
+
+template <typename T>
+__device__ __forceinline__ void bc_row(const BCTable& bc, int face, const T u[3], const T G[3][3], const T S[3][3],
+                                       T f[3]) {
+    double nd[3];
+    region_normal(face, nd);
+    const T n[3] = {T(nd[0]), T(nd[1]), T(nd[2])};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int t = bc.t[face][c];
+        T v = T(0);
+        if (t == DISPLACEMENT) v = u[c];
+        else if (t == TRACTION) v = S[c][0] * n[0] + S[c][1] * n[1] + S[c][2] * n[2];
+        else if (t == NEUMANN) v = G[c][0] * n[0] + G[c][1] * n[1] + G[c][2] * n[2];
+        f[c] = v;
+    }
+}
+
+// (A u) at a boundary node.
+template <typename T, int SYM>
+__device__ __forceinline__ void apply_boundary(const Geom& g, const BCTable& bc, const T* __restrict__ C,
+                                               const T* __restrict__ u, int i, int j, int k, T f[3]) {
+    const int64_t m = i + g.sy * j + g.sz * k;
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int idx[3] = {i, j, k};
+    const int64_t st[3] = {1, g.sy, g.sz};
+    T G[3][3], S[3][3], uu[3], c[NCoef<SYM>::value];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* up = u + p * g.tot;
+        uu[p] = up[m];
+#pragma unroll
+        for (int d = 0; d < 3; d++) G[p][d] = d1(up, m, st[d], idx[d], hi[d], T(g.dx[d]));
+    }
+    load_coef<T, SYM>(C, g.tot, m, c);
+    c_dot<SYM>(c, G, S);
+    bc_row(bc, classify(i, j, k, hi), uu, G, S, f);
+}
+
+// Diagonal of the boundary row for each component (Elastic.cpp:352-360 with
+// Gradient_Diagonal<Set::Matrix>, Stencil.H:906-930).
+template <typename T, int SYM>
+__device__ __forceinline__ void diag_boundary(const Geom& g, const BCTable& bc, const T* __restrict__ C, int i, int j,
+                                              int k, T D[3]) {
+    const int64_t m = i + g.sy * j + g.sz * k;
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int idx[3] = {i, j, k};
+    T c[NCoef<SYM>::value];
+    load_coef<T, SYM>(C, g.tot, m, c);
+    T cd[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        bool mn = idx[d] == 0, mx = idx[d] == hi[d];
+        cd[d] = (T(mx ? 1 : 0) - T(mn ? 1 : 0)) / (T(mn || mx ? 1 : 2) * T(g.dx[d]));
+    }
+    const int face = classify(i, j, k, hi);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        T G[3][3], S[3][3], uu[3] = {T(0), T(0), T(0)}, f[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) G[a][d] = (a == p) ? cd[d] : T(0);
+        c_dot<SYM>(c, G, S);
+        uu[p] = T(1);
+        bc_row(bc, face, uu, G, S, f);
+        D[p] = f[p];
+    }
+}
+
+// Interior diagonal D_p = sum_j C_pjpj (-2/dx_j^2)   (Elastic.cpp:361-365, Stencil.H:932-963)
+template <typename T, int SYM>
+__device__ __forceinline__ void diag_interior(const Geom& g, const T* c, T D[3]) {
+    const T w[3] = {T(-2.0 / (g.dx[0] * g.dx[0])), T(-2.0 / (g.dx[1] * g.dx[1])), T(-2.0 / (g.dx[2] * g.dx[2]))};
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        T s = T(0);
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++) s += C4<SYM>(c, p, jj, p, jj) * w[jj];
+        D[p] = s;
+    }
+}
+
+// (A u) at an interior node, generic form (any SYM), direct global loads.
+template <typename T, int SYM>
+__device__ __forceinline__ void apply_interior(const Geom& g, const T* __restrict__ C, const T* __restrict__ u,
+                                               int64_t m, T f[3], T* cout /* may be null: receives C(m) */) {
+    constexpr int NC = NCoef<SYM>::value;
+    const int64_t st[3] = {1, g.sy, g.sz};
+    const T dx[3] = {T(g.dx[0]), T(g.dx[1]), T(g.dx[2])};
+    T gg[3][3][3], G[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* up = u + p * g.tot;
+        const T u0 = up[m];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T up1 = up[m + st[a]], um1 = up[m - st[a]];
+            gg[p][a][a] = (up1 - T(2) * u0 + um1) / dx[a] / dx[a];
+            G[p][a] = (up1 - um1) * T(0.5) / dx[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = a + 1; b < 3; b++) {
+                const T v = T(0.25) * (up[m + st[a] + st[b]] + up[m - st[a] - st[b]] - up[m + st[a] - st[b]] -
+                                       up[m - st[a] + st[b]]) / (dx[a] * dx[b]);
+                gg[p][a][b] = v;
+                gg[p][b][a] = v;
+            }
+    }
+    T c[NC];
+    load_coef<T, SYM>(C, g.tot, m, c);
+    if (cout) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) cout[q] = c[q];
+    }
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+        T s = T(0);
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+                for (int ll = 0; ll < 3; ll++) s += C4<SYM>(c, ii, jj, kk, ll) * gg[kk][ll][jj];
+        f[ii] = s;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        T dc[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) dc[q] = (C[q * g.tot + m + st[d]] - C[q * g.tot + m - st[d]]) * T(0.5) / dx[d];
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) {
+            T s = T(0);
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+                for (int ll = 0; ll < 3; ll++) s += C4<SYM>(dc, ii, d, kk, ll) * G[kk][ll];
+            f[ii] += s;
+        }
+    }
+}
+
+}  // namespace alamo

@@ -1,0 +1,182 @@
+"""Host-side mirror of the reference's operator/solver plugin surface over the C ABI.
+
+Names, argument meaning and error behaviour follow
+
+* ``Operator::Elastic<SYM>``       (src/Operator/Elastic.H:21-190, Elastic.cpp)
+* ``Operator::Operator<Grid::Node>`` (src/Operator/Operator.H:27-125)
+* ``Solver::Nonlocal::Linear``      (src/Solver/Nonlocal/Linear.H:18-389)
+
+Device fields are torch tensors (float64, on the GPU) used only as memory: every
+computation runs in libalamo_amd.so.  A reference ``Util::Abort`` becomes an
+``AlamoError`` exception.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import MGDesc, SolveStats, check, lib
+from .problems import ISO, MAJORMINOR, DIAGONAL, NCOEF
+
+LAYOUT_AOS, LAYOUT_SOA = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+CYCLE_V, CYCLE_F = 0, 1
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Elastic:
+    """Operator::Elastic<SYM> on one box covering a non-periodic nodal domain.
+
+    ``bc`` is the pair (bc_type[26][3], bc_val[26][3]) of BC::Operator::Elastic::Constant."""
+
+    def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
+                 post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
+                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0):
+        d = MGDesc()
+        check(lib().alamo_mg_desc_default(ctypes.byref(d)))
+        for q in range(3):
+            d.n_cell[q] = int(ncell[q])
+            d.dx[q] = float(dx[q])
+        d.sym = int(sym)
+        if bc is not None:
+            t, v = bc
+            t = np.asarray(t, dtype=np.int32).reshape(26, 3)
+            v = np.asarray(v, dtype=np.float64).reshape(26, 3)
+            for f in range(26):
+                for c in range(3):
+                    d.bc_type[f][c] = int(t[f, c])
+                    d.bc_val[f][c] = float(v[f, c])
+        d.precision, d.cycle = precision, cycle
+        d.pre_smooth, d.post_smooth, d.sweeps_per_smooth = pre_smooth, post_smooth, sweeps_per_smooth
+        d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
+        d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
+        self.desc = d
+        self.sym = sym
+        self.ncell = tuple(int(n) for n in ncell)
+        self.dx = tuple(float(x) for x in dx)
+        h = ctypes.c_void_p()
+        check(lib().alamo_mg_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                lib().alamo_mg_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # --- hierarchy -------------------------------------------------------------
+    def NMGLevels(self):
+        n = ctypes.c_int()
+        check(lib().alamo_mg_num_levels(self.h, ctypes.byref(n)))
+        return n.value
+
+    def LevelNCell(self, mglev):
+        a = (ctypes.c_int * 3)()
+        check(lib().alamo_mg_level_ncell(self.h, mglev, a))
+        return tuple(a)
+
+    def LevelNodes(self, mglev):
+        n = self.LevelNCell(mglev)
+        return (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
+
+    # --- coefficients ----------------------------------------------------------
+    def SetModel(self, C, layout=LAYOUT_AOS, amrlev=0):
+        """Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116); C as numpy (host) or tensor (device)."""
+        if isinstance(C, np.ndarray):
+            C = np.ascontiguousarray(C, dtype=np.float64)
+            expect = NCOEF[self.sym] * self.LevelNodes(0)
+            if C.size != expect:
+                raise _lib.AlamoError(1, "Inconsistent # of components: got %d values, need %d" % (C.size, expect))
+            check(lib().alamo_mg_set_model_host(self.h, amrlev, C.ctypes.data_as(ctypes.c_void_p), layout))
+        else:
+            check(lib().alamo_mg_set_model(self.h, amrlev, _ptr(C), layout))
+
+    def SetModelGrains(self, gid, table):
+        gid = np.ascontiguousarray(gid, dtype=np.uint16).ravel()
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        check(lib().alamo_mg_set_model_grains_host(self.h, gid.ctypes.data_as(ctypes.c_void_p),
+                                                   table.ctypes.data_as(ctypes.c_void_p), table.shape[0]))
+
+    def prepareForSolve(self):
+        check(lib().alamo_mg_prepare(self.h))
+
+    def GetCoeff(self, mglev, out):
+        check(lib().alamo_mg_get_coeff(self.h, mglev, _ptr(out)))
+
+    # --- operator virtuals -----------------------------------------------------
+    def Fapply(self, amrlev, mglev, out, inp, stream=None):
+        check(lib().alamo_mg_fapply(self.h, amrlev, mglev, _ptr(out), _ptr(inp), stream))
+
+    def Apply(self, amrlev, mglev, out, inp):
+        self.Fapply(amrlev, mglev, out, inp)
+
+    def Diagonal(self, amrlev, mglev, diag, stream=None):
+        check(lib().alamo_mg_diagonal(self.h, amrlev, mglev, _ptr(diag), stream))
+
+    def Fsmooth(self, amrlev, mglev, x, b, stream=None):
+        check(lib().alamo_mg_fsmooth(self.h, amrlev, mglev, _ptr(x), _ptr(b), stream))
+
+    def correctionResidual(self, amrlev, mglev, resid, x, b, stream=None):
+        check(lib().alamo_mg_residual(self.h, amrlev, mglev, _ptr(resid), _ptr(x), _ptr(b), stream))
+
+    def solutionResidual(self, amrlev, resid, x, b, stream=None):
+        self.correctionResidual(amrlev, 0, resid, x, b, stream)
+
+    def restriction(self, amrlev, cmglev, crse, fine, stream=None):
+        check(lib().alamo_mg_restrict(self.h, amrlev, cmglev, _ptr(crse), _ptr(fine), stream))
+
+    def interpolation(self, amrlev, fmglev, fine, crse, stream=None):
+        check(lib().alamo_mg_interpolate(self.h, amrlev, fmglev, _ptr(fine), _ptr(crse), stream))
+
+    def Norms(self, mglev, x, stream=None):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().alamo_mg_norms(self.h, mglev, _ptr(x), ctypes.byref(a), ctypes.byref(b), stream))
+        return a.value, b.value
+
+    # --- right-hand sides ------------------------------------------------------
+    def BCInit(self, rhs, homogeneous=False):
+        """BC::Operator::Elastic::Constant::Init (Constant.H:197-286)."""
+        check(lib().alamo_bc_init(self.h, _ptr(rhs), int(homogeneous)))
+
+    def NewtonRHS(self, u, b, F0, rhs):
+        """Newton<T>::prepareForSolve residual form (Newton.H:144-231)."""
+        check(lib().alamo_newton_rhs(self.h, _ptr(u), _ptr(b), _ptr(F0), _ptr(rhs)))
+
+
+class Linear:
+    """Solver::Nonlocal::Linear (Linear.H:18-389)."""
+
+    def __init__(self, op=None):
+        self.linop = None
+        if op is not None:
+            self.Define(op)
+
+    def Define(self, op):
+        if self.linop is not None:
+            raise _lib.AlamoError(3, "Solver cannot be re-defined")
+        self.linop = op
+
+    def Clear(self):
+        if self.linop is None:
+            raise _lib.AlamoError(3, "Solver cannot be cleared if not defined")
+        self.linop = None
+
+    def solve(self, sol, rhs, tol_rel, tol_abs, raise_on_fail=True):
+        """Returns the stats dict; sol holds the initial guess on entry (Linear.H:131-147)."""
+        if self.linop is None:
+            raise _lib.AlamoError(3, "Solver not defined")
+        st = SolveStats()
+        rc = lib().alamo_linear_solve(self.linop.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(st))
+        if rc != 0 and raise_on_fail:
+            check(rc)
+        out = st.as_dict()
+        out["rc"] = rc
+        return out

@@ -1,0 +1,139 @@
+/* C ABI of the MI355X nodal elastic multigrid library (libalamo_amd.so).
+ *
+ * This is the drop-in boundary beneath Integrator::Mechanics: each entry point
+ * replaces one member of the reference's operator/solver plugin surface, cited
+ * as file:line into the alamo source tree.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - A handle owns one multigrid hierarchy of a single-box, non-periodic nodal
+ *     domain (amrlev 0 only).  mglev 0 is the finest level.
+ *   - Node (i,j,k) of a level with n[d] cells sits at flat index
+ *     i + (n0+1)*(j + (n1+1)*k); vector fields are 3 SoA planes of that size
+ *     (component slowest, the amrex::Array4 order).
+ *   - Pointers passed to entry points without a `_host` suffix are device
+ *     pointers (HBM); `stream` is a hipStream_t (NULL = default stream).
+ *   - Every function returns 0 on success and a nonzero ALAMO_ERR_* code on
+ *     failure; alamo_last_error() gives the message.  No exceptions cross the ABI.
+ *   - One handle per host thread; calls are stream-ordered.
+ */
+#ifndef ALAMO_AMD_H
+#define ALAMO_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Set::Sym subset (src/Set/Base.H) */
+enum { ALAMO_SYM_ISOTROPIC = 0, ALAMO_SYM_MAJORMINOR = 1, ALAMO_SYM_DIAGONAL = 2 };
+/* BC::Operator::Elastic::Elastic::Type, same numbering (src/BC/Operator/Elastic/Elastic.H:26) */
+enum { ALAMO_BC_DISPLACEMENT = 0, ALAMO_BC_TRACTION = 1, ALAMO_BC_PERIODIC = 2, ALAMO_BC_NEUMANN = 3,
+       ALAMO_BC_NONE = 4 };
+/* stiffness layouts accepted by alamo_mg_set_model */
+enum { ALAMO_LAYOUT_AOS = 0, /* BaseFab<Set::Matrix4<3,SYM>>: ncoef doubles per node */
+       ALAMO_LAYOUT_SOA = 1  /* ncoef planes of (n0+1)(n1+1)(n2+1) doubles */ };
+enum { ALAMO_PREC_FP64 = 0, ALAMO_PREC_MIXED = 1 /* FP32 cycles inside FP64 iterative refinement */ };
+enum { ALAMO_CYCLE_V = 0, ALAMO_CYCLE_F = 1 };
+enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, ALAMO_ERR_NOCONV = 4,
+       ALAMO_ERR_NAN = 5 };
+
+typedef struct alamo_mg alamo_mg;
+
+/* Geometry, boundary table and solver options of one hierarchy.  Field names
+ * follow the reference's input keys (solver.* of Solver/Nonlocal/Linear.H:296-382,
+ * bc.constant.* of BC/Operator/Elastic/Constant.H:439-648). */
+typedef struct {
+    int n_cell[3];             /* cells per direction at mglev 0 (amr.n_cell) */
+    double dx[3];              /* cell size (Geometry::CellSize) */
+    int sym;                   /* ALAMO_SYM_* */
+    int bc_type[26][3];        /* per Face (Elastic.H:35-43) and component, ALAMO_BC_* */
+    double bc_val[26][3];      /* prescribed values, written into b by alamo_bc_init */
+    int max_coarsening_level;  /* -1: coarsen while every direction keeps >= 2 cells */
+    int precision;             /* ALAMO_PREC_* */
+    int cycle;                 /* ALAMO_CYCLE_*; the reference runs F-cycles (Linear.H:261) */
+    int pre_smooth, post_smooth; /* Fsmooth calls per level (solver.pre_smooth/post_smooth) */
+    int sweeps_per_smooth;     /* Jacobi sweeps per Fsmooth call (reference: 2, Operator.cpp:107) */
+    double omega;              /* damping (solver.omega; Operator.H:123 default 2/3) */
+    int max_iter;              /* solver.max_iter */
+    int bottom_max_iter;       /* solver.bottom_max_iter */
+    double bottom_tol_rel;     /* solver.bottom_tol_rel */
+    int verbose;
+} alamo_mg_desc;
+
+typedef struct {
+    int iterations;            /* multigrid (or refinement) iterations performed */
+    int converged;             /* 1 if |b - A x|_inf <= target */
+    double initial_res_inf, final_res_inf, final_res_l2, rhs_inf, target;
+    double seconds;            /* device-synchronised wall time of the solve */
+    int fallback_events;       /* learned solve: plain V-cycle fallbacks taken */
+} alamo_solve_stats;
+
+const char* alamo_last_error(void);
+int alamo_version(void);
+
+/* Fill `d` with the reference defaults (Dirichlet-0 on all 26 regions, F-cycle,
+ * 2 pre / 2 post smooths of 2 sweeps, omega 2/3, max_iter 200, bottom 200 / 1e-4). */
+int alamo_mg_desc_default(alamo_mg_desc* d);
+
+/* Operator::Elastic<SYM>::Elastic(geom, grids, dmap, info) + define()
+ * (src/Operator/Elastic.cpp:9-55) together with Solver::Nonlocal::Linear::Define
+ * (src/Solver/Nonlocal/Linear.H:36-43): allocates the hierarchy on the current device. */
+int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** h);
+/* ~Elastic / Linear::Clear (Linear.H:44-51) */
+int alamo_mg_destroy(alamo_mg* h);
+int alamo_mg_set_stream(alamo_mg* h, void* stream);
+int alamo_mg_num_levels(alamo_mg* h, int* nlev);
+int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3);
+
+/* Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116): copy the per-node stiffness of
+ * mglev 0.  `C` is a device pointer (alamo_mg_set_model_host: host pointer). */
+int alamo_mg_set_model(alamo_mg* h, int amrlev, const double* C, int layout);
+int alamo_mg_set_model_host(alamo_mg* h, int amrlev, const double* C, int layout);
+/* Polycrystal fast path: per-node grain id (uint16) and a grain table of ngrains
+ * 21-slot MajorMinor records (host pointers).  Same operator as SetModel with
+ * C(node) = table[gid[node]]; the finest level then reads 2 bytes per node of stiffness. */
+int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const double* table, int ngrains);
+
+/* Operator<Grid::Node>::prepareForSolve (Operator.cpp:253-260): coefficient restriction
+ * to every mglev (Elastic::averageDownCoeffs, Elastic.cpp:605-626, 746-836) and the
+ * diagonal (Operator::Diagonal(true)). */
+int alamo_mg_prepare(alamo_mg* h);
+
+/* Copy the stiffness of level mglev out (SoA planes, device pointer). */
+int alamo_mg_get_coeff(alamo_mg* h, int mglev, double* C_soa);
+
+/* Elastic::Fapply (Elastic.cpp:139-292): out = A in on level mglev. */
+int alamo_mg_fapply(alamo_mg* h, int amrlev, int mglev, double* out, const double* in, void* stream);
+/* Elastic::Diagonal (Elastic.cpp:296-379). */
+int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* stream);
+/* Operator<Grid::Node>::Fsmooth (Operator.cpp:88-154): sweeps_per_smooth damped-Jacobi sweeps. */
+int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream);
+/* solutionResidual / correctionResidual (Operator.cpp:635-657): r = b - A x. */
+int alamo_mg_residual(alamo_mg* h, int amrlev, int mglev, double* r, const double* x, const double* b,
+                      void* stream);
+/* Operator<Grid::Node>::restriction (Operator.cpp:262-363): crse (level cmglev) = R fine. */
+int alamo_mg_restrict(alamo_mg* h, int amrlev, int cmglev, double* crse, const double* fine, void* stream);
+/* Operator<Grid::Node>::interpolation (Operator.cpp:365-434): fine (level fmglev) += P crse. */
+int alamo_mg_interpolate(alamo_mg* h, int amrlev, int fmglev, double* fine, const double* crse, void* stream);
+/* max-norm and L2 norm of a level vector (MLMG's ResNormInf; 3 components). Synchronises. */
+int alamo_mg_norms(alamo_mg* h, int mglev, const double* x, double* norm_inf, double* norm_l2, void* stream);
+
+/* BC::Operator::Elastic::Constant::Init (BC/Operator/Elastic/Constant.H:197-286):
+ * write the prescribed boundary values into b (device pointer, level 0). */
+int alamo_bc_init(alamo_mg* h, double* b, int homogeneous);
+/* Newton<T>::prepareForSolve residual form (Solver/Nonlocal/Newton.H:144-231) for a
+ * linear/affine model DW = C:(grad u - F0): rhs = b - div(DW) inside, b - BC(u, grad u, DW)
+ * on the boundary.  F0 may be NULL (9 doubles per node, row-major, device pointer). */
+int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs);
+
+/* Solver::Nonlocal::Linear::solve (Linear.H:131-147) -> amrex::MLMG::solve: sol is the
+ * initial guess on entry and the solution on exit (device pointers, level 0).  Stops when
+ * |rhs - A sol|_inf <= max(tol_abs, tol_rel * max(|rhs|_inf, |r0|_inf)).  Returns
+ * ALAMO_ERR_NOCONV if max_iter is reached (stats still filled). */
+int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       alamo_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+"""ctypes view of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg as the checker.  The product package never imports
+this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+
+# Face enum of BC/Operator/Elastic/Elastic.H:35-43
+FACES = ["XLO", "YLO", "ZLO", "XHI", "YHI", "ZHI",
+         "YLO_ZLO", "YLO_ZHI", "YHI_ZLO", "YHI_ZHI",
+         "ZLO_XLO", "ZLO_XHI", "ZHI_XLO", "ZHI_XHI",
+         "XLO_YLO", "XLO_YHI", "XHI_YLO", "XHI_YHI",
+         "XLO_YLO_ZLO", "XLO_YLO_ZHI", "XLO_YHI_ZLO", "XLO_YHI_ZHI",
+         "XHI_YLO_ZLO", "XHI_YLO_ZHI", "XHI_YHI_ZLO", "XHI_YHI_ZHI", "INT"]
+FACE = {n: i for i, n in enumerate(FACES)}
+DISPLACEMENT, TRACTION, PERIODIC, NEUMANN, NONE = range(5)
+ISO, MAJORMINOR, DIAGONAL = range(3)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(D)
+
+
+def _i(a):
+    return a.ctypes.data_as(I)
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def ncoef(sym):
+    return {ISO: 2, MAJORMINOR: 21, DIAGONAL: 9}[sym]
+
+
+def nodes(ncell):
+    return tuple(int(n) + 1 for n in ncell)
+
+
+def fapply(ncell, dx, sym, C, bct, u):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); u = _c(u)
+    f = np.zeros_like(u)
+    lib().ora_fapply(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(u), _d(f))
+    return f
+
+
+def diagonal(ncell, dx, sym, C, bct):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    N = np.prod(nodes(ncell))
+    Dg = np.zeros(3 * N)
+    lib().ora_diagonal(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(Dg))
+    return Dg
+
+
+def jacobi(ncell, dx, sym, C, bct, Dg, omega, sweeps, x, b):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    Dg = _c(Dg); x = _c(x).copy(); b = _c(b)
+    lib().ora_jacobi(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(Dg), ctypes.c_double(omega), sweeps, _d(x), _d(b))
+    return x
+
+
+def residual(ncell, dx, sym, C, bct, x, b):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); x = _c(x); b = _c(b)
+    r = np.zeros_like(b)
+    lib().ora_residual(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(x), _d(b), _d(r))
+    return r
+
+
+def restrict(ncell_fine, dx_fine, fine, w=1, planes=3):
+    ncell_fine = _c(ncell_fine, np.int32); dx_fine = _c(dx_fine); fine = _c(fine)
+    nc = tuple(int(n) // 2 + 1 for n in ncell_fine)
+    N = int(np.prod(nc))
+    out = np.zeros((planes if planes > 0 else 1) * N * w)
+    lib().ora_restrict(_i(ncell_fine), _d(dx_fine), w, planes, _d(fine), _d(out))
+    return out
+
+
+def interp_add(ncell_fine, dx_fine, crse, fine):
+    ncell_fine = _c(ncell_fine, np.int32); dx_fine = _c(dx_fine); crse = _c(crse); fine = _c(fine).copy()
+    lib().ora_interp_add(_i(ncell_fine), _d(dx_fine), _d(crse), _d(fine))
+    return fine
+
+
+def mg_solve(ncell, dx, sym, C, bct, b, x0=None, tol_rel=1e-10, tol_abs=0.0, cycle=1, nu1=2, nu2=2, sweeps=2,
+             omega=2.0 / 3.0, max_iter=200, max_coarsening=-1):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); b = _c(b)
+    x = np.zeros_like(b) if x0 is None else _c(x0).copy()
+    hist = np.zeros(max_iter + 1)
+    fr = ctypes.c_double(0)
+    nl = ctypes.c_int(0)
+    it = lib().ora_mg_solve(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(x), _d(b), ctypes.c_double(tol_rel),
+                            ctypes.c_double(tol_abs), cycle, nu1, nu2, sweeps, ctypes.c_double(omega), max_iter,
+                            max_coarsening, _d(hist), max_iter + 1, ctypes.byref(fr), ctypes.byref(nl))
+    return x, it, hist[:abs(it) + 1], fr.value, nl.value
+
+
+def coeff_level(ncell, dx, sym, C, level):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C)
+    nc = tuple(int(n) // (2 ** level) + 1 for n in ncell)
+    out = np.zeros(int(np.prod(nc)) * ncoef(sym))
+    lib().ora_coeff_level(_i(ncell), _d(dx), sym, _d(C), level, _d(out))
+    return out
+
+
+def newton_rhs(ncell, dx, sym, C, F0, bct, u, b):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); u = _c(u); b = _c(b)
+    rhs = np.zeros_like(b)
+    f0p = None if F0 is None else _c(F0)
+    lib().ora_newton_rhs(_i(ncell), _d(dx), sym, _d(C), None if f0p is None else _d(f0p), _i(bct), _d(u), _d(b),
+                         _d(rhs))
+    return rhs
+
+
+def bc_init(ncell, bct, bcv, homogeneous=False, b=None):
+    ncell = _c(ncell, np.int32); bct = _c(bct, np.int32); bcv = _c(bcv)
+    N = int(np.prod(nodes(ncell)))
+    b = np.zeros(3 * N) if b is None else _c(b).copy()
+    lib().ora_bc_init(_i(ncell), _i(bct), _d(bcv), int(homogeneous), _d(b))
+    return b
+
+
+def trac_xhi(ncell, dx, sym, C, F0, u):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); u = _c(u)
+    out = np.zeros(3)
+    f0p = None if F0 is None else _c(F0)
+    lib().ora_trac_xhi(_i(ncell), _d(dx), sym, _d(C), None if f0p is None else _d(f0p), _d(u), _d(out))
+    return out
+
+
+def cubic(C11, C12, C44, phi1=0.0, Phi=0.0, phi2=0.0):
+    out = np.zeros(21)
+    lib().ora_cubic(ctypes.c_double(C11), ctypes.c_double(C12), ctypes.c_double(C44), ctypes.c_double(phi1),
+                    ctypes.c_double(Phi), ctypes.c_double(phi2), _d(out))
+    return out
+
+
+def euler_R(phi1, Phi, phi2):
+    R = np.zeros(9)
+    lib().ora_euler_R(ctypes.c_double(phi1), ctypes.c_double(Phi), ctypes.c_double(phi2), _d(R))
+    return R.reshape(3, 3)
+
+
+def mm_slot(i, j, k, l):
+    return lib().ora_mm_slot(i, j, k, l)
+
+
+def c4(sym, c, i, j, k, l):
+    lib().ora_c4.restype = ctypes.c_double
+    c = _c(c)
+    return lib().ora_c4(sym, _d(c), i, j, k, l)
+
+
+def classify(i, j, k, lo, hi):
+    lo = _c(lo, np.int32); hi = _c(hi, np.int32)
+    return lib().ora_classify(i, j, k, _i(lo), _i(hi))
+
+
+def normal(face):
+    n = np.zeros(3)
+    lib().ora_normal(face, _d(n))
+    return n

@@ -1,0 +1,45 @@
+"""Shared builders for the tests: random fields, boundary tables, device round trips."""
+import numpy as np
+
+from alamo_amd import problems as P
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+def random_iso(nodes, seed, lo=0.5, hi=2.0):
+    r = rng(seed)
+    lam = r.uniform(lo, hi, nodes)
+    mu = r.uniform(lo, hi, nodes)
+    return P.iso_field(lam, mu)
+
+
+def random_mm(nodes, seed, ngr=5):
+    r = rng(seed)
+    tab = np.array([P.cubic_mm(1.68, 1.21, 0.75, P.euler_R(*r.uniform(0, 2 * np.pi, 3))) for _ in range(ngr)])
+    gid = r.integers(0, ngr, nodes)
+    w = r.uniform(0.8, 1.2, nodes)[:, None]
+    return (tab[gid] * w).ravel()
+
+
+def random_diag(nodes, seed):
+    return rng(seed).uniform(0.5, 2.0, nodes * 9)
+
+
+def random_bc(seed, types=(P.DISPLACEMENT, P.TRACTION, P.NEUMANN, P.NONE)):
+    r = rng(seed)
+    t = r.choice(np.array(types, dtype=np.int32), size=(26, 3)).astype(np.int32)
+    v = r.uniform(-1, 1, (26, 3))
+    return t, v
+
+
+def to_dev(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+
+def to_host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
