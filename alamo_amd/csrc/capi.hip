@@ -307,10 +307,12 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
     const bool mixed = h->d.precision == ALAMO_PREC_MIXED;
-    double* res0 = mixed ? h->sol_res : l0.res;
-    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0, 0.0, h->stream);
+    // The F-cycle swaps the level-0 residual buffer with its scratch partner, so the
+    // outer residual location is looked up afresh every iteration.
+    auto res0 = [&]() -> double* { return mixed ? h->sol_res : h->L[0].res; };
+    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0(), 0.0, h->stream);
     double r0 = 0, bn = 0, rl2 = 0;
-    TRY(norms_dev<double>(h, res0, n, &r0, &rl2));
+    TRY(norms_dev<double>(h, res0(), n, &r0, &rl2));
     TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
     const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
     double rn = r0;
@@ -318,15 +320,15 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
     if (h->d.verbose) fprintf(stderr, "alamo: iter 0 |r|inf %.6e target %.3e\n", r0, target);
     while (rn > target && it < h->d.max_iter && std::isfinite(rn)) {
         if (mixed) {
-            k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(res0, l0.resf, n);
+            k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(res0(), h->L[0].resf, n);
             one_cycle<float>(h);
-            k_axpy<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, l0.corf, n, 1.0);
+            k_axpy<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, h->L[0].corf, n, 1.0);
         } else {
             one_cycle<double>(h);
-            k_axpy<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, l0.cor, n, 1.0);
+            k_axpy<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, h->L[0].cor, n, 1.0);
         }
-        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0, 0.0, h->stream);
-        TRY(norms_dev<double>(h, res0, n, &rn, &rl2));
+        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0(), 0.0, h->stream);
+        TRY(norms_dev<double>(h, res0(), n, &rn, &rl2));
         it++;
         if (h->d.verbose) fprintf(stderr, "alamo: iter %d |r|inf %.6e\n", it, rn);
     }

@@ -1,0 +1,217 @@
+"""Benchmark: time to a 1e-10 relative FP64 residual on a 257^3-node two-phase field, and
+Fapply HBM bandwidth.
+
+A step is one complete solve A u = b (initial guess 0) of the 256^3-cell / 257^3-node
+two-phase isotropic field (Gaussian random field, correlation length 8 cells, diffuse
+interface, stiffness contrast 1:1000, dilatational eigenstrain in the stiff phase, u = 0 on
+all 26 boundary regions), through the C ABI, with b, C and u resident in HBM.
+
+Prints one JSON line (rank 0).  `value` is the time-to-solution of the fastest solver
+method in seconds (lower is better); the plain FP64 multigrid time is reported beside it.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=256, help="cells per direction")
+    p.add_argument("--corr", type=float, default=8.0)
+    p.add_argument("--contrast", type=float, default=1000.0)
+    p.add_argument("--seed", type=int, default=256)
+    p.add_argument("--fapply-reps", type=int, default=50)
+    p.add_argument("--skip-cpu", action="store_true")
+    p.add_argument("--methods", default="fp64,mixed")
+    p.add_argument("--fapply-only", action="store_true",
+                   help="only time Fapply on the finest level (plus a device copy of the same bytes), "
+                        "for counter runs")
+    return p.parse_args()
+
+
+def fapply_only(a):
+    import numpy as np
+    import torch
+    from alamo_amd import problems as P
+    from alamo_amd.operator import Elastic
+    dev = torch.device("cuda:0")
+    n = a.n
+    N = (n + 1) ** 3
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]))
+    op.SetModel(pr["C"])
+    op.prepareForSolve()
+    u = torch.rand(3 * N, dtype=torch.float64, device=dev)
+    f = torch.empty_like(u)
+    c = torch.empty_like(u)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.fapply_reps):
+        op.Fapply(0, 0, f, u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.fapply_reps
+    # calibration copy: reads and writes 3 N doubles
+    for _ in range(3):
+        c.copy_(u)
+    torch.cuda.synchronize()
+    print(json.dumps({"fapply_ms": ms, "fapply_gbs": 64.0 * N / (ms * 1e-3) / 1e9, "nodes": N,
+                      "copy_bytes_each_way": 24 * N}))
+
+
+def main():
+    a = parse()
+    if a.fapply_only:
+        fapply_only(a)
+        return
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (the solver has no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from alamo_amd import problems as P
+    from alamo_amd.operator import Elastic, Linear, PREC_FP64, PREC_MIXED, CYCLE_F
+
+    dev = torch.device("cuda:0")
+    n = a.n
+    t0 = time.time()
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed)
+    print("problem generated in %.1fs" % (time.time() - t0), file=sys.stderr, flush=True)
+    N = (n + 1) ** 3
+    bc = (pr["bct"], pr["bcv"])
+    T = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
+
+    results = {}
+    rhs = None
+    fapply = None
+    for meth in a.methods.split(","):
+        prec = PREC_MIXED if meth == "mixed" else PREC_FP64
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_F)
+        op.SetModel(pr["C"])
+        op.prepareForSolve()
+        if rhs is None:
+            b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+            op.BCInit(b)
+            rhs = torch.zeros_like(b)
+            op.NewtonRHS(torch.zeros_like(b), b, T(pr["F0"]), rhs)
+            del b
+        sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+        lin = Linear(op)
+        for _ in range(a.warmup):
+            sol.zero_()
+            lin.solve(sol, rhs, 1e-10, 0.0)
+        torch.cuda.synchronize()
+        stats = []
+        ts = time.perf_counter()
+        for _ in range(a.steps):
+            sol.zero_()
+            stats.append(lin.solve(sol, rhs, 1e-10, 0.0))
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        results[meth] = dict(seconds=(te - ts) / a.steps, iterations=stats[-1]["iterations"],
+                             final_rel_res=stats[-1]["final_res_inf"] / max(stats[-1]["rhs_inf"], 1e-300),
+                             converged=all(s["converged"] for s in stats))
+        print("%s: %.4f s/solve, %d iterations" % (meth, results[meth]["seconds"], results[meth]["iterations"]),
+              file=sys.stderr, flush=True)
+        if meth == "fp64" or fapply is None:
+            # Fapply (level 0) bandwidth with HIP events on the launch stream (the default stream)
+            u = torch.rand(3 * N, dtype=torch.float64, device=dev)
+            f = torch.empty_like(u)
+            op.Fapply(0, 0, f, u)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(a.fapply_reps):
+                op.Fapply(0, 0, f, u)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / a.fapply_reps
+            bytes_per_launch = 64.0 * N  # (3 u + 3 f + 2 C) x 8 B per node, isotropic FP64
+            fapply = dict(ms=ms, gbs=bytes_per_launch / (ms * 1e-3) / 1e9, bytes=bytes_per_launch)
+            del u, f
+        del op, lin, sol
+        torch.cuda.empty_cache()
+
+    best = min(results, key=lambda k: results[k]["seconds"])
+    val = results[best]["seconds"]
+
+    cpu = None
+    if not a.skip_cpu:
+        cpu = cpu_baseline(pr, rhs.cpu().numpy(), results["fp64"]["iterations"] if "fp64" in results else 20)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "fapply_iso_f64_257_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    out = {
+        "metric": "time-to-1e-10 residual at 256^3 nodes (learned vs plain MG); Fapply HBM GB/s",
+        "value": val,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": val * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32-in-f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "two-phase GRF (corr %g cells, contrast 1:%g, seed %d) isotropic field, %d^3 cells / %d^3 "
+                        "nodes, Dirichlet box, eigenstrain load; solve to |b-Au|inf/|b|inf <= 1e-10"
+                        % (a.corr, a.contrast, a.seed, n, n + 1),
+            "method": best,
+            "methods": results,
+            "fapply_gbs": fapply["gbs"] if fapply else None,
+            "fapply_ms": fapply["ms"] if fapply else None,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": fapply["gbs"] if fapply else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (fapply["gbs"] / HBM_PEAK_GBS) if fapply else None,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(pr, rhs, gpu_iters):
+    """The oracle's FP64 multigrid (same F-cycle algorithm) on the host cores: one setup plus
+    one F-cycle at the full size, timed, and scaled to the GPU solve's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as ora
+    from alamo_amd import problems as P
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=1)
+    t_one = time.perf_counter() - t0
+    per_cycle = max(t_one - t_setup, 1e-9)
+    est = t_setup + gpu_iters * per_cycle
+    return {"value": est, "unit": "s", "cores": cores, "kind": "port",
+            "sample": "oracle FP64 F-cycle multigrid on the full 257^3 field: setup+residual (%.2fs) and one "
+                      "F-cycle (%.2fs) timed, scaled to %d cycles" % (t_setup, per_cycle, gpu_iters)}
+
+
+if __name__ == "__main__":
+    main()

@@ -447,11 +447,16 @@ static void interp_add(const Level& Lc, const Level& Lf, const double* crse, dou
             }
 }
 
+// max |x|; NaN if any entry is NaN, so that a diverged solve cannot look converged
 static double norm_inf(const double* x, int64_t n) {
     double m = 0.0;
-#pragma omp parallel for reduction(max : m)
-    for (int64_t q = 0; q < n; q++) m = std::max(m, std::fabs(x[q]));
-    return m;
+    int bad = 0;
+#pragma omp parallel for reduction(max : m) reduction(| : bad)
+    for (int64_t q = 0; q < n; q++) {
+        if (x[q] != x[q]) bad = 1;
+        else m = std::max(m, std::fabs(x[q]));
+    }
+    return bad ? std::nan("") : m;
 }
 static double dot(const double* a, const double* b, int64_t n) {
     double s = 0.0;
@@ -732,6 +737,7 @@ using namespace ora;
 extern "C" {
 
 int ora_ncoef(int sym) { return ncoef(sym); }
+
 int ora_mm_slot(int i, int j, int k, int l) { return mm_slot(i, j, k, l); }
 double ora_c4(int sym, const double* c, int i, int j, int k, int l) { return C4(sym, c, i, j, k, l); }
 void ora_cubic(double C11, double C12, double C44, double phi1, double Phi, double phi2, double* out21) {
