@@ -31,6 +31,11 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KrylovOpts(ctypes.Structure):
+    _fields_ = [("restart", c_int), ("precond", c_int), ("max_iter", c_int), ("stagnation_ratio", c_double),
+                ("verbose", c_int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "alamo_last_error": (ctypes.c_char_p, []),
@@ -56,6 +61,9 @@ SIGNATURES = {
     "alamo_bc_init": (c_int, [c_void_p, c_void_p, c_int]),
     "alamo_newton_rhs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_linear_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(SolveStats)]),
+    "alamo_krylov_opts_default": (c_int, [P(KrylovOpts)]),
+    "alamo_krylov_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(KrylovOpts),
+                                   P(SolveStats)]),
 }
 
 _lib = None

@@ -18,6 +18,7 @@
 #include "../../include/alamo_amd.h"
 #include "kernels_generic.h"
 #include "kernels_iso.h"
+#include "krylov.h"
 
 using namespace alamo;
 
@@ -67,12 +68,19 @@ struct alamo_mg {
     std::vector<LevelBuf> L;
     double* partial = nullptr;      // reduction partials
     double* red = nullptr;          // device reduction result (4 doubles)
-    double* red_host = nullptr;     // pinned host mirror
+    double* red_host = nullptr;     // pinned host mirror (KRY_MAXV + 4 doubles)
     double* bottom_work = nullptr;  // 8 x 3 x tot(bottom)
     int* bottom_iters = nullptr;
     double* sol_res = nullptr;      // level-0 FP64 residual for the mixed-precision outer loop
     bool model_set = false, prepared = false;
     int nblk_red = 1024;
+    // flexible GMRES workspace (allocated on first use)
+    double* kV = nullptr;     // (m+1) basis vectors
+    double* kZ = nullptr;     // m preconditioned vectors
+    int kry_m = 0;
+    double* kcoef = nullptr;  // device copy of Gram-Schmidt / solution coefficients
+    double* kpart = nullptr;  // multi-dot partials
+    double* kdots = nullptr;  // multi-dot results (device)
     std::vector<void*> allocs;
 };
 
@@ -352,6 +360,149 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
     return ALAMO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Flexible GMRES(m) (FP64) with a multigrid-cycle preconditioner
+// ---------------------------------------------------------------------------
+static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    if (kind == ALAMO_PRECOND_NONE) {
+        HIPCHK(hipMemcpyAsync(z, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else if (h->d.precision == ALAMO_PREC_MIXED) {
+        k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, h->L[0].resf, n);
+        one_cycle<float>(h);
+        k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(h->L[0].corf, z, n);
+    } else {
+        HIPCHK(hipMemcpyAsync(h->L[0].res, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        one_cycle<double>(h);
+        HIPCHK(hipMemcpyAsync(z, h->L[0].cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    return ALAMO_OK;
+}
+
+// d[0..nv) = w . V_i  (device result copied to the pinned host buffer `out`)
+static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out) {
+    const int nb = (int)std::min<int64_t>(512, (n + RED_BLOCK - 1) / RED_BLOCK);
+    k_multidot<<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kpart);
+    k_sum_rows<<<nv, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, nv * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < nv; i++) out[i] = h->red_host[i];
+    return ALAMO_OK;
+}
+
+static int upload_coef(alamo_mg* h, const double* c, int nv) {
+    for (int i = 0; i < nv; i++) h->red_host[i] = c[i];
+    HIPCHK(hipMemcpyAsync(h->kcoef, h->red_host, nv * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    // the pinned staging buffer is reused by the next reduction: wait for the copy
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ALAMO_OK;
+}
+
+static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       const alamo_krylov_opts* o, alamo_solve_stats* st) {
+    if (h->L.back().g.tot > 17 * 17 * 17 && o->precond != ALAMO_PRECOND_NONE)
+        FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
+    const int m = o->restart;
+    if (m < 1 || m >= KRY_MAXV) FAIL(ALAMO_ERR_ARG, "restart must be in [1, 31]");
+    if (!h->prepared) TRY(prepare(h));
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    if (h->kry_m < m) {
+        TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * n));
+        TRY(dalloc(h, &h->kZ, (int64_t)m * n));
+        if (!h->kcoef) {
+            TRY(dalloc(h, &h->kcoef, KRY_MAXV));
+            TRY(dalloc(h, &h->kpart, (int64_t)KRY_MAXV * 512));
+            TRY(dalloc(h, &h->kdots, KRY_MAXV));
+        }
+        h->kry_m = m;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    double* V = h->kV;
+    double* Z = h->kZ;
+    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
+    double r0 = 0, beta = 0, bn = 0;
+    TRY(norms_dev<double>(h, V, n, &r0, &beta));
+    TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
+    double rinf = r0;
+    int it = 0;
+    double H[KRY_MAXV + 1][KRY_MAXV], cs[KRY_MAXV], sn[KRY_MAXV], gv[KRY_MAXV + 1], y[KRY_MAXV];
+    double d1[KRY_MAXV], d2[KRY_MAXV];
+    if (o->verbose) fprintf(stderr, "alamo fgmres: iter 0 |r|inf %.6e |r|2 %.6e target %.3e\n", r0, beta, target);
+    while (rinf > target && it < o->max_iter && std::isfinite(rinf) && beta > 0.0) {
+        k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(V, V, n, 1.0 / beta);
+        for (int i = 0; i <= m; i++) gv[i] = 0.0;
+        gv[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m; j++) {
+            double* vj = V + (int64_t)j * n;
+            double* zj = Z + (int64_t)j * n;
+            double* w = V + (int64_t)(j + 1) * n;
+            TRY(precond_apply(h, o->precond, vj, zj));
+            launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
+            // CGS2
+            TRY(multidot(h, w, V, n, j + 1, d1));
+            TRY(upload_coef(h, d1, j + 1));
+            k_multiaxpy<<<gs_blocks(n), 256, 0, h->stream>>>(w, V, n, n, j + 1, h->kcoef);
+            TRY(multidot(h, w, V, n, j + 1, d2));
+            TRY(upload_coef(h, d2, j + 1));
+            k_multiaxpy<<<gs_blocks(n), 256, 0, h->stream>>>(w, V, n, n, j + 1, h->kcoef);
+            double hn = 0;
+            TRY(norms_dev<double>(h, w, n, nullptr, &hn));
+            for (int i = 0; i <= j; i++) H[i][j] = d1[i] + d2[i];
+            H[j + 1][j] = hn;
+            if (hn > 0.0) k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / hn);
+            for (int i = 0; i < j; i++) {
+                const double a = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+                H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+                H[i][j] = a;
+            }
+            const double den = std::hypot(H[j][j], H[j + 1][j]);
+            cs[j] = den > 0 ? H[j][j] / den : 1.0;
+            sn[j] = den > 0 ? H[j + 1][j] / den : 0.0;
+            H[j][j] = den;
+            H[j + 1][j] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            it++;
+            jend = j + 1;
+            if (o->verbose > 1) fprintf(stderr, "alamo fgmres: step %d est |r|2 %.6e\n", it, std::fabs(gv[j + 1]));
+            if (std::fabs(gv[j + 1]) <= target || it >= o->max_iter || hn == 0.0 || !std::isfinite(hn)) break;
+        }
+        for (int i = jend - 1; i >= 0; i--) {
+            double s = gv[i];
+            for (int k = i + 1; k < jend; k++) s -= H[i][k] * y[k];
+            y[i] = H[i][i] != 0.0 ? s / H[i][i] : 0.0;
+        }
+        TRY(upload_coef(h, y, jend));
+        k_multiaxpy_add<<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
+        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
+        TRY(norms_dev<double>(h, V, n, &rinf, &beta));
+        if (o->verbose) fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e\n", it, rinf, beta);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    const bool conv = rinf <= target;
+    if (st) {
+        st->iterations = it;
+        st->converged = conv ? 1 : 0;
+        st->initial_res_inf = r0;
+        st->final_res_inf = rinf;
+        st->final_res_l2 = beta;
+        st->rhs_inf = bn;
+        st->target = target;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->fallback_events = 0;
+    }
+    if (!std::isfinite(rinf)) FAIL(ALAMO_ERR_NAN, "alamo_krylov_solve: residual is not finite");
+    if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_krylov_solve: max_iter reached before the tolerance");
+    return ALAMO_OK;
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -447,7 +598,7 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     rc |= dalloc(h, &h->bottom_work, 8 * 3 * gs.back().tot);
     rc |= dalloc(h, &h->bottom_iters, 1);
     if (mixed) rc |= dalloc(h, &h->sol_res, 3 * gs[0].tot);
-    if (rc == ALAMO_OK && hipHostMalloc((void**)&h->red_host, 4 * sizeof(double)) != hipSuccess) {
+    if (rc == ALAMO_OK && hipHostMalloc((void**)&h->red_host, (KRY_MAXV + 4) * sizeof(double)) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         rc = ALAMO_ERR_HIP;
     }
@@ -625,6 +776,28 @@ int alamo_mg_norms(alamo_mg* h, int mglev, const double* x, double* norm_inf, do
 
 int alamo_bc_init(alamo_mg* h, double* b, int homogeneous);
 int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs);
+
+int alamo_krylov_opts_default(alamo_krylov_opts* o) {
+    if (!o) FAIL(ALAMO_ERR_ARG, "null opts");
+    o->restart = 16;
+    o->precond = ALAMO_PRECOND_MG;
+    o->max_iter = 400;
+    o->stagnation_ratio = 0.9;
+    o->verbose = 0;
+    return ALAMO_OK;
+}
+
+int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       const alamo_krylov_opts* opts, alamo_solve_stats* stats) {
+    if (!h || !sol || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    alamo_krylov_opts o;
+    if (opts) o = *opts;
+    else alamo_krylov_opts_default(&o);
+    if (o.precond != ALAMO_PRECOND_NONE && o.precond != ALAMO_PRECOND_MG)
+        FAIL(ALAMO_ERR_ARG, "unknown preconditioner");
+    return fgmres_impl(h, sol, rhs, tol_rel, tol_abs, &o, stats);
+}
 
 int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        alamo_solve_stats* stats) {

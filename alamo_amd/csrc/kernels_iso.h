@@ -13,9 +13,28 @@
 
 namespace alamo {
 
-constexpr int ISO_TX = 64;
-constexpr int ISO_TY = 4;
-constexpr int ISO_KCH = 32;
+#ifndef ALAMO_ISO_TY
+#define ALAMO_ISO_TY 4
+#endif
+#ifndef ALAMO_ISO_KCH
+#define ALAMO_ISO_KCH 32
+#endif
+#ifndef ALAMO_ISO_XCD
+#define ALAMO_ISO_XCD 0
+#endif
+constexpr int ISO_TX = 64;            // one wavefront per row segment
+constexpr int ISO_TY = ALAMO_ISO_TY;  // rows per workgroup
+constexpr int ISO_KCH = ALAMO_ISO_KCH;  // z planes marched per workgroup
+constexpr int NUM_XCD = 8;            // MI355X accelerator complex dies, one L2 each
+
+// Workgroups are dispatched round-robin over the XCDs; when enabled, give each XCD a
+// contiguous range of tiles (x fastest, then y, then the z chunk) so neighbouring tiles
+// share halo rows in the same L2.  The map is a bijection of [0, nblk).
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+    const int per = nblk / NUM_XCD, rem = nblk % NUM_XCD;
+    const int x = bid % NUM_XCD, r = bid / NUM_XCD;
+    return x * per + min(x, rem) + r;
+}
 
 template <typename T>
 struct IsoPlane {
@@ -23,7 +42,11 @@ struct IsoPlane {
     T lam, mu, lx, ly, mx, my;
 };
 
-inline bool iso_tiled_ok(const Geom& g) { return g.n[0] >= 16 && g.n[1] >= 16 && g.n[2] >= 16; }
+// The tiled kernel marches z serially inside a workgroup, so it only pays on levels large
+// enough to give every CU several workgroups; smaller levels use the node-parallel kernel.
+inline bool iso_tiled_ok(const Geom& g) {
+    return g.n[0] >= 16 && g.n[1] >= 16 && g.n[2] >= 16 && g.tot >= (int64_t)1 << 20;
+}
 
 template <typename T>
 __device__ __forceinline__ void iso_stage_plane(T* __restrict__ s, const Geom& g, const T* __restrict__ u,
@@ -79,8 +102,17 @@ __global__ void __launch_bounds__(ISO_TX* ISO_TY) k_iso_interior(Geom g, const T
     constexpr int A = (ISO_TX + 2) * (ISO_TY + 2);
     __shared__ T smem[2][5 * A];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = 1 + blockIdx.x * ISO_TX, j0 = 1 + blockIdx.y * ISO_TY;
-    const int k0 = 1 + blockIdx.z * ISO_KCH;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (ALAMO_ISO_XCD) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int bid = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const int t = xcd_tile(bid, gx * gy * gridDim.z);
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
+    const int i0 = 1 + bx * ISO_TX, j0 = 1 + by * ISO_TY;
+    const int k0 = 1 + bz * ISO_KCH;
     const int k1 = min(k0 + ISO_KCH, g.n[2]);  // exclusive; interior k <= n2 - 1
     const int i = i0 + tx, j = j0 + ty;
     const bool act = i < g.n[0] && j < g.n[1];

@@ -1,0 +1,88 @@
+// Flexible GMRES(m) in FP64 around a preconditioner that may change from one application
+// to the next (a multigrid cycle with a BiCGStab bottom, or the learned network), after
+// Saad, "A flexible inner-outer preconditioned GMRES algorithm", SIAM J. Sci. Comput. 14 (1993).
+//
+// The operator is the nonsymmetric strong-form Operator::Elastic, so the outer Krylov method
+// is GMRES rather than CG.  Orthogonalisation is classical Gram-Schmidt with one
+// re-orthogonalisation pass (CGS2): each pass is one fused multi-dot kernel and one
+// multi-axpy kernel over the basis, so a step reads the basis twice instead of once per
+// vector as modified Gram-Schmidt would.
+#pragma once
+#include "kernels_generic.h"
+
+namespace alamo {
+
+constexpr int KRY_MAXV = 32;  // largest restart length + 1
+
+// partial[i * nb + block] = sum over the block's share of w . V_i, i < nv
+__global__ void __launch_bounds__(RED_BLOCK) k_multidot(const double* __restrict__ w, const double* __restrict__ V,
+                                                        int64_t n, int64_t ldv, int nv,
+                                                        double* __restrict__ partial) {
+    double acc[KRY_MAXV];
+#pragma unroll
+    for (int i = 0; i < KRY_MAXV; i++) acc[i] = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double wq = w[q];
+#pragma unroll
+        for (int i = 0; i < KRY_MAXV; i++)
+            if (i < nv) acc[i] += wq * V[i * ldv + q];
+    }
+    __shared__ double sh[RED_BLOCK / 64];
+#pragma unroll
+    for (int i = 0; i < KRY_MAXV; i++) {
+        if (i >= nv) break;
+        double v = wave_sum(acc[i]);
+        if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int q = 0; q < RED_BLOCK / 64; q++) s += sh[q];
+            partial[i * gridDim.x + blockIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// out[i] = sum_b partial[i * nb + b]; one block per vector
+__global__ void __launch_bounds__(RED_BLOCK) k_sum_rows(const double* __restrict__ partial, int nb,
+                                                        double* __restrict__ out) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[blockIdx.x * nb + b];
+    s = wave_sum(s);
+    __shared__ double sh[RED_BLOCK / 64];
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < RED_BLOCK / 64; q++) t += sh[q];
+        out[blockIdx.x] = t;
+    }
+}
+
+// w -= sum_{i < nv} c_i V_i
+__global__ void __launch_bounds__(256) k_multiaxpy(double* __restrict__ w, const double* __restrict__ V, int64_t n,
+                                                   int64_t ldv, int nv, const double* __restrict__ c) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        double s = w[q];
+        for (int i = 0; i < nv; i++) s -= c[i] * V[i * ldv + q];
+        w[q] = s;
+    }
+}
+
+// x += sum_{i < nv} c_i Z_i
+__global__ void __launch_bounds__(256) k_multiaxpy_add(double* __restrict__ x, const double* __restrict__ Z, int64_t n,
+                                                       int64_t ldz, int nv, const double* __restrict__ c) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        double s = x[q];
+        for (int i = 0; i < nv; i++) s += c[i] * Z[i * ldz + q];
+        x[q] = s;
+    }
+}
+
+// y = a * x
+__global__ void k_scale(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        y[q] = a * x[q];
+}
+
+}  // namespace alamo

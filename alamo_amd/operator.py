@@ -15,10 +15,11 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import MGDesc, SolveStats, check, lib
+from ._lib import KrylovOpts, MGDesc, SolveStats, check, lib
 from .problems import ISO, MAJORMINOR, DIAGONAL, NCOEF
 
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
+PRECOND_NONE, PRECOND_MG, PRECOND_LEARNED = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 CYCLE_V, CYCLE_F = 0, 1
 
@@ -175,6 +176,29 @@ class Linear:
             raise _lib.AlamoError(3, "Solver not defined")
         st = SolveStats()
         rc = lib().alamo_linear_solve(self.linop.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(st))
+        if rc != 0 and raise_on_fail:
+            check(rc)
+        out = st.as_dict()
+        out["rc"] = rc
+        return out
+
+
+class Krylov:
+    """Flexible GMRES(restart) around the operator's multigrid cycle (no reference counterpart:
+    an outer accelerator for the same MLMG-style cycle, stopping on the reference's max-norm test)."""
+
+    def __init__(self, op, restart=16, precond=PRECOND_MG, max_iter=400, stagnation_ratio=0.9, verbose=0):
+        self.op = op
+        o = KrylovOpts()
+        check(lib().alamo_krylov_opts_default(ctypes.byref(o)))
+        o.restart, o.precond, o.max_iter, o.stagnation_ratio, o.verbose = (restart, precond, max_iter,
+                                                                           stagnation_ratio, verbose)
+        self.opts = o
+
+    def solve(self, sol, rhs, tol_rel, tol_abs, raise_on_fail=True):
+        st = SolveStats()
+        rc = lib().alamo_krylov_solve(self.op.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(self.opts),
+                                      ctypes.byref(st))
         if rc != 0 and raise_on_fail:
             check(rc)
         out = st.as_dict()

@@ -133,6 +133,25 @@ int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double
 int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        alamo_solve_stats* stats);
 
+/* Outer Krylov solve (no reference counterpart; the reference stops at amrex::MLMG).
+ * Flexible GMRES(restart) in FP64, right-preconditioned by one multigrid cycle of the
+ * handle's kind (ALAMO_PRECOND_MG: cycle, smoothing and precision of the descriptor, so a
+ * mixed-precision handle preconditions with FP32 cycles) or by nothing.  The stopping test
+ * is the reference's: |rhs - A sol|_inf of the FP64 true residual <= max(tol_abs,
+ * tol_rel * max(|rhs|_inf, |r0|_inf)).  stats->iterations counts preconditioner applications. */
+enum { ALAMO_PRECOND_NONE = 0, ALAMO_PRECOND_MG = 1, ALAMO_PRECOND_LEARNED = 2 };
+typedef struct {
+    int restart;               /* Krylov vectors per restart cycle (1..31) */
+    int precond;               /* ALAMO_PRECOND_* */
+    int max_iter;              /* total preconditioner applications */
+    double stagnation_ratio;   /* learned preconditioner: fall back to the cycle when a restart
+                                  reduces |r|_2 by less than this factor */
+    int verbose;
+} alamo_krylov_opts;
+int alamo_krylov_opts_default(alamo_krylov_opts* o);
+int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       const alamo_krylov_opts* opts, alamo_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from alamo_amd import problems as P
-from alamo_amd.operator import Elastic, Linear, LAYOUT_AOS, PREC_MIXED, CYCLE_V
+from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, CYCLE_V, PRECOND_NONE
 from helpers import random_iso, random_mm, random_diag, random_bc, to_dev, to_host, rng
 
 pytestmark = pytest.mark.gpu
@@ -16,11 +16,13 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-12
 
 CASES = [
-    # (sym, ncell)  -- 8^3 and 12x10x9 run the node-parallel kernels, >= 16 the tiled iso kernel
+    # (sym, ncell)  -- levels of >= 2^20 nodes run the LDS-tiled isotropic kernel (incl. the
+    # folded x-face rows), smaller ones the node-parallel kernels
     (P.ISO, (8, 8, 8)),
     (P.ISO, (12, 10, 9)),
-    (P.ISO, (16, 16, 16)),
     (P.ISO, (70, 33, 40)),
+    (P.ISO, (128, 100, 80)),
+    (P.ISO, (130, 97, 83)),
     (P.MAJORMINOR, (8, 8, 8)),
     (P.MAJORMINOR, (11, 7, 9)),
     (P.DIAGONAL, (9, 8, 10)),
@@ -164,6 +166,45 @@ def test_linear_solve_eshelby(cuda, ora, precision, cycle):
     err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
     print("iterations", st["iterations"], "rel L2 err", err)
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("precision,cycle,precond", [(0, 1, 1), (PREC_MIXED, 0, 1), (0, 0, PRECOND_NONE)])
+def test_krylov_solve_eshelby(cuda, ora, precision, cycle, precond):
+    """Flexible GMRES (multigrid-cycle or no preconditioner) converges to the oracle's solution."""
+    n = 16
+    pr, bc, rhs = eshelby_rhs(ora, n)
+    xref, it, h, fr, nl = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-12)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=precision, cycle=cycle,
+                 pre_smooth=1 if cycle == 0 else 2, post_smooth=1 if cycle == 0 else 2)
+    op.SetModel(pr["C"])
+    sol = to_dev(np.zeros_like(rhs), cuda)
+    st = Krylov(op, restart=16, precond=precond, max_iter=3000).solve(sol, to_dev(rhs, cuda), 1e-12, 0.0)
+    x = to_host(sol)
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    print("krylov iterations", st["iterations"], "rel L2 err", err)
+    assert st["converged"] == 1 and err <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_large_solve_true_residual(cuda, ora, precision):
+    """128^3 cells (tiled kernels on the finest level): the GPU solution's residual, evaluated by
+    the oracle's operator on the CPU, meets the 1e-10 target (1e-9 allowing rounding)."""
+    n = 128
+    pr = P.eshelby_problem(n)
+    bct, bcv = P.bc_constant()
+    N = (n + 1) ** 3
+    b = ora.bc_init(pr["ncell"], bct, bcv)
+    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, np.zeros(3 * N), b)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(bct, bcv), precision=precision)
+    op.SetModel(pr["C"])
+    sol = to_dev(np.zeros(3 * N), cuda)
+    print("solve 128^3 precision", precision, flush=True)
+    st = Linear(op).solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
+    x = to_host(sol)
+    r = ora.residual(pr["ncell"], pr["dx"], P.ISO, pr["C"], bct, x, rhs)
+    print("iterations", st["iterations"], "oracle |r|/|b|", np.abs(r).max() / np.abs(rhs).max())
+    assert st["converged"] == 1
+    assert np.abs(r).max() <= 1e-9 * np.abs(rhs).max()
 
 
 def test_solid_linear_cubic_golden(cuda, ora):
