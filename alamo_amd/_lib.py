@@ -37,6 +37,20 @@ class KrylovOpts(ctypes.Structure):
 
 
 # name -> (restype, argtypes)
+class TrainOpts(ctypes.Structure):
+    _fields_ = [("npairs", c_int), ("steps", c_int), ("lr", c_double), ("seed", ctypes.c_uint64),
+                ("solve_max_iter", c_int), ("solve_tol", c_double), ("verbose", c_int)]
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("pairs", c_int), ("pairs_converged", c_int), ("steps", c_int),
+                ("first_loss", c_double), ("last_loss", c_double),
+                ("seconds_data", c_double), ("seconds_train", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 SIGNATURES = {
     "alamo_last_error": (ctypes.c_char_p, []),
     "alamo_version": (c_int, []),
@@ -64,6 +78,21 @@ SIGNATURES = {
     "alamo_krylov_opts_default": (c_int, [P(KrylovOpts)]),
     "alamo_krylov_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(KrylovOpts),
                                    P(SolveStats)]),
+    "alamo_net_create": (c_int, [P(c_void_p), ctypes.c_uint64]),
+    "alamo_net_destroy": (c_int, [c_void_p]),
+    "alamo_net_num_params": (c_int, [c_void_p, P(ctypes.c_int64)]),
+    "alamo_net_get_params": (c_int, [c_void_p, c_void_p]),
+    "alamo_net_set_params": (c_int, [c_void_p, c_void_p]),
+    "alamo_net_get_grads": (c_int, [c_void_p, c_void_p]),
+    "alamo_net_zero_grads": (c_int, [c_void_p]),
+    "alamo_net_forward_host": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_net_backward_host": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_net_adam_step": (c_int, [c_void_p, c_double]),
+    "alamo_net_train": (c_int, [c_void_p, c_void_p, P(TrainOpts), P(TrainStats)]),
+    "alamo_net_save": (c_int, [c_void_p, ctypes.c_char_p]),
+    "alamo_net_load": (c_int, [c_void_p, ctypes.c_char_p]),
+    "alamo_mg_set_net": (c_int, [c_void_p, c_void_p]),
+    "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

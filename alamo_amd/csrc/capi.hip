@@ -81,6 +81,8 @@ struct alamo_mg {
     double* kcoef = nullptr;  // device copy of Gram-Schmidt / solution coefficients
     double* kpart = nullptr;  // multi-dot partials
     double* kdots = nullptr;  // multi-dot results (device)
+    alamo_net* net = nullptr; // learned correction (non-owning)
+    double* lq = nullptr;     // FP64 scratch of the learned stage
     std::vector<void*> allocs;
 };
 
@@ -363,9 +365,17 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
 // ---------------------------------------------------------------------------
 // Flexible GMRES(m) (FP64) with a multigrid-cycle preconditioner
 // ---------------------------------------------------------------------------
+#include "learned_host.h"
+
 static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
+    if (kind == ALAMO_PRECOND_LEARNED) {
+        // one multigrid cycle, then the network's correction of the cycle's remaining error
+        TRY(precond_apply(h, ALAMO_PRECOND_MG, v, z));
+        if (!h->lq) TRY(dalloc(h, &h->lq, n));
+        return net_correct(h->net, h, v, z, h->lq);
+    }
     if (kind == ALAMO_PRECOND_NONE) {
         HIPCHK(hipMemcpyAsync(z, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     } else if (h->d.precision == ALAMO_PREC_MIXED) {
@@ -382,8 +392,8 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
 
 // d[0..nv) = w . V_i  (device result copied to the pinned host buffer `out`)
 static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out) {
-    const int nb = (int)std::min<int64_t>(512, (n + RED_BLOCK - 1) / RED_BLOCK);
-    k_multidot<<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kpart);
+    const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
+    launch_multidot(w, V, n, n, nv, nb, h->kpart, h->stream);
     k_sum_rows<<<nv, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
     HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, nv * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -413,7 +423,7 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
         TRY(dalloc(h, &h->kZ, (int64_t)m * n));
         if (!h->kcoef) {
             TRY(dalloc(h, &h->kcoef, KRY_MAXV));
-            TRY(dalloc(h, &h->kpart, (int64_t)KRY_MAXV * 512));
+            TRY(dalloc(h, &h->kpart, (int64_t)KRY_MAXV * KRY_DOT_BLOCKS));
             TRY(dalloc(h, &h->kdots, KRY_MAXV));
         }
         h->kry_m = m;
@@ -428,11 +438,14 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
     const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
     double rinf = r0;
-    int it = 0;
+    int it = 0, fallbacks = 0;
+    int kind = o->precond;
+    double beta_prev = beta;
     double H[KRY_MAXV + 1][KRY_MAXV], cs[KRY_MAXV], sn[KRY_MAXV], gv[KRY_MAXV + 1], y[KRY_MAXV];
     double d1[KRY_MAXV], d2[KRY_MAXV];
     if (o->verbose) fprintf(stderr, "alamo fgmres: iter 0 |r|inf %.6e |r|2 %.6e target %.3e\n", r0, beta, target);
     while (rinf > target && it < o->max_iter && std::isfinite(rinf) && beta > 0.0) {
+        const double ratio = rinf / beta;
         k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(V, V, n, 1.0 / beta);
         for (int i = 0; i <= m; i++) gv[i] = 0.0;
         gv[0] = beta;
@@ -441,7 +454,7 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
             double* vj = V + (int64_t)j * n;
             double* zj = Z + (int64_t)j * n;
             double* w = V + (int64_t)(j + 1) * n;
-            TRY(precond_apply(h, o->precond, vj, zj));
+            TRY(precond_apply(h, kind, vj, zj));
             launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             // CGS2
             TRY(multidot(h, w, V, n, j + 1, d1));
@@ -470,7 +483,10 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
             it++;
             jend = j + 1;
             if (o->verbose > 1) fprintf(stderr, "alamo fgmres: step %d est |r|2 %.6e\n", it, std::fabs(gv[j + 1]));
-            if (std::fabs(gv[j + 1]) <= target || it >= o->max_iter || hn == 0.0 || !std::isfinite(hn)) break;
+            // |g_{j+1}| estimates |r|_2; the stopping test is on |r|_inf, so scale by the inf/2 ratio
+            // of the last true residual (the true residual at the restart decides convergence)
+            if (std::fabs(gv[j + 1]) * ratio <= target || it >= o->max_iter || hn == 0.0 || !std::isfinite(hn))
+                break;
         }
         for (int i = jend - 1; i >= 0; i--) {
             double s = gv[i];
@@ -482,6 +498,19 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
         TRY(norms_dev<double>(h, V, n, &rinf, &beta));
         if (o->verbose) fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e\n", it, rinf, beta);
+        // the plain cycle judges the learned stage: a restart that does not cut |r|_2 by the
+        // stagnation ratio (or a non-finite residual) switches to the plain cycle for good
+        if (kind == ALAMO_PRECOND_LEARNED && !(beta <= o->stagnation_ratio * beta_prev)) {
+            kind = ALAMO_PRECOND_MG;
+            fallbacks++;
+            if (!std::isfinite(beta)) {
+                // restart from the last finite iterate is not kept; recompute from sol as is
+                launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
+                TRY(norms_dev<double>(h, V, n, &rinf, &beta));
+            }
+            if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage stagnated, falling back to the cycle\n");
+        }
+        beta_prev = beta;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -496,7 +525,7 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
         st->rhs_inf = bn;
         st->target = target;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
-        st->fallback_events = 0;
+        st->fallback_events = fallbacks;
     }
     if (!std::isfinite(rinf)) FAIL(ALAMO_ERR_NAN, "alamo_krylov_solve: residual is not finite");
     if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_krylov_solve: max_iter reached before the tolerance");
@@ -794,8 +823,9 @@ int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_r
     alamo_krylov_opts o;
     if (opts) o = *opts;
     else alamo_krylov_opts_default(&o);
-    if (o.precond != ALAMO_PRECOND_NONE && o.precond != ALAMO_PRECOND_MG)
+    if (o.precond != ALAMO_PRECOND_NONE && o.precond != ALAMO_PRECOND_MG && o.precond != ALAMO_PRECOND_LEARNED)
         FAIL(ALAMO_ERR_ARG, "unknown preconditioner");
+    if (o.precond == ALAMO_PRECOND_LEARNED && !h->net) FAIL(ALAMO_ERR_STATE, "no network attached (alamo_mg_set_net)");
     return fgmres_impl(h, sol, rhs, tol_rel, tol_abs, &o, stats);
 }
 
@@ -912,3 +942,292 @@ extern "C" int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, c
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
+
+// ===========================================================================
+// Learned correction network: C ABI
+// ===========================================================================
+namespace {
+__global__ void k_pack_target(int64_t V, const double* __restrict__ e, double inv_s, float* __restrict__ t) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (int c = 0; c < 3; c++) t[v * 3 + c] = (float)(e[c * V + v] * inv_s);
+}
+__global__ void k_f32_to_bf16_host_layout(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        b[q] = net::f2bf(a[q]);
+}
+}  // namespace
+
+static const char NET_MAGIC[8] = {'A', 'L', 'A', 'M', 'O', 'N', 'E', 'T'};
+
+extern "C" {
+
+int alamo_net_create(alamo_net** net, uint64_t seed) {
+    if (!net) FAIL(ALAMO_ERR_ARG, "null argument");
+    return net_create(net, seed);
+}
+
+int alamo_net_destroy(alamo_net* n) {
+    if (!n) return ALAMO_OK;
+    (void)hipDeviceSynchronize();
+    net_free_bufs(n->B);
+    for (void* p : n->allocs) (void)hipFree(p);
+    delete n;
+    return ALAMO_OK;
+}
+
+int alamo_net_num_params(alamo_net* n, int64_t* np) {
+    if (!n || !np) FAIL(ALAMO_ERR_ARG, "null argument");
+    *np = n->nparam;
+    return ALAMO_OK;
+}
+
+int alamo_net_get_params(alamo_net* n, float* host) {
+    if (!n || !host) FAIL(ALAMO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(n->s));
+    HIPCHK(hipMemcpy(host, n->w, n->nparam * 4, hipMemcpyDeviceToHost));
+    return ALAMO_OK;
+}
+
+int alamo_net_set_params(alamo_net* n, const float* host) {
+    if (!n || !host) FAIL(ALAMO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(n->s));
+    HIPCHK(hipMemcpy(n->w, host, n->nparam * 4, hipMemcpyHostToDevice));
+    n->wdirty = true;
+    return ALAMO_OK;
+}
+
+int alamo_net_get_grads(alamo_net* n, float* host) {
+    if (!n || !host) FAIL(ALAMO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(n->s));
+    HIPCHK(hipMemcpy(host, n->gw, n->nparam * 4, hipMemcpyDeviceToHost));
+    return ALAMO_OK;
+}
+
+int alamo_net_zero_grads(alamo_net* n) {
+    if (!n) FAIL(ALAMO_ERR_ARG, "null argument");
+    HIPCHK(hipMemsetAsync(n->gw, 0, n->nparam * 4, n->s));
+    return ALAMO_OK;
+}
+
+// Forward on host data: x [V][16] f32 (rounded to bf16), y [V][16] f32.
+int alamo_net_forward_host(alamo_net* n, const int* nnodes, const float* x, float* y) {
+    if (!n || !nnodes || !x || !y) FAIL(ALAMO_ERR_ARG, "null argument");
+    TRY(net_bufs(n, nnodes));
+    const int64_t V = n->B.g[0].V;
+    float* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, V * 16 * 4));
+    HIPCHK(hipMemcpy(tmp, x, V * 16 * 4, hipMemcpyHostToDevice));
+    k_f32_to_bf16_host_layout<<<nb256(V * 16), 256, 0, n->s>>>(tmp, n->B.x, V * 16);
+    int rc = net_forward(n);
+    if (rc == ALAMO_OK) {
+        HIPCHK(hipStreamSynchronize(n->s));
+        HIPCHK(hipMemcpy(y, n->B.y, V * 16 * 4, hipMemcpyDeviceToHost));
+    }
+    (void)hipFree(tmp);
+    return rc;
+}
+
+// Forward + backward on host data; gradients accumulate into the network; dx (may be NULL)
+// receives d(loss)/dx [V][16] for the given dy [V][16].
+int alamo_net_backward_host(alamo_net* n, const int* nnodes, const float* x, const float* dy, float* dx) {
+    if (!n || !nnodes || !x || !dy) FAIL(ALAMO_ERR_ARG, "null argument");
+    TRY(net_bufs(n, nnodes));
+    const int64_t V = n->B.g[0].V;
+    float* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, V * 16 * 4));
+    HIPCHK(hipMemcpy(tmp, x, V * 16 * 4, hipMemcpyHostToDevice));
+    k_f32_to_bf16_host_layout<<<nb256(V * 16), 256, 0, n->s>>>(tmp, n->B.x, V * 16);
+    TRY(net_forward(n));
+    HIPCHK(hipMemcpy(n->B.dy, dy, V * 16 * 4, hipMemcpyHostToDevice));
+    TRY(net_backward(n, dx != nullptr));
+    HIPCHK(hipStreamSynchronize(n->s));
+    if (dx) HIPCHK(hipMemcpy(dx, n->B.dx, V * 16 * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(tmp);
+    return ALAMO_OK;
+}
+
+int alamo_net_adam_step(alamo_net* n, double lr) {
+    if (!n) FAIL(ALAMO_ERR_ARG, "null argument");
+    return net_adam(n, (float)lr);
+}
+
+int alamo_mg_set_net(alamo_mg* h, alamo_net* n) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    h->net = n;
+    if (n) {
+        n->B.N0[0] = n->B.N0[1] = n->B.N0[2] = 0;  // features are rebuilt for this handle
+        net_free_bufs(n->B);
+    }
+    return ALAMO_OK;
+}
+
+int alamo_net_save(alamo_net* n, const char* path) {
+    if (!n || !path) FAIL(ALAMO_ERR_ARG, "null argument");
+    std::vector<float> w(n->nparam);
+    TRY(alamo_net_get_params(n, w.data()));
+    FILE* f = fopen(path, "wb");
+    if (!f) FAIL(ALAMO_ERR_ARG, std::string("cannot open ") + path);
+    const int32_t ver = 1, nl = (int32_t)n->L.size();
+    fwrite(NET_MAGIC, 1, 8, f);
+    fwrite(&ver, 4, 1, f);
+    fwrite(&nl, 4, 1, f);
+    for (auto& ly : n->L) {
+        const int32_t d[5] = {ly.cin, ly.cout, ly.cin_real, ly.level, ly.relu};
+        fwrite(d, 4, 5, f);
+    }
+    fwrite(&n->seed, 8, 1, f);
+    fwrite(&n->train_seed, 8, 1, f);
+    fwrite(&n->nparam, 8, 1, f);
+    fwrite(w.data(), 4, w.size(), f);
+    fclose(f);
+    return ALAMO_OK;
+}
+
+int alamo_net_load(alamo_net* n, const char* path) {
+    if (!n || !path) FAIL(ALAMO_ERR_ARG, "null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) FAIL(ALAMO_ERR_ARG, std::string("cannot open ") + path);
+    char mg[8];
+    int32_t ver = 0, nl = 0;
+    int64_t np = 0;
+    uint64_t s0 = 0, s1 = 0;
+    bool ok = fread(mg, 1, 8, f) == 8 && memcmp(mg, NET_MAGIC, 8) == 0 && fread(&ver, 4, 1, f) == 1 && ver == 1 &&
+              fread(&nl, 4, 1, f) == 1 && nl == (int32_t)n->L.size();
+    for (int l = 0; ok && l < nl; l++) {
+        int32_t d[5];
+        ok = fread(d, 4, 5, f) == 5 && d[0] == n->L[l].cin && d[1] == n->L[l].cout;
+    }
+    ok = ok && fread(&s0, 8, 1, f) == 1 && fread(&s1, 8, 1, f) == 1 && fread(&np, 8, 1, f) == 1 && np == n->nparam;
+    std::vector<float> w(ok ? np : 0);
+    ok = ok && fread(w.data(), 4, np, f) == (size_t)np;
+    fclose(f);
+    if (!ok) FAIL(ALAMO_ERR_ARG, "not a network checkpoint of this architecture");
+    n->seed = s0;
+    n->train_seed = s1;
+    return alamo_net_set_params(n, w.data());
+}
+
+// Generate (residual after one cycle, remaining error) pairs with the handle's operator and
+// train the network on them with Adam.
+int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_train_stats* st) {
+    if (!n || !h || !o) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    if (o->npairs < 1 || o->steps < 1) FAIL(ALAMO_ERR_ARG, "npairs and steps must be positive");
+    if (!h->prepared) TRY(prepare(h));
+    TRY(net_level0(n, h));
+    n->train_seed = o->seed;
+    LevelBuf& l0 = h->L[0];
+    const int64_t V = l0.g.tot, n3 = 3 * V;
+    const int nlev = (int)h->L.size();
+    std::vector<void*> tmp;
+    auto talloc = [&](void** p, size_t b) -> int { return net_alloc(tmp, p, b); };
+    int rc = ALAMO_OK;
+    double *r = nullptr, *z = nullptr, *q = nullptr, *e = nullptr;
+    std::vector<double*> lv(nlev, nullptr);
+    uint16_t* X = nullptr;
+    float* T = nullptr;
+    rc |= talloc((void**)&r, n3 * 8);
+    rc |= talloc((void**)&z, n3 * 8);
+    rc |= talloc((void**)&q, n3 * 8);
+    rc |= talloc((void**)&e, n3 * 8);
+    for (int l = 1; l < nlev; l++) rc |= talloc((void**)&lv[l], 3 * h->L[l].g.tot * 8);
+    rc |= talloc((void**)&X, (size_t)o->npairs * V * 16 * 2);
+    rc |= talloc((void**)&T, (size_t)o->npairs * V * 3 * 4);
+    auto cleanup = [&]() { (void)hipStreamSynchronize(h->stream); for (void* p : tmp) (void)hipFree(p); };
+    if (rc != ALAMO_OK) { cleanup(); return ALAMO_ERR_HIP; }
+    auto t0 = std::chrono::steady_clock::now();
+    alamo_krylov_opts ko;
+    alamo_krylov_opts_default(&ko);
+    ko.precond = ALAMO_PRECOND_MG;
+    ko.max_iter = o->solve_max_iter;
+    int solved = 0;
+    for (int p = 0; p < o->npairs; p++) {
+        // smooth random right-hand side: white noise on a coarse level, prolonged to level 0
+        const int lc = 1 + (int)((o->seed + 7919ull * p) % (uint64_t)std::max(1, std::min(3, nlev - 1)));
+        double* cur = lv[lc];
+        k_noise<<<gs_blocks(3 * h->L[lc].g.tot), 256, 0, h->stream>>>(3 * h->L[lc].g.tot,
+                                                                       o->seed * 1000003ull + p, cur);
+        for (int l = lc - 1; l >= 0; l--) {
+            double* fine = l == 0 ? r : lv[l];
+            vfill<double>(fine, 3 * h->L[l].g.tot, 0.0, h->stream);
+            k_interp_add<double><<<nblocks(h->L[l].g.tot), 256, 0, h->stream>>>(h->L[l + 1].g, h->L[l].g, cur, fine);
+            cur = fine;
+        }
+        // z = V(r), q = r - A z, e = A^-1 q
+        rc = precond_apply(h, ALAMO_PRECOND_MG, r, z);
+        if (rc) break;
+        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, z, r, q, 0.0, h->stream);
+        vfill<double>(e, n3, 0.0, h->stream);
+        alamo_solve_stats ss{};
+        const int src = fgmres_impl(h, e, q, o->solve_tol, 0.0, &ko, &ss);
+        if (src != ALAMO_OK && src != ALAMO_ERR_NOCONV) { rc = src; break; }
+        solved += ss.converged;
+        double ql2 = 0;
+        rc = norms_dev<double>(h, q, n3, nullptr, &ql2);
+        if (rc) break;
+        const double s = std::max(ql2 / std::sqrt((double)n3), 1e-300);
+        net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(V, q, 1.0 / s, n->B.feat, X + (int64_t)p * V * 16);
+        k_pack_target<<<nb256(V), 256, 0, h->stream>>>(V, e, 1.0 / s, T + (int64_t)p * V * 3);
+        if (o->verbose) fprintf(stderr, "alamo train: pair %d generated (solve %d its)\n", p, ss.iterations);
+    }
+    if (rc) { cleanup(); return rc; }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    double first = -1, last = -1;
+    n->s = h->stream;
+    for (int step = 0; step < o->steps && rc == ALAMO_OK; step++) {
+        const int p = (int)(((uint64_t)step * 2654435761ull + o->seed) % (uint64_t)o->npairs);
+        HIPCHK(hipMemcpyAsync(n->B.x, X + (int64_t)p * V * 16, V * 16 * 2, hipMemcpyDeviceToDevice, n->s));
+        rc = net_forward(n);
+        if (rc) break;
+        HIPCHK(hipMemsetAsync(n->B.loss, 0, 4, n->s));
+        net::k_mse<<<(unsigned)std::min<int64_t>(1024, (V + 255) / 256), 256, 0, n->s>>>(
+            V, n->B.y, T + (int64_t)p * V * 3, n->B.dy, n->B.loss);
+        rc = net_backward(n, false);
+        if (rc) break;
+        const bool rep = step == 0 || step == o->steps - 1 || (o->verbose && step % 50 == 0);
+        if (rep) {
+            float lv_ = 0;
+            HIPCHK(hipMemcpyAsync(&lv_, n->B.loss, 4, hipMemcpyDeviceToHost, n->s));
+            HIPCHK(hipStreamSynchronize(n->s));
+            if (step == 0) first = lv_;
+            last = lv_;
+            if (o->verbose) fprintf(stderr, "alamo train: step %d loss %.6e\n", step, lv_);
+        }
+        rc = net_adam(n, (float)o->lr);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto t2 = std::chrono::steady_clock::now();
+    cleanup();
+    if (st) {
+        st->pairs = o->npairs;
+        st->pairs_converged = solved;
+        st->steps = o->steps;
+        st->first_loss = first;
+        st->last_loss = last;
+        st->seconds_data = std::chrono::duration<double>(t1 - t0).count();
+        st->seconds_train = std::chrono::duration<double>(t2 - t1).count();
+    }
+    return rc;
+}
+
+// MFMA lane-layout probe (one 16x16x16 bf16 product of host matrices), for the tests.
+int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
+    uint16_t *a = nullptr, *b = nullptr;
+    float* d = nullptr;
+    HIPCHK(hipMalloc(&a, 512));
+    HIPCHK(hipMalloc(&b, 512));
+    HIPCHK(hipMalloc(&d, 1024));
+    HIPCHK(hipMemcpy(a, A, 512, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b, B, 512, hipMemcpyHostToDevice));
+    net::k_mfma_probe<<<1, 64>>>(a, b, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(D, d, 1024, hipMemcpyDeviceToHost));
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(d);
+    return ALAMO_OK;
+}
+
+}  // extern "C"

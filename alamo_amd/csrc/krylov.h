@@ -12,34 +12,53 @@
 
 namespace alamo {
 
-constexpr int KRY_MAXV = 32;  // largest restart length + 1
+constexpr int KRY_MAXV = 32;         // largest restart length + 1
+constexpr int KRY_DOT_BLOCKS = 4096;  // workgroups of the multi-dot pass (enough to saturate HBM)
 
-// partial[i * nb + block] = sum over the block's share of w . V_i, i < nv
+// partial[i * nb + block] = sum over the block's share of w . V_i, i < NV.  NV is a
+// compile-time count so the NV accumulators stay in registers.
+template <int NV>
 __global__ void __launch_bounds__(RED_BLOCK) k_multidot(const double* __restrict__ w, const double* __restrict__ V,
-                                                        int64_t n, int64_t ldv, int nv,
-                                                        double* __restrict__ partial) {
-    double acc[KRY_MAXV];
+                                                        int64_t n, int64_t ldv, double* __restrict__ partial) {
+    double acc[NV];
 #pragma unroll
-    for (int i = 0; i < KRY_MAXV; i++) acc[i] = 0.0;
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         const double wq = w[q];
 #pragma unroll
-        for (int i = 0; i < KRY_MAXV; i++)
-            if (i < nv) acc[i] += wq * V[i * ldv + q];
+        for (int i = 0; i < NV; i++) acc[i] += wq * __builtin_nontemporal_load(V + i * ldv + q);
     }
-    __shared__ double sh[RED_BLOCK / 64];
+    __shared__ double sh[NV][RED_BLOCK / 64];
 #pragma unroll
-    for (int i = 0; i < KRY_MAXV; i++) {
-        if (i >= nv) break;
-        double v = wave_sum(acc[i]);
-        if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int q = 0; q < RED_BLOCK / 64; q++) s += sh[q];
-            partial[i * gridDim.x + blockIdx.x] = s;
+    for (int i = 0; i < NV; i++) {
+        const double v = wave_sum(acc[i]);
+        if (threadIdx.x % 64 == 0) sh[i][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int q = 0; q < RED_BLOCK / 64; q++) s += sh[threadIdx.x][q];
+        partial[threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Launch k_multidot over nv <= KRY_MAXV vectors in register-sized groups of at most 8.
+inline void launch_multidot(const double* w, const double* V, int64_t n, int64_t ldv, int nv, int nb,
+                            double* partial, hipStream_t s) {
+    for (int i0 = 0; i0 < nv; i0 += 8) {
+        const int k = nv - i0 < 8 ? nv - i0 : 8;
+        const double* Vi = V + i0 * ldv;
+        double* P = partial + (int64_t)i0 * nb;
+        switch (k) {
+            case 1: k_multidot<1><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 2: k_multidot<2><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 3: k_multidot<3><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 4: k_multidot<4><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 5: k_multidot<5><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 6: k_multidot<6><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 7: k_multidot<7><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            default: k_multidot<8><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
         }
-        __syncthreads();
     }
 }
 

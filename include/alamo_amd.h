@@ -38,6 +38,7 @@ enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, 
        ALAMO_ERR_NAN = 5 };
 
 typedef struct alamo_mg alamo_mg;
+typedef struct alamo_net alamo_net;
 
 /* Geometry, boundary table and solver options of one hierarchy.  Field names
  * follow the reference's input keys (solver.* of Solver/Nonlocal/Linear.H:296-382,
@@ -151,6 +152,44 @@ typedef struct {
 int alamo_krylov_opts_default(alamo_krylov_opts* o);
 int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        const alamo_krylov_opts* opts, alamo_solve_stats* stats);
+
+/* Learned correction network (no reference counterpart).  A 3-level convolutional network
+ * (16/32/64 channels, 3x3x3 kernels, BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) that maps the residual left by one multigrid cycle, together with log(lambda),
+ * log(mu), to that cycle's remaining error.  Used by alamo_krylov_solve with
+ * ALAMO_PRECOND_LEARNED: z = cycle(r); z += net(r - A z); the plain cycle takes over for good
+ * when a restart reduces |r|_2 by less than opts.stagnation_ratio.  Isotropic operators only;
+ * level-0 grids need (n+1-1) divisible by 4. */
+typedef struct {
+    int npairs;                /* (residual, error) pairs generated with the handle's operator */
+    int steps;                 /* Adam steps (one pair per step) */
+    double lr;                 /* Adam learning rate */
+    uint64_t seed;             /* training-set seed (stored in checkpoints) */
+    int solve_max_iter;        /* FGMRES iterations allowed for each pair's error */
+    double solve_tol;          /* relative tolerance of those solves */
+    int verbose;
+} alamo_train_opts;
+typedef struct {
+    int pairs, pairs_converged, steps;
+    double first_loss, last_loss, seconds_data, seconds_train;
+} alamo_train_stats;
+int alamo_net_create(alamo_net** net, uint64_t seed);
+int alamo_net_destroy(alamo_net* net);
+int alamo_net_num_params(alamo_net* net, int64_t* nparam);
+int alamo_net_get_params(alamo_net* net, float* host);
+int alamo_net_set_params(alamo_net* net, const float* host);
+int alamo_net_get_grads(alamo_net* net, float* host);
+int alamo_net_zero_grads(alamo_net* net);
+/* x, y, dy, dx: host [V][16] float arrays on a grid of nnodes[3] nodes (inputs rounded to bf16) */
+int alamo_net_forward_host(alamo_net* net, const int* nnodes, const float* x, float* y);
+int alamo_net_backward_host(alamo_net* net, const int* nnodes, const float* x, const float* dy, float* dx);
+int alamo_net_adam_step(alamo_net* net, double lr);
+int alamo_net_train(alamo_net* net, alamo_mg* h, const alamo_train_opts* opts, alamo_train_stats* stats);
+int alamo_net_save(alamo_net* net, const char* path);
+int alamo_net_load(alamo_net* net, const char* path);
+int alamo_mg_set_net(alamo_mg* h, alamo_net* net);
+/* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
+int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+// C++ façade over the C ABI of libalamo_amd.so, mirroring the reference's class and
+// method names so that code written against
+//   Operator::Elastic<SYM>             (src/Operator/Elastic.H:21-190)
+//   Operator::Operator<Grid::Node>     (src/Operator/Operator.H:27-125)
+//   Solver::Nonlocal::Linear           (src/Solver/Nonlocal/Linear.H:18-389)
+//   BC::Operator::Elastic::Constant    (src/BC/Operator/Elastic/Constant.H:31-649)
+// keeps its call sequence (construct, SetBC, SetModel, solver.Define, solver.solve, Clear).
+// Fields are device pointers to the single-box nodal layout described in alamo_amd.h.
+// Where the reference calls Util::Abort, this header throws alamo::Abort.
+#ifndef ALAMO_AMD_HPP
+#define ALAMO_AMD_HPP
+
+#include <array>
+#include <stdexcept>
+#include <string>
+
+#include "alamo_amd.h"
+
+namespace alamo {
+struct Abort : std::runtime_error {
+    int code;
+    Abort(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+inline void check(int rc) {
+    if (rc != ALAMO_OK) throw Abort(rc, alamo_last_error());
+}
+}  // namespace alamo
+
+namespace BC {
+namespace Operator {
+namespace Elastic {
+// BC::Operator::Elastic::Constant: 26 regions x 3 components of (type, value)
+class Constant {
+public:
+    enum Type { Displacement = ALAMO_BC_DISPLACEMENT, Traction = ALAMO_BC_TRACTION, Periodic = ALAMO_BC_PERIODIC,
+                Neumann = ALAMO_BC_NEUMANN, None = ALAMO_BC_NONE };
+    enum Face { XLO, YLO, ZLO, XHI, YHI, ZHI, YLO_ZLO, YLO_ZHI, YHI_ZLO, YHI_ZHI, ZLO_XLO, ZLO_XHI, ZHI_XLO,
+                ZHI_XHI, XLO_YLO, XLO_YHI, XHI_YLO, XHI_YHI, XLO_YLO_ZLO, XLO_YLO_ZHI, XLO_YHI_ZLO, XLO_YHI_ZHI,
+                XHI_YLO_ZLO, XHI_YLO_ZHI, XHI_YHI_ZLO, XHI_YHI_ZHI, INT };
+    enum Direction { X = 0, Y = 1, Z = 2 };
+    Constant() {
+        for (int f = 0; f < 26; f++)
+            for (int d = 0; d < 3; d++) { type[f][d] = Displacement; val[f][d] = 0.0; }
+    }
+    void Set(Face face, Direction dir, Type t, double value) {
+        type[face][dir] = t;
+        val[face][dir] = value;
+    }
+    int type[26][3];
+    double val[26][3];
+};
+}  // namespace Elastic
+}  // namespace Operator
+}  // namespace BC
+
+namespace Operator {
+
+enum SymKind { Isotropic = ALAMO_SYM_ISOTROPIC, MajorMinor = ALAMO_SYM_MAJORMINOR, Diagonal = ALAMO_SYM_DIAGONAL };
+
+// Operator::Elastic<SYM> over one nodal box [0, n_cell] with cell size dx.
+template <int SYM>
+class Elastic {
+public:
+    Elastic(const std::array<int, 3>& n_cell, const std::array<double, 3>& dx, int max_coarsening_level = -1) {
+        alamo::check(alamo_mg_desc_default(&desc_));
+        for (int d = 0; d < 3; d++) { desc_.n_cell[d] = n_cell[d]; desc_.dx[d] = dx[d]; }
+        desc_.sym = SYM;
+        desc_.max_coarsening_level = max_coarsening_level;
+    }
+    ~Elastic() { if (h_) alamo_mg_destroy(h_); }
+    Elastic(const Elastic&) = delete;
+    Elastic& operator=(const Elastic&) = delete;
+
+    // Elastic::SetBC (Elastic.H:61-65); must precede SetModel, which builds the hierarchy
+    void SetBC(const BC::Operator::Elastic::Constant* bc) {
+        for (int f = 0; f < 26; f++)
+            for (int d = 0; d < 3; d++) { desc_.bc_type[f][d] = bc->type[f][d]; desc_.bc_val[f][d] = bc->val[f][d]; }
+    }
+    void SetOmega(double omega) { desc_.omega = omega; }                  // Operator.H:57
+    void SetPrecision(int p) { desc_.precision = p; }
+    // Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116); C: device, AoS Matrix4 records
+    void SetModel(int amrlev, const double* C, int layout = ALAMO_LAYOUT_AOS) {
+        ensure();
+        alamo::check(alamo_mg_set_model(h_, amrlev, C, layout));
+    }
+    void prepareForSolve() { ensure(); alamo::check(alamo_mg_prepare(h_)); }
+    // Operator<Grid::Node>::Apply -> Elastic::Fapply (Operator.H:55, Elastic.cpp:139-292)
+    void Apply(int amrlev, int mglev, double* out, const double* in) const {
+        alamo::check(alamo_mg_fapply(h_, amrlev, mglev, out, in, nullptr));
+    }
+    void Fsmooth(int amrlev, int mglev, double* x, const double* b) const {
+        alamo::check(alamo_mg_fsmooth(h_, amrlev, mglev, x, b, nullptr));
+    }
+    void Diagonal(int amrlev, int mglev, double* diag) const {
+        alamo::check(alamo_mg_diagonal(h_, amrlev, mglev, diag, nullptr));
+    }
+    int NMGLevels() const { int n = 0; alamo::check(alamo_mg_num_levels(h_, &n)); return n; }
+    alamo_mg* handle() { ensure(); return h_; }
+    alamo_mg_desc& desc() { return desc_; }
+
+private:
+    void ensure() {
+        if (!h_) alamo::check(alamo_mg_create(&desc_, &h_));
+    }
+    alamo_mg_desc desc_{};
+    alamo_mg* h_ = nullptr;
+};
+
+}  // namespace Operator
+
+namespace Solver {
+namespace Nonlocal {
+// Solver::Nonlocal::Linear (Linear.H:18-389) over Operator::Elastic<SYM>
+template <int SYM>
+class Linear {
+public:
+    Linear() = default;
+    explicit Linear(::Operator::Elastic<SYM>& op) { Define(op); }
+    void Define(::Operator::Elastic<SYM>& op) {
+        if (linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver cannot be re-defined");
+        linop_ = &op;
+    }
+    void Clear() {
+        if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver cannot be cleared if not defined");
+        linop_ = nullptr;
+    }
+    void setMaxIter(int n) { linop_->desc().max_iter = n; }
+    void setPreSmooth(int n) { linop_->desc().pre_smooth = n; }
+    void setPostSmooth(int n) { linop_->desc().post_smooth = n; }
+    // Linear::solve (Linear.H:131-147): sol (initial guess in, solution out), rhs; returns |r|_inf
+    double solve(double* sol, const double* rhs, double tol_rel, double tol_abs) {
+        if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver not defined");
+        alamo_solve_stats st{};
+        alamo::check(alamo_linear_solve(linop_->handle(), sol, rhs, tol_rel, tol_abs, &st));
+        last_ = st;
+        return st.final_res_inf;
+    }
+    const alamo_solve_stats& stats() const { return last_; }
+
+private:
+    ::Operator::Elastic<SYM>* linop_ = nullptr;
+    alamo_solve_stats last_{};
+};
+}  // namespace Nonlocal
+}  // namespace Solver
+
+#endif

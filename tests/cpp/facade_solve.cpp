@@ -1,0 +1,47 @@
+// Drives the library through the C++ façade exactly as Integrator::Base::Mechanics drives the
+// reference (Base/Mechanics.H:193-208): build the operator, SetBC, SetModel, Define, solve, Clear.
+// Writes the solution (3 SoA planes, float64) to argv[1] and prints iterations and |r|inf.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "alamo_amd.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 2) { fprintf(stderr, "usage: facade_solve out.bin\n"); return 2; }
+    const int n = 16;
+    const int N = (n + 1) * (n + 1) * (n + 1);
+    BC::Operator::Elastic::Constant bc;  // Dirichlet 0 everywhere (Constant.H:36-45)
+    bc.Set(BC::Operator::Elastic::Constant::XHI, BC::Operator::Elastic::Constant::X,
+           BC::Operator::Elastic::Constant::Displacement, 0.01);
+    Operator::Elastic<Operator::Isotropic> op({n, n, n}, {1.0 / n, 1.0 / n, 1.0 / n});
+    op.SetBC(&bc);
+    std::vector<double> C(2 * N), b(3 * N, 0.0), x(3 * N, 0.0);
+    for (int m = 0; m < N; m++) {
+        const int i = m % (n + 1);
+        C[2 * m] = 1.0 + 0.5 * i / n;  // lambda
+        C[2 * m + 1] = 1.0;            // mu
+    }
+    double *dC, *db, *dx;
+    if (hipMalloc(&dC, C.size() * 8) || hipMalloc(&db, b.size() * 8) || hipMalloc(&dx, x.size() * 8)) return 3;
+    hipMemcpy(dC, C.data(), C.size() * 8, hipMemcpyHostToDevice);
+    hipMemcpy(dx, x.data(), x.size() * 8, hipMemcpyHostToDevice);
+    try {
+        op.SetModel(0, dC);
+        alamo::check(alamo_bc_init(op.handle(), db, 0));  // BC::Operator::Elastic::Constant::Init
+        Solver::Nonlocal::Linear<Operator::Isotropic> solver;
+        solver.Define(op);
+        double r = solver.solve(dx, db, 1e-12, 0.0);
+        printf("iterations %d res_inf %.3e converged %d\n", solver.stats().iterations, r, solver.stats().converged);
+        solver.Clear();
+    } catch (const alamo::Abort& e) {
+        fprintf(stderr, "abort: %s\n", e.what());
+        return 1;
+    }
+    hipMemcpy(x.data(), dx, x.size() * 8, hipMemcpyDeviceToHost);
+    FILE* f = fopen(argv[1], "wb");
+    fwrite(x.data(), 8, x.size(), f);
+    fclose(f);
+    return 0;
+}
