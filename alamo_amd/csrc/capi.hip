@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -23,6 +24,7 @@
 using namespace alamo;
 
 static thread_local std::string g_err;
+static std::atomic<uint64_t> g_model_version{0};
 
 #define FAIL(code, msg)            \
     do {                           \
@@ -73,6 +75,7 @@ struct alamo_mg {
     int* bottom_iters = nullptr;
     double* sol_res = nullptr;      // level-0 FP64 residual for the mixed-precision outer loop
     bool model_set = false, prepared = false;
+    uint64_t model_version = 0;  // unique per SetModel call (the network's features follow it)
     int nblk_red = 1024;
     // flexible GMRES workspace (allocated on first use)
     double* kV = nullptr;     // (m+1) basis vectors
@@ -493,21 +496,28 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
             for (int k = i + 1; k < jend; k++) s -= H[i][k] * y[k];
             y[i] = H[i][i] != 0.0 ? s / H[i][i] : 0.0;
         }
+        bool finite = true;
+        for (int i = 0; i < jend; i++) finite = finite && std::isfinite(y[i]);
+        if (!finite && kind == ALAMO_PRECOND_LEARNED) {
+            // a non-finite learned stage leaves the iterate untouched; the plain cycle takes over
+            kind = ALAMO_PRECOND_MG;
+            fallbacks++;
+            launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
+            TRY(norms_dev<double>(h, V, n, &rinf, &beta));
+            if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
+            beta_prev = beta;
+            continue;
+        }
         TRY(upload_coef(h, y, jend));
         k_multiaxpy_add<<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
         TRY(norms_dev<double>(h, V, n, &rinf, &beta));
         if (o->verbose) fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e\n", it, rinf, beta);
         // the plain cycle judges the learned stage: a restart that does not cut |r|_2 by the
-        // stagnation ratio (or a non-finite residual) switches to the plain cycle for good
+        // stagnation ratio switches to the plain cycle for good
         if (kind == ALAMO_PRECOND_LEARNED && !(beta <= o->stagnation_ratio * beta_prev)) {
             kind = ALAMO_PRECOND_MG;
             fallbacks++;
-            if (!std::isfinite(beta)) {
-                // restart from the last finite iterate is not kept; recompute from sol as is
-                launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
-                TRY(norms_dev<double>(h, V, n, &rinf, &beta));
-            }
             if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage stagnated, falling back to the cycle\n");
         }
         beta_prev = beta;
@@ -694,6 +704,7 @@ static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, 
     }
     h->model_set = true;
     h->prepared = false;
+    h->model_version = ++g_model_version;
     return ALAMO_OK;
 }
 
@@ -1055,10 +1066,6 @@ int alamo_net_adam_step(alamo_net* n, double lr) {
 int alamo_mg_set_net(alamo_mg* h, alamo_net* n) {
     if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
     h->net = n;
-    if (n) {
-        n->B.N0[0] = n->B.N0[1] = n->B.N0[2] = 0;  // features are rebuilt for this handle
-        net_free_bufs(n->B);
-    }
     return ALAMO_OK;
 }
 
@@ -1115,6 +1122,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
     if (o->npairs < 1 || o->steps < 1) FAIL(ALAMO_ERR_ARG, "npairs and steps must be positive");
     if (!h->prepared) TRY(prepare(h));
+    if (h->L.size() < 2) FAIL(ALAMO_ERR_ARG, "training needs at least two multigrid levels");
     TRY(net_level0(n, h));
     n->train_seed = o->seed;
     LevelBuf& l0 = h->L[0];

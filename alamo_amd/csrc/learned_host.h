@@ -46,6 +46,8 @@ struct alamo_net {
     int adam_t = 0;
     uint64_t seed = 0, train_seed = 0;
     bool wdirty = true;
+    const alamo_mg* feat_owner = nullptr;  // handle and model version the features were built from
+    uint64_t feat_version = 0;
     std::vector<void*> allocs;
 };
 
@@ -310,10 +312,15 @@ static int net_features(alamo_net* n, alamo_mg* h) {
 
 static int net_level0(alamo_net* n, alamo_mg* h) {
     const int N0[3] = {h->L[0].g.N[0], h->L[0].g.N[1], h->L[0].g.N[2]};
-    const bool fresh = !(n->B.N0[0] == N0[0] && n->B.N0[1] == N0[1] && n->B.N0[2] == N0[2]);
+    const bool fresh = !(n->B.N0[0] == N0[0] && n->B.N0[1] == N0[1] && n->B.N0[2] == N0[2]) ||
+                       n->feat_owner != h || n->feat_version != h->model_version;
     TRY(net_bufs(n, N0));
     n->s = h->stream;
-    if (fresh) TRY(net_features(n, h));
+    if (fresh) {
+        TRY(net_features(n, h));
+        n->feat_owner = h;
+        n->feat_version = h->model_version;
+    }
     return ALAMO_OK;
 }
 

@@ -204,3 +204,84 @@ class Krylov:
         out = st.as_dict()
         out["rc"] = rc
         return out
+
+
+class Net:
+    """The learned correction network (no reference counterpart).  Parameters live on the GPU in
+    libalamo_amd.so; host arrays here are float32 numpy arrays of the library's layout."""
+
+    def __init__(self, seed=0):
+        h = ctypes.c_void_p()
+        check(lib().alamo_net_create(ctypes.byref(h), ctypes.c_uint64(seed)))
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                lib().alamo_net_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def num_params(self):
+        n = ctypes.c_int64()
+        check(lib().alamo_net_num_params(self.h, ctypes.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        check(lib().alamo_net_get_params(self.h, p.ctypes.data_as(ctypes.c_void_p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("expected %d parameters" % self.num_params())
+        check(lib().alamo_net_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def get_grads(self):
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        check(lib().alamo_net_get_grads(self.h, g.ctypes.data_as(ctypes.c_void_p)))
+        return g
+
+    def zero_grads(self):
+        check(lib().alamo_net_zero_grads(self.h))
+
+    def forward(self, nnodes, x):
+        nn = np.ascontiguousarray(nnodes, dtype=np.int32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((int(np.prod(nn)), 16), dtype=np.float32)
+        check(lib().alamo_net_forward_host(self.h, nn.ctypes.data_as(ctypes.c_void_p), x.ctypes.data_as(ctypes.c_void_p),
+                                           y.ctypes.data_as(ctypes.c_void_p)))
+        return y
+
+    def backward(self, nnodes, x, dy, want_dx=True):
+        nn = np.ascontiguousarray(nnodes, dtype=np.int32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        dx = np.zeros((int(np.prod(nn)), 16), dtype=np.float32) if want_dx else None
+        check(lib().alamo_net_backward_host(self.h, nn.ctypes.data_as(ctypes.c_void_p),
+                                            x.ctypes.data_as(ctypes.c_void_p), dy.ctypes.data_as(ctypes.c_void_p),
+                                            dx.ctypes.data_as(ctypes.c_void_p) if want_dx else None))
+        return dx
+
+    def adam_step(self, lr):
+        check(lib().alamo_net_adam_step(self.h, float(lr)))
+
+    def train(self, op, npairs=8, steps=200, lr=1e-3, seed=1, solve_max_iter=60, solve_tol=1e-6, verbose=0):
+        o = _lib.TrainOpts(npairs, steps, lr, seed, solve_max_iter, solve_tol, verbose)
+        st = _lib.TrainStats()
+        check(lib().alamo_net_train(self.h, op.h, ctypes.byref(o), ctypes.byref(st)))
+        return st.as_dict()
+
+    def save(self, path):
+        check(lib().alamo_net_save(self.h, str(path).encode()))
+
+    def load(self, path):
+        check(lib().alamo_net_load(self.h, str(path).encode()))
+
+    def attach(self, op):
+        """use this network for op's ALAMO_PRECOND_LEARNED solves (keeps a reference to it)"""
+        check(lib().alamo_mg_set_net(op.h, self.h))
+        op._net = self

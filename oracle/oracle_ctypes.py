@@ -184,3 +184,124 @@ def normal(face):
     n = np.zeros(3)
     lib().ora_normal(face, _d(n))
     return n
+
+
+# ---------------------------------------------------------------------------
+# learned correction network (oracle/conv_oracle.cpp)
+# ---------------------------------------------------------------------------
+NET_LAYERS = [(16, 16, 1), (16, 32, 1), (32, 64, 1), (64, 32, 0), (32, 16, 1), (16, 16, 0)]
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def net_nparam():
+    f = lib().ocv_nparam
+    f.restype = ctypes.c_int64
+    return int(f())
+
+
+def bf16(x):
+    f = lib().ocv_bf16
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double]
+    return np.vectorize(lambda v: f(float(v)))(np.asarray(x, dtype=np.float64))
+
+
+def conv3(N, cin, cout, x, W, bias=None):
+    """out[v][co] = bias + sum_{o,ci} W[o][co][ci] x[v+off(o)][ci]; x [V][cin]"""
+    N = _c(N, np.int32)
+    x = _c(x)
+    W = _c(W)
+    out = np.zeros((int(np.prod(N)), cout))
+    b = _c(bias) if bias is not None else None
+    lib().ocv_conv3(_i(N), cin, cout, _d(x), _d(W), _d(b) if b is not None else None, _d(out))
+    return out
+
+
+def conv3_T(N, cin, cout, dout, W):
+    N = _c(N, np.int32)
+    dout = _c(dout)
+    W = _c(W)
+    out = np.zeros((int(np.prod(N)), cin))
+    lib().ocv_conv3_T(_i(N), cin, cout, _d(dout), _d(W), _d(out))
+    return out
+
+
+def conv3_wgrad(N, cin, cout, x, dout):
+    N = _c(N, np.int32)
+    x = _c(x)
+    dout = _c(dout)
+    dW = np.zeros((27, cout, cin))
+    dB = np.zeros(cout)
+    lib().ocv_conv3_wgrad(_i(N), cin, cout, _d(x), _d(dout), _d(dW), _d(dB))
+    return dW, dB
+
+
+def _coarse(N):
+    return np.array([(n - 1) // 2 + 1 for n in N], dtype=np.int32)
+
+
+def net_prolong(Nf, C, c):
+    Nf = _c(Nf, np.int32)
+    c = _c(c)
+    out = np.zeros((int(np.prod(Nf)), C))
+    lib().ocv_prolong(_i(Nf), C, _d(c), _d(out))
+    return out
+
+
+def net_prolong_T(Nf, C, gu):
+    Nf = _c(Nf, np.int32)
+    gu = _c(gu)
+    out = np.zeros((int(np.prod(_coarse(Nf))), C))
+    lib().ocv_prolong_T(_i(Nf), C, _d(gu), _d(out))
+    return out
+
+
+def net_inject(Nf, C, a):
+    Nf = _c(Nf, np.int32)
+    a = _c(a)
+    out = np.zeros((int(np.prod(_coarse(Nf))), C))
+    lib().ocv_inject(_i(Nf), C, _d(a), _d(out))
+    return out
+
+
+def net_inject_T(Nf, C, gd):
+    Nf = _c(Nf, np.int32)
+    gd = _c(gd)
+    out = np.zeros((int(np.prod(Nf)), C))
+    lib().ocv_inject_T(_i(Nf), C, _d(gd), _d(out))
+    return out
+
+
+def net_forward(N0, params, x, bf16=True):
+    """y [V][16] of the network with float32 params (library layout) on input x [V][16]"""
+    N0 = _c(N0, np.int32)
+    p = _c(params, np.float32)
+    x = _c(x, np.float32)
+    y = np.zeros((int(np.prod(N0)), 16))
+    lib().ocv_net_forward(_i(N0), _f(p), _f(x), _d(y), int(bf16))
+    return y
+
+
+def net_backward(N0, params, x, dy, bf16=True, want_dx=True):
+    """(grads [nparam], dx [V][16]) of <y(x), dy>"""
+    N0 = _c(N0, np.int32)
+    p = _c(params, np.float32)
+    x = _c(x, np.float32)
+    dy = _c(dy, np.float32)
+    g = np.zeros(net_nparam())
+    dx = np.zeros((int(np.prod(N0)), 16)) if want_dx else None
+    lib().ocv_net_backward(_i(N0), _f(p), _f(x), _f(dy), _d(g), _d(dx) if want_dx else None, int(bf16))
+    return g, dx
+
+
+def net_layer_slices():
+    """[(W slice, W shape, bias slice)] of each layer in the parameter vector"""
+    out, off = [], 0
+    for cin, cout, _ in NET_LAYERS:
+        nw = 27 * cin * cout
+        out.append((slice(off, off + nw), (27, cout, cin), slice(off + nw, off + nw + cout)))
+        off += nw + cout
+    return out

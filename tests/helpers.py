@@ -43,3 +43,22 @@ def to_host(t):
     import torch
     torch.cuda.synchronize()
     return t.detach().cpu().numpy()
+
+
+def eshelby_rhs(ora, n):
+    """Eshelby inclusion problem of n^3 cells and its Newton right-hand side (oracle-built)."""
+    pr = P.eshelby_problem(n)
+    bct, bcv = P.bc_constant()
+    N = (n + 1) ** 3
+    b = ora.bc_init(pr["ncell"], bct, bcv)
+    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, np.zeros(3 * N), b)
+    return pr, (bct, bcv), rhs
+
+
+def two_phase_rhs(ora, n, corr_cells=4, contrast=1000.0, seed=7):
+    """Two-phase 1:contrast problem of n^3 cells and its Newton right-hand side (oracle-built)."""
+    pr = P.two_phase_problem(n, corr_cells=corr_cells, contrast=contrast, seed=seed)
+    N = (n + 1) ** 3
+    b = ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"])
+    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], pr["bct"], np.zeros(3 * N), b)
+    return pr, (pr["bct"], pr["bcv"]), rhs

@@ -1,0 +1,194 @@
+"""GPU parity of the learned correction network against its oracle (oracle/conv_oracle.cpp).
+
+The device stores activations, weights and the gradients fed to convolutions in bfloat16 and
+accumulates in FP32; the oracle rounds to bfloat16 at the same points and accumulates in FP64,
+so the two differ only by accumulation order (plus the rare bfloat16 rounding that this
+flips).  Tolerance: relative L2 error 2e-3 on outputs and 5e-3 on gradients, far below the
+O(1) error of a transposed operand, a wrong offset or a missing skip connection.
+"""
+import numpy as np
+import pytest
+
+from alamo_amd import problems as P
+from alamo_amd.operator import Elastic, Krylov, Net, PRECOND_LEARNED, PRECOND_MG, CYCLE_V, PREC_MIXED
+from alamo_amd._lib import AlamoError
+from helpers import eshelby_rhs, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(a):
+    """round-to-nearest-even float32 -> bfloat16 bit patterns"""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _bf16_val(bits):
+    return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _probe(A, B):
+    from alamo_amd._lib import lib, check
+    import ctypes
+    D = np.zeros((16, 16), dtype=np.float32)
+    check(lib().alamo_net_mfma_probe(A.ctypes.data_as(ctypes.c_void_p), B.ctypes.data_as(ctypes.c_void_p),
+                                      D.ctypes.data_as(ctypes.c_void_p)))
+    return D
+
+
+def test_mfma_lane_layout(cuda):
+    """The operand/result lane layout the convolution kernels assume for
+    v_mfma_f32_16x16x16_bf16: integer-valued products are exact, so D must equal A @ B exactly."""
+    r = np.random.default_rng(0)
+    A = r.integers(-8, 9, (16, 16)).astype(np.float32)
+    B = r.integers(-8, 9, (16, 16)).astype(np.float32)
+    D = _probe(_bf16_bits(A), _bf16_bits(B))
+    np.testing.assert_array_equal(D, (A.astype(np.float64) @ B.astype(np.float64)).astype(np.float32))
+    # general values: FP32 accumulation of exact bf16 products
+    A = r.standard_normal((16, 16)).astype(np.float32)
+    B = r.standard_normal((16, 16)).astype(np.float32)
+    Ab, Bb = _bf16_bits(A), _bf16_bits(B)
+    ref = _bf16_val(Ab) @ _bf16_val(Bb)
+    np.testing.assert_allclose(_probe(Ab, Bb), ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def _random_params(ora, seed):
+    r = np.random.default_rng(seed)
+    p = np.zeros(ora.net_nparam(), dtype=np.float32)
+    for (cin, cout, _), (ws, _, bs) in zip(ora.NET_LAYERS, ora.net_layer_slices()):
+        p[ws] = r.standard_normal(ws.stop - ws.start) * np.sqrt(2.0 / (27 * cin))
+        p[bs] = 0.1 * r.standard_normal(cout)
+    return p
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+# x rows of 33 and 9 nodes: whole and ragged 16-voxel tiles of the convolution kernels
+GRIDS = [(9, 13, 5), (33, 17, 9)]
+
+
+@pytest.mark.parametrize("N0", GRIDS)
+def test_net_forward_parity(cuda, ora, N0):
+    V = int(np.prod(N0))
+    p = _random_params(ora, 11)
+    x = np.random.default_rng(12).standard_normal((V, 16)).astype(np.float32)
+    net = Net(seed=1)
+    net.set_params(p)
+    print("net forward", N0, flush=True)
+    y = net.forward(N0, x)
+    yref = ora.net_forward(N0, p, x, bf16=True)
+    err = _rel(y, yref)
+    print("forward rel L2 err", err)
+    assert np.all(np.isfinite(y)) and err <= 2e-3
+    assert np.abs(y - yref).max() <= 2e-2 * np.abs(yref).max()
+
+
+@pytest.mark.parametrize("N0", GRIDS)
+def test_net_backward_parity(cuda, ora, N0):
+    V = int(np.prod(N0))
+    p = _random_params(ora, 21)
+    r = np.random.default_rng(22)
+    x = r.standard_normal((V, 16)).astype(np.float32)
+    dy = r.standard_normal((V, 16)).astype(np.float32)
+    net = Net(seed=1)
+    net.set_params(p)
+    net.zero_grads()
+    print("net backward", N0, flush=True)
+    dx = net.backward(N0, x, dy)
+    g = net.get_grads()
+    gref, dxref = ora.net_backward(N0, p, x, dy, bf16=True)
+    for l, (ws, _, bs) in enumerate(ora.net_layer_slices()):
+        ew, eb = _rel(g[ws], gref[ws]), _rel(g[bs], gref[bs])
+        print("layer", l, "dW rel err", ew, "dB rel err", eb)
+        assert ew <= 5e-3 and eb <= 5e-3, l
+    assert _rel(dx, dxref) <= 5e-3
+
+
+def test_adam_matches_torch(cuda, ora):
+    """Two Adam steps on the device equal torch.optim.Adam (CPU, float32) on the same gradients."""
+    import torch
+    N0 = (9, 9, 9)
+    V = int(np.prod(N0))
+    p0 = _random_params(ora, 31)
+    r = np.random.default_rng(32)
+    x = r.standard_normal((V, 16)).astype(np.float32)
+    net = Net(seed=1)
+    net.set_params(p0)
+    net.zero_grads()
+    tp = torch.nn.Parameter(torch.from_numpy(p0.copy()))
+    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for step in range(2):
+        dy = r.standard_normal((V, 16)).astype(np.float32)
+        net.backward(N0, x, dy, want_dx=False)
+        g = net.get_grads()
+        net.adam_step(1e-2)
+        tp.grad = torch.from_numpy(g.copy())
+        opt.step()
+    np.testing.assert_allclose(net.get_params(), tp.detach().numpy(), rtol=1e-5, atol=1e-7)
+    assert np.all(net.get_grads() == 0)  # the step clears the gradients
+
+
+def test_checkpoint_roundtrip(cuda, ora, tmp_path):
+    N0 = (9, 9, 5)
+    x = np.random.default_rng(41).standard_normal((int(np.prod(N0)), 16)).astype(np.float32)
+    a, b = Net(seed=5), Net(seed=6)
+    assert not np.array_equal(a.get_params(), b.get_params())
+    a.save(tmp_path / "net.bin")
+    b.load(tmp_path / "net.bin")
+    np.testing.assert_array_equal(a.get_params(), b.get_params())
+    np.testing.assert_array_equal(a.forward(N0, x), b.forward(N0, x))
+    (tmp_path / "bad.bin").write_bytes(b"not a checkpoint")
+    with pytest.raises(AlamoError):
+        b.load(tmp_path / "bad.bin")
+
+
+def _eshelby_op(pr, bc):
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=PREC_MIXED, cycle=CYCLE_V,
+                 pre_smooth=1, post_smooth=1)
+    op.SetModel(pr["C"])
+    return op
+
+
+def test_learned_krylov_solve(cuda, ora):
+    """Train on the operator, then solve with the learned stage: converged, and the solution
+    equals the oracle's FP64 multigrid solution."""
+    n = 32
+    pr, bc, rhs = eshelby_rhs(ora, n)
+    xref, *_ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-12)
+    op = _eshelby_op(pr, bc)
+    net = Net(seed=3)
+    print("train", flush=True)
+    st = net.train(op, npairs=4, steps=120, lr=1e-3, seed=9)
+    print("train stats", st)
+    assert st["pairs"] == 4 and np.isfinite(st["last_loss"])
+    net.attach(op)
+    res = {}
+    for kind in (PRECOND_MG, PRECOND_LEARNED):
+        sol = to_dev(np.zeros_like(rhs), cuda)
+        res[kind] = Krylov(op, restart=16, precond=kind, max_iter=400).solve(sol, to_dev(rhs, cuda), 1e-11, 0.0)
+        err = np.linalg.norm(to_host(sol) - xref) / np.linalg.norm(xref)
+        print("precond", kind, "iterations", res[kind]["iterations"], "fallbacks",
+              res[kind]["fallback_events"], "rel err", err)
+        assert res[kind]["converged"] == 1 and err <= 1e-8
+
+
+def test_learned_fallback_on_non_finite_network(cuda, ora):
+    """A network that returns NaN is dropped for the plain cycle and the solve still converges."""
+    n = 16
+    pr, bc, rhs = eshelby_rhs(ora, n)
+    op = _eshelby_op(pr, bc)
+    with pytest.raises(AlamoError):  # no network attached
+        Krylov(op, precond=PRECOND_LEARNED).solve(to_dev(np.zeros_like(rhs), cuda), to_dev(rhs, cuda), 1e-10, 0.0)
+    net = Net(seed=3)
+    p = net.get_params()
+    p[:] = np.nan
+    net.set_params(p)
+    net.attach(op)
+    sol = to_dev(np.zeros_like(rhs), cuda)
+    st = Krylov(op, restart=8, precond=PRECOND_LEARNED, max_iter=400).solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
+    print(st)
+    assert st["converged"] == 1 and st["fallback_events"] >= 1
+    assert np.all(np.isfinite(to_host(sol)))

@@ -9,7 +9,7 @@ import pytest
 
 from alamo_amd import problems as P
 from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, CYCLE_V, PRECOND_NONE
-from helpers import random_iso, random_mm, random_diag, random_bc, to_dev, to_host, rng
+from helpers import random_iso, random_mm, random_diag, random_bc, to_dev, to_host, rng, eshelby_rhs
 
 pytestmark = pytest.mark.gpu
 
@@ -138,15 +138,6 @@ def test_bc_init_and_newton_rhs(cuda, ora):
     rhs = to_dev(np.zeros(3 * nodes), cuda)
     op.NewtonRHS(to_dev(u, cuda), b, to_dev(F0, cuda), rhs)
     close(to_host(rhs), ora.newton_rhs(ncell, dx, P.MAJORMINOR, C, F0, pr_bc[0], u, bref))
-
-
-def eshelby_rhs(ora, n):
-    pr = P.eshelby_problem(n)
-    bct, bcv = P.bc_constant()
-    N = (n + 1) ** 3
-    b = ora.bc_init(pr["ncell"], bct, bcv)
-    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, np.zeros(3 * N), b)
-    return pr, (bct, bcv), rhs
 
 
 @pytest.mark.parametrize("precision,cycle", [(0, 1), (0, 0), (PREC_MIXED, 1)])
