@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libalamo_amd.so")
+# ALAMO_AMD_LIB selects another build of the library (comparisons of kernel variants)
+LIB_PATH = os.environ.get("ALAMO_AMD_LIB") or os.path.join(HERE, "lib", "libalamo_amd.so")
 
 c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 P = ctypes.POINTER

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,12 +20,19 @@
 #include "../../include/alamo_amd.h"
 #include "kernels_generic.h"
 #include "kernels_iso.h"
+#include "kernels_iso_rows.h"
 #include "krylov.h"
 
 using namespace alamo;
 
 static thread_local std::string g_err;
 static std::atomic<uint64_t> g_model_version{0};
+// interior kernel of large isotropic levels: 0 = row-wavefront kernel (default), 1 = 64x4
+// LDS-tile kernel; ALAMO_ISO_KERNEL=tiles selects the latter (for comparisons)
+static const int g_iso_kernel = [] {
+    const char* e = getenv("ALAMO_ISO_KERNEL");
+    return e && strcmp(e, "tiles") == 0 ? 1 : 0;
+}();
 
 #define FAIL(code, msg)            \
     do {                           \
@@ -135,7 +143,8 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
     if (sym == ISO) {
         // large isotropic levels take the LDS-tiled interior kernel plus the boundary kernel
         if (mode != DIAG && iso_tiled_ok(g)) {
-            launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
+            if (g_iso_kernel == 1) launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
+            else launch_iso_rows<T>(mode, g, bc, C, u, b, out, omega, s);
             return;
         }
         launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);

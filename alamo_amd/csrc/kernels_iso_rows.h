@@ -1,0 +1,461 @@
+// Row-wavefront kernel for the interior rows of the isotropic operator on gfx950.
+//
+// Same interior formula as k_iso_interior (Operator/Elastic.cpp:204-254 with
+// Set/Matrix4_Isotropic.H:137-173):
+//   f_i = mu (Lap u_i + d_i div u) + lambda d_i div u + d_i(lambda) div u + sum_d d_d(mu) (u_i,d + u_d,i)
+// Work split:
+//   * one wavefront = one x row segment of 64 nodes (x0-1 .. x0+62); lanes 1..62 produce
+//     output, lanes 0 and 63 are the x halo.  x neighbours come from the neighbouring lane
+//     through DPP wavefront shifts, never from memory;
+//   * a workgroup = ROWS_RW such rows stacked in y; each plane's (u, lambda, mu) records of
+//     the rows (plus one halo row each side) are staged in LDS, where only the y neighbours
+//     read them.  Three plane slots, one barrier per plane;
+//   * z is marched serially over a chunk of planes; each lane keeps a three-plane register
+//     queue of u, lambda, mu and the in-plane differences that the x-z and y-z cross
+//     derivatives need (d_xz u = D_z(D_x u), exactly the four-corner formula).
+// HBM sees each value once plus the tile halos: (RW + 2) / RW in y, 64 / 62 in x.
+// When both x faces are Dirichlet in every component, their face rows (f = u) are written
+// here too, so the boundary kernel only visits the y and z faces.
+#pragma once
+#include "kernels_iso.h"
+
+namespace alamo {
+
+#ifndef ALAMO_ROWS_RW
+#define ALAMO_ROWS_RW 4
+#endif
+#ifndef ALAMO_ROWS_KCH
+#define ALAMO_ROWS_KCH 16
+#endif
+constexpr int ROWS_RW = ALAMO_ROWS_RW;    // wavefronts (rows) per workgroup
+constexpr int ROWS_KCH = ALAMO_ROWS_KCH;  // z planes per workgroup
+constexpr int ROWS_OUT = 62;              // output nodes per row segment
+static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row each");
+
+#ifndef ALAMO_ROWS_XCD
+#define ALAMO_ROWS_XCD 1
+#endif
+#ifndef ALAMO_ROWS_WPE
+#define ALAMO_ROWS_WPE 1
+#endif
+#define ALAMO_ROWS_ATTR __launch_bounds__(64 * ROWS_RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
+
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T from_right(T v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
+template <typename T>
+__device__ __forceinline__ T from_left(T v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
+
+// per-lane record of one plane of the lane's own row
+template <typename T>
+struct RowPlane {
+    T u[3], lam, mu;
+    T dxux, dxuz, dyuy, dyuz;  // D_x u_x, D_x u_z, D_y u_y, D_y u_z
+};
+
+inline bool iso_rows_ok(const Geom& g) { return g.n[0] >= 2 && g.n[1] >= 2 && g.n[2] >= 2; }
+
+// FOLD: 0 = interior rows only; 1 = also the x-face rows (both x faces Dirichlet).
+template <typename T, int MODE, int FOLD>
+__global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, const T* __restrict__ u,
+                                                           const T* __restrict__ b, T* __restrict__ out, T omega) {
+    constexpr int NR = ROWS_RW + 2;  // staged rows: halo, RW rows, halo
+    constexpr int A = NR * 64;
+    __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 64 + lane]
+    const int lane = threadIdx.x, w = threadIdx.y;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (ALAMO_ROWS_XCD) {
+        // neighbouring tiles (x fastest, then y, then the z chunk) share an XCD and its L2
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
+    const int x0 = 1 + ROWS_OUT * bx;  // lane l <-> node x0 - 1 + l
+    const int y0 = 1 + ROWS_RW * by;
+    const int k0 = 1 + ROWS_KCH * bz;
+    const int k1 = min(k0 + ROWS_KCH, g.n[2]);  // output planes k0 .. k1 - 1; loads k0 - 1 .. k1
+    const int x = x0 - 1 + lane;
+    const int xl = min(x, g.n[0]);
+    const int y = y0 + w;
+    const bool row_out = y <= g.n[1] - 1;
+    const bool out_i = row_out && lane >= 1 && lane <= ROWS_OUT && x <= g.n[0] - 1;
+    const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
+
+    const int64_t col = (int64_t)xl + g.sy * (int64_t)min(y, g.n[1]);
+    // the first and last wavefronts also stage the halo rows
+    const int hrow = w == 0 ? 0 : (w == ROWS_RW - 1 ? NR - 1 : -1);
+    const int64_t hcol = (int64_t)xl + g.sy * (int64_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]));
+    const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
+    const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
+    const int ln = lane < 63 ? lane + 1 : 63, lw = lane > 0 ? lane - 1 : 0;
+    const int rs = w * 64 + lane, rc = (w + 1) * 64 + lane, rn = (w + 2) * 64 + lane;
+
+    // prefetched values of the next plane (own row and halo row)
+    T pv[5], ph[5];
+    auto fetch = [&](int p) {
+        const int64_t pz = g.sz * (int64_t)p;
+#pragma unroll
+        for (int v = 0; v < 3; v++) pv[v] = u[v * g.tot + col + pz];
+        pv[3] = C[col + pz];
+        pv[4] = C[g.tot + col + pz];
+        if (hrow >= 0) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) ph[v] = u[v * g.tot + hcol + pz];
+            ph[3] = C[hcol + pz];
+            ph[4] = C[g.tot + hcol + pz];
+        }
+    };
+    RowPlane<T> qm, qc, qn;
+    fetch(k0 - 1);
+    int slot = 0;
+    for (int p = k0 - 1; p <= k1; p++) {
+        // stage plane p
+#pragma unroll
+        for (int v = 0; v < 5; v++) lds[slot][v][rc] = pv[v];
+        if (hrow >= 0) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) lds[slot][v][hrow * 64 + lane] = ph[v];
+        }
+        qm = qc;
+        qc = qn;
+#pragma unroll
+        for (int v = 0; v < 3; v++) qn.u[v] = pv[v];
+        qn.lam = pv[3];
+        qn.mu = pv[4];
+        if (p < k1) fetch(p + 1);
+        qn.dxux = (from_right(qn.u[0]) - from_left(qn.u[0])) * hx;
+        qn.dxuz = (from_right(qn.u[2]) - from_left(qn.u[2])) * hx;
+        const int64_t mc = col + g.sz * (int64_t)(p - 1);
+        T bq[3] = {T(0), T(0), T(0)};
+        if (MODE != APPLY && p >= k0 + 1 && (out_i || out_f)) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) bq[v] = b[v * g.tot + mc];
+        }
+        __syncthreads();
+        const T(*sp)[A] = lds[slot];
+        qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
+        qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
+        if (p >= k0 + 1) {
+            // output plane c = p - 1: planes c - 1 (qm), c (qc, LDS slot - 1), c + 1 (qn)
+            const T(*sc)[A] = lds[slot == 0 ? 2 : slot - 1];
+            // x neighbours of the centre plane; DPP reads of a lane whose exec bit is off
+            // return 0, so every shift runs with the whole wavefront active
+            T ue[3], uw[3];
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                ue[v] = from_right(qc.u[v]);
+                uw[v] = from_left(qc.u[v]);
+            }
+            const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
+            const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
+            if (out_i) {
+                T ux[3], uy[3], uz[3], lap[3];
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    const T c0 = qc.u[v], e = ue[v], wv = uw[v];
+                    const T nn = sc[v][rn], ss = sc[v][rs];
+                    ux[v] = (e - wv) * hx;
+                    uy[v] = (nn - ss) * hy;
+                    uz[v] = (qn.u[v] - qm.u[v]) * hz;
+                    lap[v] = (e - T(2) * c0 + wv) * idx * idx + (nn - T(2) * c0 + ss) * idy * idy +
+                             (qn.u[v] - T(2) * c0 + qm.u[v]) * idz * idz;
+                }
+                const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * idx * idx;
+                const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * idy * idy;
+                const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * idz * idz;
+                const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] +
+                                sc[0][rs - lane + lw]) * hx * hy;
+                const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] +
+                                sc[1][rs - lane + lw]) * hx * hy;
+                const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
+                const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
+                const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
+                const T lam = qc.lam, mu = qc.mu;
+                const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
+                const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
+                const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    T f = mu * (lap[v] + gd[v]) + lam * gd[v];
+                    f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
+                         dm[2] * (G[v][2] + G[2][v]);
+                    const int64_t q = v * g.tot + mc;
+                    if (MODE == APPLY) out[q] = f;
+                    else if (MODE == RESID) out[q] = bq[v] - f;
+                    else {
+                        const T w2[3] = {idx * idx, idy * idy, idz * idz};
+                        const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (w2[0] + w2[1] + w2[2] - w2[v]));
+                        out[q] = (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D;
+                    }
+                }
+            } else if (out_f) {
+                // Dirichlet face row f = u (diagonal 1)
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    const int64_t q = v * g.tot + mc;
+                    if (MODE == APPLY) out[q] = qc.u[v];
+                    else if (MODE == RESID) out[q] = bq[v] - qc.u[v];
+                    else out[q] = (T(1) - omega) * qc.u[v] + omega * bq[v];
+                }
+            }
+        }
+        slot = slot == 2 ? 0 : slot + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant with the planes staged by direct-to-LDS loads (global_load_lds_dword), two planes
+// ahead in a four-slot ring, so no registers hold prefetched data.  Each wavefront issues the
+// same number DMA_L of DMA loads per plane; it waits with vmcnt(DMA_L) for its loads of the
+// plane about to be read (vector memory operations retire in order), then the barrier makes
+// every wavefront's loads visible.
+// ---------------------------------------------------------------------------
+#ifndef ALAMO_DMA_RW
+#define ALAMO_DMA_RW 4
+#endif
+constexpr int DMA_RW = ALAMO_DMA_RW;
+constexpr int DMA_NR = DMA_RW + 2;
+static_assert((DMA_NR * 10) % DMA_RW == 0, "DMA loads must split evenly over the wavefronts");
+constexpr int DMA_L = DMA_NR * 10 / DMA_RW;  // dword DMA loads per wavefront per plane
+constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
+
+template <typename T, int MODE, int FOLD>
+__global__ void __launch_bounds__(64 * DMA_RW) k_iso_rows_dma(Geom g, const T* __restrict__ C,
+                                                              const T* __restrict__ u, const T* __restrict__ b,
+                                                              T* __restrict__ out, T omega) {
+    static_assert(sizeof(T) == 8, "the DMA staging moves doubles as dword pairs");
+    constexpr int NR = DMA_NR;
+    constexpr int A = NR * 64;
+    __shared__ T lds[4][5][A];
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (ALAMO_ROWS_XCD) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
+    const int x0 = 1 + ROWS_OUT * bx;
+    const int y0 = 1 + DMA_RW * by;
+    const int k0 = 1 + ROWS_KCH * bz;
+    const int k1 = min(k0 + ROWS_KCH, g.n[2]);
+    const int x = x0 - 1 + lane;
+    const int y = y0 + w;
+    const bool row_out = y <= g.n[1] - 1;
+    const bool out_i = row_out && lane >= 1 && lane <= ROWS_OUT && x <= g.n[0] - 1;
+    const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
+    const int64_t col = (int64_t)min(x, g.n[0]) + g.sy * (int64_t)min(y, g.n[1]);
+    const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
+    const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
+    const int ln = lane < 63 ? lane + 1 : 63, lw = lane > 0 ? lane - 1 : 0;
+    const int rs = w * 64 + lane, rc = (w + 1) * 64 + lane, rn = (w + 2) * 64 + lane;
+    // this lane's dword of each DMA: half h of a row-array covers doubles 32 h .. 32 h + 31
+    const int xd0 = min(x0 - 1 + lane / 2, g.n[0]), xd1 = min(x0 - 1 + 32 + lane / 2, g.n[0]);
+    const int wo = 4 * (lane & 1);
+    auto issue = [&](int p) {
+        // job j = (row r, array v, half h), r in [0, NR), v in [0, 5), h in [0, 2); wave w
+        // takes jobs w, w + RW, ...
+        const int64_t pz = g.sz * (int64_t)p;
+        const int sl = p & 3;
+#pragma unroll
+        for (int q = 0; q < DMA_L; q++) {
+            const int j = w + DMA_RW * q;
+            const int h = j & 1, v = (j >> 1) % 5, r = (j >> 1) / 5;
+            const int yr = min(max(y0 - 1 + r, 0), g.n[1]);
+            const T* base = v < 3 ? u + (int64_t)v * g.tot : C + (int64_t)(v - 3) * g.tot;
+            const char* src = (const char*)(base + (h ? xd1 : xd0) + g.sy * (int64_t)yr + pz) + wo;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)((char*)&lds[sl][v][r * 64] + 256 * h),
+                                             4, 0, 0);
+        }
+    };
+    RowPlane<T> qm, qc, qn;
+    issue(k0 - 1);
+    issue(k0);
+    for (int p = k0 - 1; p <= k1; p++) {
+        const int sl = p & 3;
+        // loads of plane p are done once at most the DMA_L loads of plane p + 1 are in flight
+        if (p + 1 <= k1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(DMA_L));
+        else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+        __builtin_amdgcn_s_barrier();
+        const int64_t mc = col + g.sz * (int64_t)(p - 1);
+        T bq[3] = {T(0), T(0), T(0)};
+        if (MODE != APPLY && p >= k0 + 1 && (out_i || out_f)) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) bq[v] = b[v * g.tot + mc];
+        }
+        // plane p + 2 goes into the slot of plane p - 2, last read before this barrier
+        if (p + 2 <= k1) issue(p + 2);
+        const T(*sp)[A] = lds[sl];
+        qm = qc;
+        qc = qn;
+#pragma unroll
+        for (int v = 0; v < 3; v++) qn.u[v] = sp[v][rc];
+        qn.lam = sp[3][rc];
+        qn.mu = sp[4][rc];
+        qn.dxux = (from_right(qn.u[0]) - from_left(qn.u[0])) * hx;
+        qn.dxuz = (from_right(qn.u[2]) - from_left(qn.u[2])) * hx;
+        qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
+        qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
+        if (p >= k0 + 1) {
+            const T(*sc)[A] = lds[(p - 1) & 3];
+            T ue[3], uw[3];
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                ue[v] = from_right(qc.u[v]);
+                uw[v] = from_left(qc.u[v]);
+            }
+            const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
+            const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
+            if (out_i) {
+                T ux[3], uy[3], uz[3], lap[3];
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    const T c0 = qc.u[v], e = ue[v], wv = uw[v];
+                    const T nn = sc[v][rn], ss = sc[v][rs];
+                    ux[v] = (e - wv) * hx;
+                    uy[v] = (nn - ss) * hy;
+                    uz[v] = (qn.u[v] - qm.u[v]) * hz;
+                    lap[v] = (e - T(2) * c0 + wv) * idx * idx + (nn - T(2) * c0 + ss) * idy * idy +
+                             (qn.u[v] - T(2) * c0 + qm.u[v]) * idz * idz;
+                }
+                const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * idx * idx;
+                const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * idy * idy;
+                const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * idz * idz;
+                const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] +
+                                sc[0][rs - lane + lw]) * hx * hy;
+                const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] +
+                                sc[1][rs - lane + lw]) * hx * hy;
+                const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
+                const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
+                const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
+                const T lam = qc.lam, mu = qc.mu;
+                const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
+                const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
+                const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    T f = mu * (lap[v] + gd[v]) + lam * gd[v];
+                    f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
+                         dm[2] * (G[v][2] + G[2][v]);
+                    const int64_t q = v * g.tot + mc;
+                    if (MODE == APPLY) out[q] = f;
+                    else if (MODE == RESID) out[q] = bq[v] - f;
+                    else {
+                        const T w2[3] = {idx * idx, idy * idy, idz * idz};
+                        const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (w2[0] + w2[1] + w2[2] - w2[v]));
+                        out[q] = (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D;
+                    }
+                }
+            } else if (out_f) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    const int64_t q = v * g.tot + mc;
+                    if (MODE == APPLY) out[q] = qc.u[v];
+                    else if (MODE == RESID) out[q] = bq[v] - qc.u[v];
+                    else out[q] = (T(1) - omega) * qc.u[v] + omega * bq[v];
+                }
+            }
+        }
+    }
+}
+
+// true when both x faces (without their edges) are Dirichlet in all three components
+inline bool x_faces_dirichlet(const BCTable& bc) {
+    for (int c = 0; c < 3; c++)
+        if (bc.t[0][c] != DISPLACEMENT || bc.t[3][c] != DISPLACEMENT) return false;
+    return true;
+}
+
+inline int64_t boundary_count_yz(const Geom& g) {
+    const int64_t N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+    return 2 * N0 * N1 + 2 * N0 * (N2 - 2);
+}
+
+inline bool all_dirichlet(const BCTable& bc) {
+    for (int f = 0; f < 26; f++)
+        for (int c = 0; c < 3; c++)
+            if (bc.t[f][c] != DISPLACEMENT) return false;
+    return true;
+}
+
+// y- and z-face rows when every region is Dirichlet: f = u (diagonal 1), coalesced rows
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_dirichlet_yz(Geom g, const T* __restrict__ u, const T* __restrict__ b,
+                                                      T* __restrict__ out, T omega, int64_t nb) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    int i, j, k;
+    boundary_ijk(g, q, i, j, k);
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const int64_t qq = v * g.tot + m;
+        const T x = u[qq];
+        if (MODE == APPLY) out[qq] = x;
+        else if (MODE == RESID) out[qq] = b[qq] - x;
+        else out[qq] = (T(1) - omega) * x + omega * b[qq];
+    }
+}
+
+#ifndef ALAMO_ROWS_DMA
+#define ALAMO_ROWS_DMA 0
+#endif
+template <typename T, int MODE, int FOLD>
+static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s) {
+    if (ALAMO_ROWS_DMA && sizeof(T) == 8) {
+        dim3 blk(64, DMA_RW);
+        dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + DMA_RW - 1) / DMA_RW,
+                  (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
+        if constexpr (sizeof(T) == 8) k_iso_rows_dma<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
+        return;
+    }
+    dim3 blk(64, ROWS_RW);
+    dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,
+              (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
+    k_iso_rows<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
+}
+
+template <typename T, int MODE>
+static void launch_iso_rows_mode(const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b, T* out, T omega,
+                                 hipStream_t s) {
+    const bool xdir = x_faces_dirichlet(bc);
+    const int64_t nb = xdir ? boundary_count_yz(g) : boundary_count(g);
+    if (all_dirichlet(bc)) {
+        launch_rows_kernel<T, MODE, 1>(g, C, u, b, out, omega, s);
+        k_dirichlet_yz<T, MODE><<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(g, u, b, out, omega, nb);
+        return;
+    }
+    if (xdir) launch_rows_kernel<T, MODE, 1>(g, C, u, b, out, omega, s);
+    else launch_rows_kernel<T, MODE, 0>(g, C, u, b, out, omega, s);
+    k_boundary<T, ISO, MODE><<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
+}
+
+template <typename T>
+static void launch_iso_rows(int mode, const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b, T* out,
+                            T omega, hipStream_t s) {
+    if (mode == APPLY) launch_iso_rows_mode<T, APPLY>(g, bc, C, u, b, out, omega, s);
+    else if (mode == RESID) launch_iso_rows_mode<T, RESID>(g, bc, C, u, b, out, omega, s);
+    else launch_iso_rows_mode<T, JACOBI>(g, bc, C, u, b, out, omega, s);
+}
+
+}  // namespace alamo

@@ -94,6 +94,28 @@ def test_diagonal_residual_fsmooth(cuda, ora, sym, ncell):
     close(to_host(xd), xs, 1e-11)
 
 
+@pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180)])
+def test_iso_dirichlet_x_faces(cuda, ora, ncell):
+    """Dirichlet x faces in all components: the row kernel writes those face rows itself and
+    the boundary kernel covers the y and z faces (their edges keep traction/Neumann rows)."""
+    t, v = random_bc(61, (P.TRACTION, P.NEUMANN, P.DISPLACEMENT))
+    t[0, :] = P.DISPLACEMENT
+    t[3, :] = P.DISPLACEMENT
+    op, C, _, nodes, dx = build(P.ISO, ncell, 62, bc=(t, v))
+    print("dirichlet x faces", ncell, flush=True)
+    u = rng(63).uniform(-1, 1, 3 * nodes)
+    b = rng(64).uniform(-1, 1, 3 * nodes)
+    out = to_dev(np.zeros(3 * nodes), cuda)
+    op.Fapply(0, 0, out, to_dev(u, cuda))
+    close(to_host(out), ora.fapply(ncell, dx, P.ISO, C, t, u))
+    op.correctionResidual(0, 0, out, to_dev(u, cuda), to_dev(b, cuda))
+    close(to_host(out), ora.residual(ncell, dx, P.ISO, C, t, u, b))
+    D = ora.diagonal(ncell, dx, P.ISO, C, t)
+    xd = to_dev(u, cuda)
+    op.Fsmooth(0, 0, xd, to_dev(b, cuda))
+    close(to_host(xd), ora.jacobi(ncell, dx, P.ISO, C, t, D, 2.0 / 3.0, 2, u, b), 1e-11)
+
+
 @pytest.mark.parametrize("ncell", [(8, 8, 8), (16, 12, 20), (32, 32, 32)])
 def test_transfers(cuda, ora, ncell):
     op, C, bc, nodes, dx = build(P.ISO, ncell, 31)
