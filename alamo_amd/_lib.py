@@ -93,6 +93,7 @@ SIGNATURES = {
     "alamo_net_save": (c_int, [c_void_p, ctypes.c_char_p]),
     "alamo_net_load": (c_int, [c_void_p, ctypes.c_char_p]),
     "alamo_mg_set_net": (c_int, [c_void_p, c_void_p]),
+    "alamo_mg_precond": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 

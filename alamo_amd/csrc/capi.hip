@@ -143,7 +143,9 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
     if (sym == ISO) {
         // large isotropic levels take the LDS-tiled interior kernel plus the boundary kernel
         if (mode != DIAG && iso_tiled_ok(g)) {
-            if (g_iso_kernel == 1) launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
+            // the row kernel addresses each array with 32-bit byte offsets
+            if (g_iso_kernel == 1 || (int64_t)sizeof(T) * g.tot >= ((int64_t)1 << 32))
+                launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
             else launch_iso_rows<T>(mode, g, bc, C, u, b, out, omega, s);
             return;
         }
@@ -377,6 +379,7 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
 // ---------------------------------------------------------------------------
 // Flexible GMRES(m) (FP64) with a multigrid-cycle preconditioner
 // ---------------------------------------------------------------------------
+static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out);
 #include "learned_host.h"
 
 static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
@@ -385,7 +388,7 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     if (kind == ALAMO_PRECOND_LEARNED) {
         // one multigrid cycle, then the network's correction of the cycle's remaining error
         TRY(precond_apply(h, ALAMO_PRECOND_MG, v, z));
-        if (!h->lq) TRY(dalloc(h, &h->lq, n));
+        if (!h->lq) TRY(dalloc(h, &h->lq, 3 * n));
         return net_correct(h->net, h, v, z, h->lq);
     }
     if (kind == ALAMO_PRECOND_NONE) {
@@ -849,6 +852,19 @@ int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_r
     return fgmres_impl(h, sol, rhs, tol_rel, tol_abs, &o, stats);
 }
 
+int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z) {
+    if (!h || !r || !z) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    if (kind != ALAMO_PRECOND_MG && kind != ALAMO_PRECOND_LEARNED) FAIL(ALAMO_ERR_ARG, "unknown preconditioner");
+    if (kind == ALAMO_PRECOND_LEARNED && !h->net) FAIL(ALAMO_ERR_STATE, "no network attached (alamo_mg_set_net)");
+    if (h->L.back().g.tot > 17 * 17 * 17)
+        FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
+    if (!h->prepared) TRY(prepare(h));
+    TRY(precond_apply(h, kind, r, z));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ALAMO_OK;
+}
+
 int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        alamo_solve_stats* stats) {
     if (!h || !sol || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
@@ -1183,9 +1199,13 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
         double ql2 = 0;
         rc = norms_dev<double>(h, q, n3, nullptr, &ql2);
         if (rc) break;
+        double el2 = 0;
+        rc = norms_dev<double>(h, e, n3, nullptr, &el2);
+        if (rc) break;
         const double s = std::max(ql2 / std::sqrt((double)n3), 1e-300);
+        const double se = std::max(el2 / std::sqrt((double)n3), 1e-300);
         net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(V, q, 1.0 / s, n->B.feat, X + (int64_t)p * V * 16);
-        k_pack_target<<<nb256(V), 256, 0, h->stream>>>(V, e, 1.0 / s, T + (int64_t)p * V * 3);
+        k_pack_target<<<nb256(V), 256, 0, h->stream>>>(V, e, 1.0 / se, T + (int64_t)p * V * 3);
         if (o->verbose) fprintf(stderr, "alamo train: pair %d generated (solve %d its)\n", p, ss.iterations);
     }
     if (rc) { cleanup(); return rc; }

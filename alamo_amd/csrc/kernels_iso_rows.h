@@ -17,6 +17,8 @@
 // When both x faces are Dirichlet in every component, their face rows (f = u) are written
 // here too, so the boundary kernel only visits the y and z faces.
 #pragma once
+#include <type_traits>
+
 #include "kernels_iso.h"
 
 namespace alamo {
@@ -33,7 +35,7 @@ constexpr int ROWS_OUT = 62;              // output nodes per row segment
 static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row each");
 
 #ifndef ALAMO_ROWS_XCD
-#define ALAMO_ROWS_XCD 1
+#define ALAMO_ROWS_XCD 0
 #endif
 #ifndef ALAMO_ROWS_WPE
 #define ALAMO_ROWS_WPE 1
@@ -68,10 +70,23 @@ struct RowPlane {
 
 inline bool iso_rows_ok(const Geom& g) { return g.n[0] >= 2 && g.n[1] >= 2 && g.n[2] >= 2; }
 
+// 32-bit byte offset from a uniform base: lets the loads and stores use the scalar-base
+// addressing form (the host checks that every offset fits)
+template <typename T>
+__device__ __forceinline__ T ldb(const T* __restrict__ base, uint32_t boff) {
+    return *(const T*)((const char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* __restrict__ base, uint32_t boff, T v) {
+    *(T*)((char*)base + boff) = v;
+}
+
 // FOLD: 0 = interior rows only; 1 = also the x-face rows (both x faces Dirichlet).
+// The z loop is unrolled by four over four register records, so the queue rotates by renaming
+// and each plane is fetched straight into the record that holds it.
 template <typename T, int MODE, int FOLD>
 __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, const T* __restrict__ u,
-                                                           const T* __restrict__ b, T* __restrict__ out, T omega) {
+                                           const T* __restrict__ b, T* __restrict__ out, T omega) {
     constexpr int NR = ROWS_RW + 2;  // staged rows: halo, RW rows, halo
     constexpr int A = NR * 64;
     __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 64 + lane]
@@ -95,127 +110,131 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, cons
     const bool row_out = y <= g.n[1] - 1;
     const bool out_i = row_out && lane >= 1 && lane <= ROWS_OUT && x <= g.n[0] - 1;
     const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
-
-    const int64_t col = (int64_t)xl + g.sy * (int64_t)min(y, g.n[1]);
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t szb = (uint32_t)g.sz * E;
+    const uint32_t colb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
     // the first and last wavefronts also stage the halo rows
     const int hrow = w == 0 ? 0 : (w == ROWS_RW - 1 ? NR - 1 : -1);
-    const int64_t hcol = (int64_t)xl + g.sy * (int64_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]));
+    const uint32_t hcolb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]))) * E;
+    const T* __restrict__ U[3] = {u, u + g.tot, u + 2 * g.tot};
+    const T* __restrict__ la = C;
+    const T* __restrict__ ma = C + g.tot;
+    const T* __restrict__ B[3] = {b, b + g.tot, b + 2 * g.tot};
+    T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
     const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
     const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
+    const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
     const int ln = lane < 63 ? lane + 1 : 63, lw = lane > 0 ? lane - 1 : 0;
     const int rs = w * 64 + lane, rc = (w + 1) * 64 + lane, rn = (w + 2) * 64 + lane;
 
-    // prefetched values of the next plane (own row and halo row)
-    T pv[5], ph[5];
-    auto fetch = [&](int p) {
-        const int64_t pz = g.sz * (int64_t)p;
+    T ph[5];  // prefetched halo row of the next plane
+    auto fetch = [&](int p, RowPlane<T>& d) {
+        const uint32_t o = colb + szb * (uint32_t)p;
 #pragma unroll
-        for (int v = 0; v < 3; v++) pv[v] = u[v * g.tot + col + pz];
-        pv[3] = C[col + pz];
-        pv[4] = C[g.tot + col + pz];
+        for (int v = 0; v < 3; v++) d.u[v] = ldb(U[v], o);
+        d.lam = ldb(la, o);
+        d.mu = ldb(ma, o);
         if (hrow >= 0) {
+            const uint32_t oh = hcolb + szb * (uint32_t)p;
 #pragma unroll
-            for (int v = 0; v < 3; v++) ph[v] = u[v * g.tot + hcol + pz];
-            ph[3] = C[hcol + pz];
-            ph[4] = C[g.tot + hcol + pz];
+            for (int v = 0; v < 3; v++) ph[v] = ldb(U[v], oh);
+            ph[3] = ldb(la, oh);
+            ph[4] = ldb(ma, oh);
         }
     };
-    RowPlane<T> qm, qc, qn;
-    fetch(k0 - 1);
-    int slot = 0;
-    for (int p = k0 - 1; p <= k1; p++) {
-        // stage plane p
+    // one plane step: stage plane p (already in qn) into its LDS slot, prefetch plane p + 1
+    // into qx, finish qn's differences and (from the third step on) write plane c = p - 1 from
+    // qm (c - 1), qc (c) and qn (c + 1)
+    auto step = [&](int p, const RowPlane<T>& qm, const RowPlane<T>& qc, RowPlane<T>& qn, RowPlane<T>& qx) {
+        const int sl = (p - k0 + 1) % 3;
+        T(*sp)[A] = lds[sl];
 #pragma unroll
-        for (int v = 0; v < 5; v++) lds[slot][v][rc] = pv[v];
+        for (int v = 0; v < 3; v++) sp[v][rc] = qn.u[v];
+        sp[3][rc] = qn.lam;
+        sp[4][rc] = qn.mu;
         if (hrow >= 0) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) lds[slot][v][hrow * 64 + lane] = ph[v];
+            for (int v = 0; v < 5; v++) sp[v][hrow * 64 + lane] = ph[v];
         }
-        qm = qc;
-        qc = qn;
-#pragma unroll
-        for (int v = 0; v < 3; v++) qn.u[v] = pv[v];
-        qn.lam = pv[3];
-        qn.mu = pv[4];
-        if (p < k1) fetch(p + 1);
+        if (p < k1) fetch(p + 1, qx);
         qn.dxux = (from_right(qn.u[0]) - from_left(qn.u[0])) * hx;
         qn.dxuz = (from_right(qn.u[2]) - from_left(qn.u[2])) * hx;
-        const int64_t mc = col + g.sz * (int64_t)(p - 1);
+        const uint32_t mcb = colb + szb * (uint32_t)(p - 1);
         T bq[3] = {T(0), T(0), T(0)};
         if (MODE != APPLY && p >= k0 + 1 && (out_i || out_f)) {
 #pragma unroll
-            for (int v = 0; v < 3; v++) bq[v] = b[v * g.tot + mc];
+            for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
         }
         __syncthreads();
-        const T(*sp)[A] = lds[slot];
         qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
         qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
-        if (p >= k0 + 1) {
-            // output plane c = p - 1: planes c - 1 (qm), c (qc, LDS slot - 1), c + 1 (qn)
-            const T(*sc)[A] = lds[slot == 0 ? 2 : slot - 1];
-            // x neighbours of the centre plane; DPP reads of a lane whose exec bit is off
-            // return 0, so every shift runs with the whole wavefront active
-            T ue[3], uw[3];
+        if (p < k0 + 1) return;
+        // output plane c = p - 1
+        const T(*sc)[A] = lds[sl == 0 ? 2 : sl - 1];
+        // x neighbours of the centre plane; DPP reads of a lane whose exec bit is off
+        // return 0, so every shift runs with the whole wavefront active
+        T ue[3], uw[3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            ue[v] = from_right(qc.u[v]);
+            uw[v] = from_left(qc.u[v]);
+        }
+        const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
+        const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
+        if (out_i) {
+            T ux[3], uy[3], uz[3], lap[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) {
-                ue[v] = from_right(qc.u[v]);
-                uw[v] = from_left(qc.u[v]);
+                const T c0 = qc.u[v], e = ue[v], wv = uw[v];
+                const T nn = sc[v][rn], ss = sc[v][rs];
+                ux[v] = (e - wv) * hx;
+                uy[v] = (nn - ss) * hy;
+                uz[v] = (qn.u[v] - qm.u[v]) * hz;
+                lap[v] = (e - T(2) * c0 + wv) * ix2 + (nn - T(2) * c0 + ss) * iy2 + (qn.u[v] - T(2) * c0 + qm.u[v]) * iz2;
             }
-            const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
-            const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
-            if (out_i) {
-                T ux[3], uy[3], uz[3], lap[3];
+            const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * ix2;
+            const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * iy2;
+            const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * iz2;
+            const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] + sc[0][rs - lane + lw]) * hxy;
+            const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] + sc[1][rs - lane + lw]) * hxy;
+            const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
+            const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
+            const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
+            const T lam = qc.lam, mu = qc.mu;
+            const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
+            const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
+            const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
+            const T tr = G[0][0] + G[1][1] + G[2][2];
 #pragma unroll
-                for (int v = 0; v < 3; v++) {
-                    const T c0 = qc.u[v], e = ue[v], wv = uw[v];
-                    const T nn = sc[v][rn], ss = sc[v][rs];
-                    ux[v] = (e - wv) * hx;
-                    uy[v] = (nn - ss) * hy;
-                    uz[v] = (qn.u[v] - qm.u[v]) * hz;
-                    lap[v] = (e - T(2) * c0 + wv) * idx * idx + (nn - T(2) * c0 + ss) * idy * idy +
-                             (qn.u[v] - T(2) * c0 + qm.u[v]) * idz * idz;
+            for (int v = 0; v < 3; v++) {
+                T f = mu * (lap[v] + gd[v]) + lam * gd[v];
+                f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
+                     dm[2] * (G[v][2] + G[2][v]);
+                if (MODE == APPLY) stb(O[v], mcb, f);
+                else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
+                else {
+                    const T w2[3] = {ix2, iy2, iz2};
+                    const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
+                    stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
                 }
-                const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * idx * idx;
-                const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * idy * idy;
-                const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * idz * idz;
-                const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] +
-                                sc[0][rs - lane + lw]) * hx * hy;
-                const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] +
-                                sc[1][rs - lane + lw]) * hx * hy;
-                const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
-                const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
-                const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
-                const T lam = qc.lam, mu = qc.mu;
-                const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
-                const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
-                const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
-                const T tr = G[0][0] + G[1][1] + G[2][2];
+            }
+        } else if (out_f) {
+            // Dirichlet face row f = u (diagonal 1)
 #pragma unroll
-                for (int v = 0; v < 3; v++) {
-                    T f = mu * (lap[v] + gd[v]) + lam * gd[v];
-                    f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
-                         dm[2] * (G[v][2] + G[2][v]);
-                    const int64_t q = v * g.tot + mc;
-                    if (MODE == APPLY) out[q] = f;
-                    else if (MODE == RESID) out[q] = bq[v] - f;
-                    else {
-                        const T w2[3] = {idx * idx, idy * idy, idz * idz};
-                        const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (w2[0] + w2[1] + w2[2] - w2[v]));
-                        out[q] = (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D;
-                    }
-                }
-            } else if (out_f) {
-                // Dirichlet face row f = u (diagonal 1)
-#pragma unroll
-                for (int v = 0; v < 3; v++) {
-                    const int64_t q = v * g.tot + mc;
-                    if (MODE == APPLY) out[q] = qc.u[v];
-                    else if (MODE == RESID) out[q] = bq[v] - qc.u[v];
-                    else out[q] = (T(1) - omega) * qc.u[v] + omega * bq[v];
-                }
+            for (int v = 0; v < 3; v++) {
+                if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
+                else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
+                else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
             }
         }
-        slot = slot == 2 ? 0 : slot + 1;
+    };
+    RowPlane<T> qa, qb, qc, qd;  // plane p lives in record (p - k0 + 1) % 4
+    fetch(k0 - 1, qa);
+    for (int p = k0 - 1; p <= k1; p += 4) {
+        step(p, qc, qd, qa, qb);
+        if (p + 1 <= k1) step(p + 1, qd, qa, qb, qc);
+        if (p + 2 <= k1) step(p + 2, qa, qb, qc, qd);
+        if (p + 3 <= k1) step(p + 3, qb, qc, qd, qa);
     }
 }
 

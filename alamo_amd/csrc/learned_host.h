@@ -7,7 +7,8 @@
 //   level 2 (N/4):  a2 = relu(conv2(inject(a1); 32->64)),  b2 = conv3(a2; 64->32)
 //   level 1:        u1 = P b2 + a1,  b1 = relu(conv4(u1; 32->16))
 //   level 0:        u0 = P b1 + a0,  y  = conv5(u0; 16->16)    (channels 0-2 used)
-// As a preconditioner stage: z = V(r); q = r - A z; z += s * y(q / s), s = rms(q).
+// As a preconditioner stage: z = V(r); q = r - A z; y = net(q / rms(q)); z += alpha y with
+// the residual-minimising alpha.  Trained on targets e / rms(e), e = A^-1 q.
 #pragma once
 #include <cstdio>
 #include <random>
@@ -324,19 +325,28 @@ static int net_level0(alamo_net* n, alamo_mg* h) {
     return ALAMO_OK;
 }
 
-// z += s * net(q / s) with q = r - A z; q is written into `q` (scratch, FP64 3 planes)
-static int net_correct(alamo_net* n, alamo_mg* h, const double* r, double* z, double* q) {
+// z += alpha y with q = r - A z, y = net(q / rms(q)) and alpha = <q, A y> / <A y, A y>, the
+// step that minimises |q - alpha A y|_2, so the stage never raises the residual.  w holds three
+// FP64 vectors: q, A y, y.
+static int net_correct(alamo_net* n, alamo_mg* h, const double* r, double* z, double* w) {
     LevelBuf& l0 = h->L[0];
-    const int64_t V = l0.g.tot;
+    const int64_t V = l0.g.tot, n3 = 3 * V;
+    double *q = w, *Ay = w + n3, *y = w + 2 * n3;
     TRY(net_level0(n, h));
     launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, z, r, q, 0.0, h->stream);
     double ql2 = 0;
-    TRY(norms_dev<double>(h, q, 3 * V, nullptr, &ql2));
-    const double s = ql2 / std::sqrt((double)(3 * V));
+    TRY(norms_dev<double>(h, q, n3, nullptr, &ql2));
+    const double s = ql2 / std::sqrt((double)n3);
     if (!(s > 0.0) || !std::isfinite(s)) return ALAMO_OK;
     net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(V, q, 1.0 / s, n->B.feat, n->B.x);
     TRY(net_forward(n));
-    net::k_unpack_output<<<nb256(V), 256, 0, h->stream>>>(V, n->B.y, s, z, 1);
+    net::k_unpack_output<<<nb256(V), 256, 0, h->stream>>>(V, n->B.y, 1.0, y, 0);
+    launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, y, nullptr, Ay, 0.0, h->stream);
+    double d[2];
+    TRY(multidot(h, Ay, q, n3, 2, d));  // <Ay, q>, <Ay, Ay>
+    const double alpha = d[0] / d[1];
+    if (!(d[1] > 0.0) || !std::isfinite(alpha)) return ALAMO_OK;
+    k_axpy<double, double><<<gs_blocks(n3), 256, 0, h->stream>>>(z, y, n3, alpha);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }

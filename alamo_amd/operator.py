@@ -183,6 +183,11 @@ class Linear:
         return out
 
 
+def precond_apply(op, kind, r, z):
+    """z = P(r) for the operator's preconditioner kind (one cycle, optionally with the network)"""
+    check(lib().alamo_mg_precond(op.h, int(kind), _ptr(r), _ptr(z)))
+
+
 class Krylov:
     """Flexible GMRES(restart) around the operator's multigrid cycle (no reference counterpart:
     an outer accelerator for the same MLMG-style cycle, stopping on the reference's max-norm test)."""

@@ -33,10 +33,15 @@ def parse():
     p.add_argument("--seed", type=int, default=256)
     p.add_argument("--fapply-reps", type=int, default=50)
     p.add_argument("--skip-cpu", action="store_true")
-    p.add_argument("--methods", default="fp64v,mixedv,fgmres-fp64v,fgmres-mixedv",
+    p.add_argument("--methods", default="fp64v,mixedv,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
                         "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps); "
-                        "fgmres-*: flexible GMRES(16) preconditioned by one such cycle")
+                        "fgmres-*: flexible GMRES(16) preconditioned by one such cycle; "
+                        "fgmres-learned-*: the same with the learned correction stage after each cycle")
+    p.add_argument("--train-n", type=int, default=128, help="cells per direction of the training fields")
+    p.add_argument("--train-fields", type=int, default=2)
+    p.add_argument("--train-pairs", type=int, default=8, help="(residual, error) pairs per training field")
+    p.add_argument("--train-steps", type=int, default=300, help="Adam steps per training field")
     p.add_argument("--verbose", type=int, default=0)
     p.add_argument("--fapply-only", action="store_true",
                    help="only time Fapply on the finest level (plus a device copy of the same bytes), "
@@ -87,7 +92,8 @@ def main():
     import numpy as np
     import torch
     from alamo_amd import problems as P
-    from alamo_amd.operator import Elastic, Linear, Krylov, PREC_FP64, PREC_MIXED, CYCLE_F, CYCLE_V
+    from alamo_amd.operator import (Elastic, Linear, Krylov, Net, PREC_FP64, PREC_MIXED, CYCLE_F, CYCLE_V,
+                                    PRECOND_MG, PRECOND_LEARNED)
 
     dev = torch.device("cuda:0")
     n = a.n
@@ -101,9 +107,16 @@ def main():
     results = {}
     rhs = None
     fapply = None
+    net, train_info = None, None
     for meth in a.methods.split(","):
         krylov = meth.startswith("fgmres-")
         base = meth[len("fgmres-"):] if krylov else meth
+        learned = base.startswith("learned-")
+        if learned:
+            base = base[len("learned-"):]
+            if net is None:
+                net, train_info = train_network(a, P, Elastic, Net, PREC_MIXED if base.startswith("mixed") else PREC_FP64,
+                                                CYCLE_V if base.endswith("v") else CYCLE_F)
         prec = PREC_MIXED if base.startswith("mixed") else PREC_FP64
         vcyc = base.endswith("v")
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
@@ -117,7 +130,10 @@ def main():
             op.NewtonRHS(torch.zeros_like(b), b, T(pr["F0"]), rhs)
             del b
         sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
-        lin = Krylov(op, restart=16, verbose=a.verbose) if krylov else Linear(op)
+        if learned:
+            net.attach(op)
+        lin = (Krylov(op, restart=16, precond=PRECOND_LEARNED if learned else PRECOND_MG, verbose=a.verbose)
+               if krylov else Linear(op))
         ok = True
         for _ in range(a.warmup):
             sol.zero_()
@@ -133,6 +149,8 @@ def main():
         results[meth] = dict(seconds=(te - ts) / len(stats), iterations=stats[-1]["iterations"],
                              final_rel_res=stats[-1]["final_res_inf"] / max(stats[-1]["rhs_inf"], 1e-300),
                              converged=all(s["converged"] for s in stats))
+        if learned:
+            results[meth]["fallback_events"] = stats[-1]["fallback_events"]
         print("%s: %.4f s/solve, %d iterations" % (meth, results[meth]["seconds"], results[meth]["iterations"]),
               file=sys.stderr, flush=True)
         if meth == "fp64" or fapply is None:
@@ -190,6 +208,7 @@ def main():
                         % (a.corr, a.contrast, a.seed, n, n + 1),
             "method": best,
             "methods": results,
+            "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
             "fapply_ms": fapply["ms"] if fapply else None,
         },
@@ -206,9 +225,33 @@ def main():
     print(json.dumps(out))
 
 
+def train_network(a, P, Elastic, Net, precision, cycle):
+    """Train the correction network on random two-phase fields of train_n^3 cells (seeds other
+    than the benchmark field's), outside the timed region."""
+    t0 = time.time()
+    net = Net(seed=12345)
+    stats = []
+    for f in range(a.train_fields):
+        pr = P.two_phase_problem(n=a.train_n, corr_cells=a.corr * a.train_n / a.n, contrast=a.contrast,
+                                 seed=a.seed + 1 + f)
+        vc = cycle == 0
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
+                     pre_smooth=1 if vc else 2, post_smooth=1 if vc else 2)
+        op.SetModel(pr["C"])
+        stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
+                               solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))
+        del op
+    info = {"fields": a.train_fields, "cells": a.train_n, "pairs_per_field": a.train_pairs,
+            "steps_per_field": a.train_steps, "first_loss": stats[0]["first_loss"],
+            "last_loss": stats[-1]["last_loss"], "seconds": time.time() - t0}
+    print("network trained: %s" % json.dumps(info), file=sys.stderr, flush=True)
+    return net, info
+
+
 def cpu_baseline(pr, rhs, gpu_iters, cycle=1):
-    """The oracle's FP64 multigrid (same F-cycle algorithm) on the host cores: one setup plus
-    one F-cycle at the full size, timed, and scaled to the GPU solve's iteration count."""
+    """The oracle's FP64 multigrid (the same cycle as the GPU method it is set beside) on the host
+    cores: one setup plus one cycle at the full size, timed, and scaled to the GPU solve's
+    iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as ora
     from alamo_amd import problems as P
@@ -223,8 +266,9 @@ def cpu_baseline(pr, rhs, gpu_iters, cycle=1):
     per_cycle = max(t_one - t_setup, 1e-9)
     est = t_setup + gpu_iters * per_cycle
     return {"value": est, "unit": "s", "cores": cores, "kind": "port",
-            "sample": "oracle FP64 F-cycle multigrid on the full 257^3 field: setup+residual (%.2fs) and one "
-                      "F-cycle (%.2fs) timed, scaled to %d cycles" % (t_setup, per_cycle, gpu_iters)}
+            "sample": "oracle FP64 %s-cycle multigrid on the full 257^3 field: setup+residual (%.2fs) and one "
+                      "cycle (%.2fs) timed, scaled to the GPU solve's %d cycles"
+                      % ("V" if cycle == 0 else "F", t_setup, per_cycle, gpu_iters)}
 
 
 if __name__ == "__main__":

@@ -188,6 +188,10 @@ int alamo_net_train(alamo_net* net, alamo_mg* h, const alamo_train_opts* opts, a
 int alamo_net_save(alamo_net* net, const char* path);
 int alamo_net_load(alamo_net* net, const char* path);
 int alamo_mg_set_net(alamo_mg* h, alamo_net* net);
+/* z = P(r): one application of the preconditioner kind (ALAMO_PRECOND_MG: one multigrid cycle
+ * from zero, the operator's MLMG settings; ALAMO_PRECOND_LEARNED: that cycle plus the network
+ * stage), device pointers, synchronous */
+int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z);
 /* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
 
