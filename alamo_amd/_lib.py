@@ -94,6 +94,10 @@ SIGNATURES = {
     "alamo_net_load": (c_int, [c_void_p, ctypes.c_char_p]),
     "alamo_mg_set_net": (c_int, [c_void_p, c_void_p]),
     "alamo_mg_precond": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "alamo_amr_fine_mask": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_amr_reflux": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 

@@ -22,6 +22,7 @@
 #include "kernels_iso.h"
 #include "kernels_iso_rows.h"
 #include "krylov.h"
+#include "amr.h"
 
 using namespace alamo;
 
@@ -469,7 +470,15 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
             double* vj = V + (int64_t)j * n;
             double* zj = Z + (int64_t)j * n;
             double* w = V + (int64_t)(j + 1) * n;
-            TRY(precond_apply(h, kind, vj, zj));
+            const int prc = precond_apply(h, kind, vj, zj);
+            if (prc == NET_NONFINITE) {
+                // zj holds the plain cycle's result; the plain cycle takes over for good
+                kind = ALAMO_PRECOND_MG;
+                fallbacks++;
+                if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
+            } else if (prc != ALAMO_OK) {
+                return prc;
+            }
             launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             // CGS2
             TRY(multidot(h, w, V, n, j + 1, d1));
@@ -860,9 +869,10 @@ int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z) {
     if (h->L.back().g.tot > 17 * 17 * 17)
         FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
     if (!h->prepared) TRY(prepare(h));
-    TRY(precond_apply(h, kind, r, z));
+    const int rc = precond_apply(h, kind, r, z);
     HIPCHK(hipStreamSynchronize(h->stream));
-    return ALAMO_OK;
+    if (rc == NET_NONFINITE) FAIL(ALAMO_ERR_NAN, "network output is not finite (z holds the plain cycle's result)");
+    return rc;
 }
 
 int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
@@ -1264,6 +1274,105 @@ int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
     (void)hipFree(a);
     (void)hipFree(b);
     (void)hipFree(d);
+    return ALAMO_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Two-level AMR helpers: C ABI
+// ===========================================================================
+static int amr_dom(const int* nc, amr::Dom& d) {
+    if (!nc) FAIL(ALAMO_ERR_ARG, "null ncell");
+    for (int q = 0; q < 3; q++) {
+        if (nc[q] < 1) FAIL(ALAMO_ERR_ARG, "coarse ncell must be positive");
+        d.nc[q] = nc[q];
+    }
+    d.N0 = nc[0] + 1;
+    d.N1 = nc[1] + 1;
+    d.tot = d.N0 * d.N1 * (nc[2] + 1);
+    return ALAMO_OK;
+}
+
+static int amr_check_box(const amr::Dom& d, const int* flo, const int* fhi) {
+    if (!flo || !fhi) FAIL(ALAMO_ERR_ARG, "null fine box");
+    for (int q = 0; q < 3; q++) {
+        if (flo[q] % 2 != 0 || fhi[q] % 2 != 1 || flo[q] > fhi[q])
+            FAIL(ALAMO_ERR_ARG, "fine boxes must be ratio-2 aligned cell boxes (lo even, hi odd, lo <= hi)");
+        if (flo[q] < 0 || fhi[q] > 2 * d.nc[q] - 1) FAIL(ALAMO_ERR_ARG, "fine box outside the refined domain");
+    }
+    return ALAMO_OK;
+}
+
+static amr::FBox amr_fbox(const amr::Dom& d, const int* flo, const int* fhi) {
+    amr::FBox fb;
+    fb.vf = 1;
+    for (int q = 0; q < 3; q++) {
+        fb.vlo[q] = flo[q] / 2;
+        fb.vhi[q] = (fhi[q] + 1) / 2;
+        fb.blo[q] = std::max(fb.vlo[q] - 1, 0);
+        fb.bhi[q] = std::min(fb.vhi[q] + 1, d.nc[q]);
+        fb.flo[q] = flo[q];
+        fb.gf[q] = fhi[q] - flo[q] + 4;
+        fb.vf *= fb.gf[q];
+    }
+    return fb;
+}
+
+extern "C" {
+
+int alamo_amr_fine_mask(const int* ncell_crse, int nbox, const int* fine_lo, const int* fine_hi, int* mask,
+                        void* stream) {
+    amr::Dom d;
+    TRY(amr_dom(ncell_crse, d));
+    if (!mask) FAIL(ALAMO_ERR_ARG, "null mask");
+    if (nbox < 0 || nbox > amr::MAX_BOXES) FAIL(ALAMO_ERR_ARG, "nbox must be in [0, 64]");
+    amr::Boxes b;
+    b.n = nbox;
+    for (int t = 0; t < nbox; t++) {
+        TRY(amr_check_box(d, fine_lo + 3 * t, fine_hi + 3 * t));
+        for (int q = 0; q < 3; q++) {
+            b.lo[t][q] = fine_lo[3 * t + q] / 2;
+            b.hi[t][q] = (fine_hi[3 * t + q] + 1) / 2 - 1;
+        }
+    }
+    amr::k_fine_mask<<<nblocks(d.tot), 256, 0, (hipStream_t)stream>>>(d, b, mask);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_amr_reflux(const int* ncell_crse, const int* fine_lo, const int* fine_hi, const int* mask, double* crse_res,
+                     const double* fine_res, void* stream) {
+    amr::Dom d;
+    TRY(amr_dom(ncell_crse, d));
+    TRY(amr_check_box(d, fine_lo, fine_hi));
+    if (!mask || !crse_res || !fine_res) FAIL(ALAMO_ERR_ARG, "null argument");
+    const amr::FBox fb = amr_fbox(d, fine_lo, fine_hi);
+    const int64_t nv = (int64_t)(fb.vhi[0] - fb.vlo[0] + 1) * (fb.vhi[1] - fb.vlo[1] + 1) * (fb.vhi[2] - fb.vlo[2] + 1);
+    amr::k_reflux<<<nblocks(nv), 256, 0, (hipStream_t)stream>>>(d, fb, mask, crse_res, fine_res);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_amr_average_down(const int* ncell_crse, const int* fine_lo, const int* fine_hi, double* crse_sol,
+                           const double* fine_sol, void* stream) {
+    amr::Dom d;
+    TRY(amr_dom(ncell_crse, d));
+    TRY(amr_check_box(d, fine_lo, fine_hi));
+    if (!crse_sol || !fine_sol) FAIL(ALAMO_ERR_ARG, "null argument");
+    const amr::FBox fb = amr_fbox(d, fine_lo, fine_hi);
+    const int64_t nv = (int64_t)(fb.vhi[0] - fb.vlo[0] + 1) * (fb.vhi[1] - fb.vlo[1] + 1) * (fb.vhi[2] - fb.vlo[2] + 1);
+    amr::k_average_down<<<nblocks(nv), 256, 0, (hipStream_t)stream>>>(d, fb, crse_sol, fine_sol);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_amr_fixup_residual_mask(const int* ncell_crse, const int* mask, int* resmsk, void* stream) {
+    amr::Dom d;
+    TRY(amr_dom(ncell_crse, d));
+    if (!mask || !resmsk) FAIL(ALAMO_ERR_ARG, "null argument");
+    amr::k_fixup_resmask<<<nblocks(d.tot), 256, 0, (hipStream_t)stream>>>(d.tot, mask, resmsk);
+    HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
 

@@ -16,6 +16,9 @@
 
 #include "learned.h"
 
+// internal status of the network stage: its output was not finite (z still holds the plain cycle)
+constexpr int NET_NONFINITE = 100;
+
 struct NetLayer {
     int cin, cout, cin_real, level, relu;
     int64_t woff, boff;  // offsets into the parameter vector
@@ -344,8 +347,9 @@ static int net_correct(alamo_net* n, alamo_mg* h, const double* r, double* z, do
     launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, y, nullptr, Ay, 0.0, h->stream);
     double d[2];
     TRY(multidot(h, Ay, q, n3, 2, d));  // <Ay, q>, <Ay, Ay>
+    if (!std::isfinite(d[0]) || !std::isfinite(d[1])) return NET_NONFINITE;  // z keeps the plain cycle
+    if (!(d[1] > 0.0)) return ALAMO_OK;                                       // y = 0: nothing to add
     const double alpha = d[0] / d[1];
-    if (!(d[1] > 0.0) || !std::isfinite(alpha)) return ALAMO_OK;
     k_axpy<double, double><<<gs_blocks(n3), 256, 0, h->stream>>>(z, y, n3, alpha);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;

@@ -153,6 +153,28 @@ int alamo_krylov_opts_default(alamo_krylov_opts* o);
 int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        const alamo_krylov_opts* opts, alamo_solve_stats* stats);
 
+/* Two-level AMR helpers of Operator<Grid::Node> (src/Operator/Operator.cpp).  The coarse level
+ * is one box covering the non-periodic domain of ncell_crse[3] cells (nodes (n+1)^3, i fastest).
+ * A fine box is an inclusive cell box in fine index space (refinement ratio 2; lo even, hi odd,
+ * inside [0, 2 n - 1]); fine fields live on its nodal box grown by one node, [lo - 1, hi + 2]^3,
+ * as 3 SoA planes (the valid nodes plus the one ghost layer the reflux stencil reads).
+ * Device pointers; work is queued on `stream`. */
+/* m_nd_fine_mask of the coarse level (read at Operator.cpp:236,553; built by
+ * amrex::MLNodeLinOp::buildMasks): per coarse node 0 crse_node, 1 crse_fine_node, 2 fine_node;
+ * fine_lo/fine_hi are host arrays of nbox (<= 64) boxes x 3 ints. */
+int alamo_amr_fine_mask(const int* ncell_crse, int nbox, const int* fine_lo, const int* fine_hi, int* mask,
+                        void* stream);
+/* Operator<Grid::Node>::reflux (Operator.cpp:526-633) for one fine box: the coarse residual on the
+ * box's crse_fine / fine nodes becomes the restricted fine residual. */
+int alamo_amr_reflux(const int* ncell_crse, const int* fine_lo, const int* fine_hi, const int* mask, double* crse_res,
+                     const double* fine_res, void* stream);
+/* Operator<Grid::Node>::averageDownSolutionRHS (Operator.cpp:435-447): nodal injection of the fine
+ * box's solution onto the coarse nodes it covers. */
+int alamo_amr_average_down(const int* ncell_crse, const int* fine_lo, const int* fine_hi, double* crse_sol,
+                           const double* fine_sol, void* stream);
+/* Operator<Grid::Node>::fixUpResidualMask (Operator.cpp:230-251): resmsk = 1 on crse_fine nodes. */
+int alamo_amr_fixup_residual_mask(const int* ncell_crse, const int* mask, int* resmsk, void* stream);
+
 /* Learned correction network (no reference counterpart).  A 3-level convolutional network
  * (16/32/64 channels, 3x3x3 kernels, BF16 weights and activations, FP32 accumulation on the
  * matrix cores) that maps the residual left by one multigrid cycle, together with log(lambda),

@@ -305,3 +305,49 @@ def net_layer_slices():
         out.append((slice(off, off + nw), (27, cout, cin), slice(off + nw, off + nw + cout)))
         off += nw + cout
     return out
+
+
+# ---------------------------------------------------------------------------
+# two-level AMR helpers (oracle/amr_oracle.cpp); fine boxes are inclusive cell boxes in fine
+# index space, fine fields live on the fine nodal box grown by one node
+# ---------------------------------------------------------------------------
+def _boxes(fine_lo, fine_hi):
+    lo = _c(np.atleast_2d(fine_lo), np.int32)
+    hi = _c(np.atleast_2d(fine_hi), np.int32)
+    return lo, hi
+
+
+def amr_fine_mask(ncell_crse, fine_lo, fine_hi):
+    nc = _c(ncell_crse, np.int32)
+    lo, hi = _boxes(fine_lo, fine_hi)
+    mask = np.zeros(int(np.prod(nc + 1)), dtype=np.int32)
+    lib().oamr_fine_mask(_i(nc), len(lo), _i(lo), _i(hi), _i(mask))
+    return mask
+
+
+def amr_grown_fine_nodes(flo, fhi):
+    return [int(fhi[d] - flo[d] + 4) for d in range(3)]
+
+
+def amr_reflux(ncell_crse, flo, fhi, mask, crse, fine):
+    nc = _c(ncell_crse, np.int32)
+    flo = _c(flo, np.int32); fhi = _c(fhi, np.int32)
+    out = _c(crse).copy()
+    lib().oamr_reflux(_i(nc), _i(flo), _i(fhi), _i(_c(mask, np.int32)), _d(out), _d(_c(fine)))
+    return out
+
+
+def amr_average_down(ncell_crse, flo, fhi, crse, fine):
+    nc = _c(ncell_crse, np.int32)
+    flo = _c(flo, np.int32); fhi = _c(fhi, np.int32)
+    out = _c(crse).copy()
+    lib().oamr_average_down(_i(nc), _i(flo), _i(fhi), _d(out), _d(_c(fine)))
+    return out
+
+
+def amr_fixup_residual_mask(mask, resmsk):
+    out = _c(resmsk, np.int32).copy()
+    f = lib().oamr_fixup_residual_mask
+    f.argtypes = [ctypes.c_int64, I, I]
+    f(len(out), _i(_c(mask, np.int32)), _i(out))
+    return out
