@@ -407,7 +407,19 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
 }
 
 // d[0..nv) = w . V_i  (device result copied to the pinned host buffer `out`)
+// device scratch of the multi-dot / coefficient kernels, allocated on first use (the
+// Krylov solver and the network stage both reach multidot)
+static int krylov_scratch(alamo_mg* h) {
+    if (h->kcoef) return ALAMO_OK;
+    TRY(dalloc(h, &h->kcoef, KRY_MAXV));
+    TRY(dalloc(h, &h->kpart, (int64_t)KRY_MAXV * KRY_DOT_BLOCKS));
+    TRY(dalloc(h, &h->kdots, KRY_MAXV));
+    return ALAMO_OK;
+}
+
 static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out) {
+    if (nv < 1 || nv > KRY_MAXV) FAIL(ALAMO_ERR_ARG, "multidot: vector count out of range");
+    TRY(krylov_scratch(h));
     const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
     launch_multidot(w, V, n, n, nv, nb, h->kpart, h->stream);
     k_sum_rows<<<nv, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
@@ -418,6 +430,7 @@ static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, in
 }
 
 static int upload_coef(alamo_mg* h, const double* c, int nv) {
+    TRY(krylov_scratch(h));
     for (int i = 0; i < nv; i++) h->red_host[i] = c[i];
     HIPCHK(hipMemcpyAsync(h->kcoef, h->red_host, nv * sizeof(double), hipMemcpyHostToDevice, h->stream));
     // the pinned staging buffer is reused by the next reduction: wait for the copy
@@ -437,11 +450,7 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     if (h->kry_m < m) {
         TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * n));
         TRY(dalloc(h, &h->kZ, (int64_t)m * n));
-        if (!h->kcoef) {
-            TRY(dalloc(h, &h->kcoef, KRY_MAXV));
-            TRY(dalloc(h, &h->kpart, (int64_t)KRY_MAXV * KRY_DOT_BLOCKS));
-            TRY(dalloc(h, &h->kdots, KRY_MAXV));
-        }
+        TRY(krylov_scratch(h));
         h->kry_m = m;
     }
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -48,62 +48,73 @@ __device__ __forceinline__ void off3(int o, int& dx, int& dy, int& dz) {
 
 // ---------------------------------------------------------------------------
 // Forward convolution: out[v][co] = bias[co] + sum_{o, ci} W[o][co][ci] in[v + o][ci]
-// (zero outside the domain), optional ReLU.  One wavefront owns 16 consecutive voxels of
-// an x row and all output channels (Cout / 16 accumulator tiles).
+// (zero outside the domain), optional ReLU.  One wavefront owns a row of CONV_MT x 16
+// consecutive voxels of an x row and all NT x 16 output channels: per (offset, 16-channel
+// input chunk) it loads the NT weight fragments once and applies them to its CONV_MT voxel
+// tiles (CONV_MT x NT accumulator tiles).  A whole source row outside the domain (y or z
+// padding) is skipped in wave-uniform control flow.
 // ---------------------------------------------------------------------------
 constexpr int CONV_WAVES = 4;  // wavefronts per workgroup
+constexpr int CONV_MT = 4;     // 16-voxel tiles per wavefront
 
-template <bool OUT_F32>
-__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin, int Cout,
-                                                               const uint16_t* __restrict__ in,
+template <int NT, bool OUT_F32>
+__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin, const uint16_t* __restrict__ in,
                                                                const uint16_t* __restrict__ W,
                                                                const float* __restrict__ bias, void* __restrict__ out,
                                                                int relu) {
+    constexpr int Cout = 16 * NT;
+    constexpr int ROW = 16 * CONV_MT;
     const int lane = threadIdx.x % 64;
     const int64_t tile = (int64_t)blockIdx.x * CONV_WAVES + threadIdx.x / 64;
-    const int tx = (g.N[0] + 15) / 16;
+    const int tx = (g.N[0] + ROW - 1) / ROW;
     const int64_t ntile = (int64_t)tx * g.N[1] * g.N[2];
     if (tile >= ntile) return;
-    const int x0 = (int)(tile % tx) * 16;
+    const int x0 = (int)(tile % tx) * ROW;
     const int y = (int)((tile / tx) % g.N[1]);
     const int z = (int)(tile / ((int64_t)tx * g.N[1]));
     const int m = lane % 16, kq = 4 * (lane / 16);
-    const int NT = Cout / 16;
-    f4 acc[4];
+    f4 acc[CONV_MT][NT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < CONV_MT; a++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[a][t] = f4{0.f, 0.f, 0.f, 0.f};
     for (int o = 0; o < 27; o++) {
         int dx, dy, dz;
         off3(o, dx, dy, dz);
-        const int xs = x0 + m + dx, ys = y + dy, zs = z + dz;
-        const bool inb = xs >= 0 && xs < g.N[0] && ys >= 0 && ys < g.N[1] && zs >= 0 && zs < g.N[2];
-        const int64_t vs = xs + (int64_t)g.N[0] * (ys + (int64_t)g.N[1] * zs);
+        const int ys = y + dy, zs = z + dz;
+        if (ys < 0 || ys >= g.N[1] || zs < 0 || zs >= g.N[2]) continue;  // uniform: zero padding row
+        const int64_t rowv = (int64_t)g.N[0] * (ys + (int64_t)g.N[1] * zs);
         for (int ci0 = 0; ci0 < Cin; ci0 += 16) {
-            s4 a = s4{0, 0, 0, 0};
-            if (inb) a = *(const s4*)(in + vs * Cin + ci0 + kq);
+            s4 bw[NT];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                if (t >= NT) break;
-                const s4 b = *(const s4*)(W + ((int64_t)o * Cout + t * 16 + m) * Cin + ci0 + kq);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int t = 0; t < NT; t++) bw[t] = *(const s4*)(W + ((int64_t)o * Cout + t * 16 + m) * Cin + ci0 + kq);
+#pragma unroll
+            for (int a = 0; a < CONV_MT; a++) {
+                const int xs = x0 + 16 * a + m + dx;
+                s4 av = s4{0, 0, 0, 0};
+                if (xs >= 0 && xs < g.N[0]) av = *(const s4*)(in + (rowv + xs) * Cin + ci0 + kq);
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc[a][t], 0, 0, 0);
             }
         }
     }
+    const int64_t rowo = (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        if (t >= NT) break;
+    for (int t = 0; t < NT; t++) {
         const int co = t * 16 + m;
         const float bb = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int xv = x0 + kq + i;
-            if (xv >= g.N[0]) continue;
-            const int64_t v = xv + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
-            float r = acc[t][i] + bb;
-            if (relu) r = fmaxf(r, 0.f);
-            if (OUT_F32) ((float*)out)[v * Cout + co] = r;
-            else ((uint16_t*)out)[v * Cout + co] = f2bf(r);
-        }
+        for (int a = 0; a < CONV_MT; a++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int xv = x0 + 16 * a + kq + i;
+                if (xv >= g.N[0]) continue;
+                const int64_t v = rowo + xv;
+                float r = acc[a][t][i] + bb;
+                if (relu) r = fmaxf(r, 0.f);
+                if (OUT_F32) ((float*)out)[v * Cout + co] = r;
+                else ((uint16_t*)out)[v * Cout + co] = f2bf(r);
+            }
     }
 }
 
@@ -163,16 +174,23 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(Grid3 g, int Cin, int Cout,
     }
 }
 
-// dB[co] += sum_v dout[v][co]
+// dB[co] += sum_v dout[v][co]; Cout divides 256: thread t reads channel t % Cout of voxel
+// t / Cout (coalesced rows), then the block folds its 256 / Cout partial sums per channel
 __global__ void __launch_bounds__(256) k_bias_grad(int64_t V, int Cout, const uint16_t* __restrict__ dout,
                                                    float* __restrict__ dB) {
-    const int co = blockIdx.y;
+    __shared__ float sh[256];
+    const int t = threadIdx.x, per = 256 / Cout;
+    const int c = t % Cout;
     float s = 0.f;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
-        s += bf2f(dout[v * Cout + co]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (threadIdx.x % 64 == 0) atomicAdd(&dB[co], s);
+    for (int64_t v = (int64_t)blockIdx.x * per + t / Cout; v < V; v += (int64_t)gridDim.x * per)
+        s += bf2f(dout[v * Cout + c]);
+    sh[t] = s;
+    __syncthreads();
+    if (t < Cout) {
+        float a = 0.f;
+        for (int q = 0; q < per; q++) a += sh[t + q * Cout];
+        atomicAdd(&dB[t], a);
+    }
 }
 
 // Weights of the backward-data convolution: Wb[26 - o][ci][co] = W[o][co][ci]

@@ -115,14 +115,21 @@ static int net_bufs(alamo_net* n, const int* N0) {
 
 static inline unsigned nb256(int64_t n) { return (unsigned)std::min<int64_t>(65535 * 8, (n + 255) / 256); }
 
+template <int NT>
+static void conv_launch_nt(alamo_net* n, const net::Grid3& g, int cin, const uint16_t* in, const uint16_t* W,
+                           const float* bias, void* out, bool out_f32, int relu, unsigned grid) {
+    if (out_f32) net::k_conv3_fwd<NT, true><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, in, W, bias, out, relu);
+    else net::k_conv3_fwd<NT, false><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, in, W, bias, out, relu);
+}
+
 static void conv_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, const uint16_t* in, const uint16_t* W,
                         const float* bias, void* out, bool out_f32, int relu) {
-    const int64_t tiles = (int64_t)((g.N[0] + 15) / 16) * g.N[1] * g.N[2];
+    const int row = 16 * net::CONV_MT;
+    const int64_t tiles = (int64_t)((g.N[0] + row - 1) / row) * g.N[1] * g.N[2];
     const unsigned grid = (unsigned)((tiles + net::CONV_WAVES - 1) / net::CONV_WAVES);
-    if (out_f32)
-        net::k_conv3_fwd<true><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, cout, in, W, bias, out, relu);
-    else
-        net::k_conv3_fwd<false><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, cout, in, W, bias, out, relu);
+    if (cout == 16) conv_launch_nt<1>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
+    else if (cout == 32) conv_launch_nt<2>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
+    else conv_launch_nt<4>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
 }
 
 static void wgrad_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, const uint16_t* in,
@@ -130,8 +137,7 @@ static void wgrad_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, c
     const int64_t nchunk = (g.V + net::WGRAD_CHUNK - 1) / net::WGRAD_CHUNK;
     const int64_t jobs = (int64_t)27 * (cout / 16) * (cin / 16) * nchunk;
     net::k_conv3_wgrad<<<(unsigned)((jobs + 3) / 4), 256, 0, n->s>>>(g, cin, cout, in, dout, dW);
-    dim3 gb((unsigned)std::min<int64_t>(1024, (g.V + 255) / 256), cout);
-    net::k_bias_grad<<<gb, 256, 0, n->s>>>(g.V, cout, dout, dB);
+    net::k_bias_grad<<<(unsigned)std::min<int64_t>(2048, (g.V * cout + 255) / 256), 256, 0, n->s>>>(g.V, cout, dout, dB);
 }
 
 static int net_sync_weights(alamo_net* n) {
