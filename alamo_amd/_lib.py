@@ -99,6 +99,7 @@ SIGNATURES = {
     "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_net_layer_times": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
 }
 
 _lib = None

@@ -1268,6 +1268,38 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     return rc;
 }
 
+// Mean duration of each convolution layer's forward kernel at the handle's level-0 size,
+// timed with HIP events on the handle's stream (reps launches per layer); ms[6].
+int alamo_net_layer_times(alamo_net* n, alamo_mg* h, int reps, double* ms) {
+    if (!n || !h || !ms || reps < 1) FAIL(ALAMO_ERR_ARG, "bad argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    TRY(net_level0(n, h));
+    TRY(net_sync_weights(n));
+    NetBufs& B = n->B;
+    struct L { const net::Grid3* g; int cin, cout; const uint16_t* in; void* out; bool f32; int relu; };
+    const L layers[6] = {{&B.g[0], 16, 16, B.x, B.a0, false, 1},  {&B.g[1], 16, 32, B.d1, B.a1, false, 1},
+                         {&B.g[2], 32, 64, B.d2, B.a2, false, 1}, {&B.g[2], 64, 32, B.a2, B.b2, false, 0},
+                         {&B.g[1], 32, 16, B.u1, B.b1, false, 1}, {&B.g[0], 16, 16, B.u0, B.y, true, 0}};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int l = 0; l < 6; l++) {
+        const L& y = layers[l];
+        conv_launch(n, *y.g, y.cin, y.cout, y.in, n->wf[l], n->w + n->L[l].boff, y.out, y.f32, y.relu);  // warm
+        HIPCHK(hipEventRecord(e0, n->s));
+        for (int r = 0; r < reps; r++)
+            conv_launch(n, *y.g, y.cin, y.cout, y.in, n->wf[l], n->w + n->L[l].boff, y.out, y.f32, y.relu);
+        HIPCHK(hipEventRecord(e1, n->s));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        ms[l] = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ALAMO_OK;
+}
+
 // MFMA lane-layout probe (one 16x16x16 bf16 product of host matrices), for the tests.
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
     uint16_t *a = nullptr, *b = nullptr;

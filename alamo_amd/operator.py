@@ -280,6 +280,12 @@ class Net:
         check(lib().alamo_net_train(self.h, op.h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
 
+    def layer_times(self, op, reps=5):
+        """mean forward-kernel milliseconds of each conv layer at op's level-0 size"""
+        ms = np.zeros(6)
+        check(lib().alamo_net_layer_times(self.h, op.h, int(reps), ms.ctypes.data_as(ctypes.c_void_p)))
+        return ms
+
     def save(self, path):
         check(lib().alamo_net_save(self.h, str(path).encode()))
 

@@ -151,6 +151,7 @@ def main():
                              converged=all(s["converged"] for s in stats))
         if learned:
             results[meth]["fallback_events"] = stats[-1]["fallback_events"]
+            results[meth]["layers"] = layer_report(net.layer_times(op, reps=5), n)
         print("%s: %.4f s/solve, %d iterations" % (meth, results[meth]["seconds"], results[meth]["iterations"]),
               file=sys.stderr, flush=True)
         if meth == "fp64" or fapply is None:
@@ -223,6 +224,25 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of MI355X
+
+
+def layer_report(ms, n):
+    """per-layer throughput of the network's forward convolutions at n^3 cells: FLOPs
+    2 27 Cin Cout per voxel, bytes (Cin + Cout) 2 per voxel, bound min(MFMA peak, HBM x AI)"""
+    spec = [(16, 16, 0), (16, 32, 1), (32, 64, 2), (64, 32, 2), (32, 16, 1), (16, 16, 0)]
+    out = []
+    for (cin, cout, lev), t in zip(spec, ms):
+        V = (n // (2 ** lev) + 1) ** 3
+        flops = 2.0 * 27 * cin * cout * V
+        byts = (cin + cout) * 2.0 * V
+        bound = min(MFMA_BF16_PEAK_TFLOPS, HBM_PEAK_GBS * flops / byts / 1e3)
+        tf = flops / (t * 1e-3) / 1e12
+        out.append({"cin": cin, "cout": cout, "level": lev, "ms": t, "tflops": tf, "bound_tflops": bound,
+                    "frac": tf / bound})
+    return out
 
 
 def train_network(a, P, Elastic, Net, precision, cycle):

@@ -214,6 +214,9 @@ int alamo_mg_set_net(alamo_mg* h, alamo_net* net);
  * from zero, the operator's MLMG settings; ALAMO_PRECOND_LEARNED: that cycle plus the network
  * stage), device pointers, synchronous */
 int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z);
+/* mean forward-kernel time of each of the 6 convolution layers at h's level-0 size, HIP events on
+ * h's stream, reps launches each: ms[6] (for throughput and roofline reports) */
+int alamo_net_layer_times(alamo_net* net, alamo_mg* h, int reps, double* ms);
 /* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
 
