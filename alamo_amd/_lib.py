@@ -69,6 +69,7 @@ SIGNATURES = {
     "alamo_mg_fapply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_diagonal": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_fsmooth": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_normalize": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_residual": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_restrict": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_interpolate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),

@@ -787,6 +787,16 @@ int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* st
     return ALAMO_OK;
 }
 
+int alamo_mg_normalize(alamo_mg* h, int amrlev, int mglev, double* x, void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    if (!h->prepared) FAIL(ALAMO_ERR_STATE, "Operator::Diagonal() must be called before using normalize");
+    if (!x) FAIL(ALAMO_ERR_ARG, "null field");
+    LevelBuf& lb = h->L[mglev];
+    k_divide<<<gs_blocks(3 * lb.g.tot), 256, 0, (hipStream_t)stream>>>(x, lb.D, 3 * lb.g.tot);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
 int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream) {
     TRY(check_level(h, amrlev, mglev));
     LevelBuf& lb = h->L[mglev];

@@ -253,6 +253,12 @@ __global__ void k_axpy(Ty* __restrict__ y, const Tx* __restrict__ x, int64_t n, 
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         y[q] = (Ty)((double)y[q] + alpha * (double)x[q]);
 }
+// x /= D (Operator<Grid::Node>::normalize)
+template <typename T>
+__global__ void k_divide(T* __restrict__ x, const T* __restrict__ D, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        x[q] /= D[q];
+}
 // y = a + b
 template <typename T>
 __global__ void k_add(T* __restrict__ y, const T* __restrict__ a, const T* __restrict__ b, int64_t n) {

@@ -122,6 +122,10 @@ class Elastic:
     def Diagonal(self, amrlev, mglev, diag, stream=None):
         check(lib().alamo_mg_diagonal(self.h, amrlev, mglev, _ptr(diag), stream))
 
+    def normalize(self, amrlev, mglev, x, stream=None):
+        """Operator<Grid::Node>::normalize (Operator.cpp:156-165): x /= D"""
+        check(lib().alamo_mg_normalize(self.h, amrlev, mglev, _ptr(x), stream))
+
     def Fsmooth(self, amrlev, mglev, x, b, stream=None):
         check(lib().alamo_mg_fsmooth(self.h, amrlev, mglev, _ptr(x), _ptr(b), stream))
 

@@ -107,6 +107,9 @@ int alamo_mg_get_coeff(alamo_mg* h, int mglev, double* C_soa);
 int alamo_mg_fapply(alamo_mg* h, int amrlev, int mglev, double* out, const double* in, void* stream);
 /* Elastic::Diagonal (Elastic.cpp:296-379). */
 int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* stream);
+/* Operator<Grid::Node>::normalize (Operator.cpp:156-165): x /= D on level mglev (the diagonal
+ * computed by prepareForSolve; error if it has not been computed). */
+int alamo_mg_normalize(alamo_mg* h, int amrlev, int mglev, double* x, void* stream);
 /* Operator<Grid::Node>::Fsmooth (Operator.cpp:88-154): sweeps_per_smooth damped-Jacobi sweeps. */
 int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream);
 /* solutionResidual / correctionResidual (Operator.cpp:635-657): r = b - A x. */

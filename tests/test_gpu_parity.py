@@ -92,6 +92,11 @@ def test_diagonal_residual_fsmooth(cuda, ora, sym, ncell):
     op.Fsmooth(0, 0, xd, to_dev(b, cuda))
     xs = ora.jacobi(ncell, dx, sym, C, t, Dref, 2.0 / 3.0, 2, x, b)
     close(to_host(xd), xs, 1e-11)
+    # normalize: x / D with the prepared diagonal
+    op.prepareForSolve()
+    xn = to_dev(x, cuda)
+    op.normalize(0, 0, xn)
+    close(to_host(xn), x / Dref)
 
 
 @pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180)])
