@@ -48,34 +48,35 @@ __device__ __forceinline__ void off3(int o, int& dx, int& dy, int& dz) {
 
 // ---------------------------------------------------------------------------
 // Forward convolution: out[v][co] = bias[co] + sum_{o, ci} W[o][co][ci] in[v + o][ci]
-// (zero outside the domain), optional ReLU.  One wavefront owns a row of CONV_MT x 16
+// (zero outside the domain), optional ReLU.  One wavefront owns a row of MT x 16
 // consecutive voxels of an x row and all NT x 16 output channels: per (offset, 16-channel
-// input chunk) it loads the NT weight fragments once and applies them to its CONV_MT voxel
-// tiles (CONV_MT x NT accumulator tiles).  A whole source row outside the domain (y or z
+// input chunk) it loads the NT weight fragments once and applies them to its MT voxel
+// tiles (MT x NT accumulator tiles).  A whole source row outside the domain (y or z
 // padding) is skipped in wave-uniform control flow.
 // ---------------------------------------------------------------------------
 constexpr int CONV_WAVES = 4;  // wavefronts per workgroup
-constexpr int CONV_MT = 4;     // 16-voxel tiles per wavefront
 
-template <int NT, bool OUT_F32>
-__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin, const uint16_t* __restrict__ in,
+// xb, tx: the wavefronts cover x in [xb, xb + tx * 16 MT) of every row (the launcher runs the
+// bulk of a 4k+1-node row with MT = 4 and its last column with MT = 1)
+template <int NT, int MT, bool OUT_F32>
+__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin, int xb, int tx,
+                                                               const uint16_t* __restrict__ in,
                                                                const uint16_t* __restrict__ W,
                                                                const float* __restrict__ bias, void* __restrict__ out,
                                                                int relu) {
     constexpr int Cout = 16 * NT;
-    constexpr int ROW = 16 * CONV_MT;
+    constexpr int ROW = 16 * MT;
     const int lane = threadIdx.x % 64;
     const int64_t tile = (int64_t)blockIdx.x * CONV_WAVES + threadIdx.x / 64;
-    const int tx = (g.N[0] + ROW - 1) / ROW;
     const int64_t ntile = (int64_t)tx * g.N[1] * g.N[2];
     if (tile >= ntile) return;
-    const int x0 = (int)(tile % tx) * ROW;
+    const int x0 = xb + (int)(tile % tx) * ROW;
     const int y = (int)((tile / tx) % g.N[1]);
     const int z = (int)(tile / ((int64_t)tx * g.N[1]));
     const int m = lane % 16, kq = 4 * (lane / 16);
-    f4 acc[CONV_MT][NT];
+    f4 acc[MT][NT];
 #pragma unroll
-    for (int a = 0; a < CONV_MT; a++)
+    for (int a = 0; a < MT; a++)
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[a][t] = f4{0.f, 0.f, 0.f, 0.f};
     for (int o = 0; o < 27; o++) {
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin,
 #pragma unroll
             for (int t = 0; t < NT; t++) bw[t] = *(const s4*)(W + ((int64_t)o * Cout + t * 16 + m) * Cin + ci0 + kq);
 #pragma unroll
-            for (int a = 0; a < CONV_MT; a++) {
+            for (int a = 0; a < MT; a++) {
                 const int xs = x0 + 16 * a + m + dx;
                 s4 av = s4{0, 0, 0, 0};
                 if (xs >= 0 && xs < g.N[0]) av = *(const s4*)(in + (rowv + xs) * Cin + ci0 + kq);
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin,
         const int co = t * 16 + m;
         const float bb = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int a = 0; a < CONV_MT; a++)
+        for (int a = 0; a < MT; a++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int xv = x0 + 16 * a + kq + i;
@@ -115,6 +116,114 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin,
                 if (OUT_F32) ((float*)out)[v * Cout + co] = r;
                 else ((uint16_t*)out)[v * Cout + co] = f2bf(r);
             }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward convolution for 16 or 32 input channels with the input staged in LDS: a workgroup of
+// CL_TY wavefronts owns a 64 x CL_TY voxel tile of an xy plane and marches CL_KZ planes in z;
+// a ring of three input planes (66 x (CL_TY + 2) voxels with their halo, zero outside the
+// domain) lives in LDS, and each wavefront reads its 27 shifted A fragments from there instead
+// of from L1/L2.  Same arithmetic and output as k_conv3_fwd.
+// ---------------------------------------------------------------------------
+constexpr int CL_TY = 4;
+constexpr int CL_KZ = 8;
+constexpr int CL_H = CL_TY + 2;
+
+template <int CIN, int NT, int MT, bool OUT_F32>
+__global__ void __launch_bounds__(64 * CL_TY) k_conv3_fwd_lds(Grid3 g, int xb, const uint16_t* __restrict__ in,
+                                                            const uint16_t* __restrict__ W,
+                                                            const float* __restrict__ bias, void* __restrict__ out,
+                                                            int relu) {
+    constexpr int Cout = 16 * NT;
+    constexpr int CW = 16 * MT + 2;                          // staged x extent
+    constexpr int CPV = CIN / 8;                             // 16-byte chunks per voxel
+    constexpr int NCH = CL_H * CW * CPV;                     // 16-byte chunks per staged plane
+    constexpr int PER = (NCH + 64 * CL_TY - 1) / (64 * CL_TY);  // chunks per work-item
+    __shared__ __align__(16) uint16_t tile[3][CL_H][CW][CIN];
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int x0 = xb + blockIdx.x * 16 * MT, y0 = blockIdx.y * CL_TY, z0 = blockIdx.z * CL_KZ;
+    const int z1 = min(z0 + CL_KZ, g.N[2]);
+    const int m = lane % 16, kq = 4 * (lane / 16);
+    auto slot = [](int p) { return (p + 3) % 3; };
+    uint4 pre[PER];
+    auto fetch = [&](int zs) {
+#pragma unroll
+        for (int c = 0; c < PER; c++) {
+            const int q = threadIdx.x + 64 * CL_TY * c;
+            const int part = q % CPV, vox = q / CPV;
+            const int xx = vox % CW, yy = vox / CW;
+            const int xs = x0 - 1 + xx, ys = y0 - 1 + yy;
+            pre[c] = make_uint4(0, 0, 0, 0);
+            if (q < NCH && zs >= 0 && zs < g.N[2] && xs >= 0 && xs < g.N[0] && ys >= 0 && ys < g.N[1])
+                pre[c] = *(const uint4*)(in + (xs + (int64_t)g.N[0] * (ys + (int64_t)g.N[1] * zs)) * CIN + 8 * part);
+        }
+    };
+    auto put = [&](int zs) {
+        const int sl = slot(zs);
+#pragma unroll
+        for (int c = 0; c < PER; c++) {
+            const int q = threadIdx.x + 64 * CL_TY * c;
+            if (q < NCH) {
+                const int part = q % CPV, vox = q / CPV;
+                *(uint4*)&tile[sl][vox / CW][vox % CW][8 * part] = pre[c];
+            }
+        }
+    };
+    fetch(z0 - 1);
+    put(z0 - 1);
+    fetch(z0);
+    put(z0);
+    fetch(z0 + 1);
+    const int y = y0 + w;
+    for (int z = z0; z < z1; z++) {
+        // plane z + 1 replaces plane z - 2, which the previous plane's outputs were the last to read
+        __syncthreads();
+        put(z + 1);
+        __syncthreads();
+        if (z + 2 <= z1) fetch(z + 2);  // in flight while this plane computes
+        if (y >= g.N[1]) continue;      // uniform per wavefront; the barriers above are reached by all
+        f4 acc[MT][NT];
+#pragma unroll
+        for (int a = 0; a < MT; a++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[a][t] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int o = 0; o < 27; o++) {
+            int dx, dy, dz;
+            off3(o, dx, dy, dz);
+            const uint16_t(*pl)[CW][CIN] = tile[slot(z + dz)];
+#pragma unroll
+            for (int ci0 = 0; ci0 < CIN; ci0 += 16) {
+                s4 bw[NT];
+#pragma unroll
+                for (int t = 0; t < NT; t++) bw[t] = *(const s4*)(W + ((int64_t)o * Cout + t * 16 + m) * CIN + ci0 + kq);
+#pragma unroll
+                for (int a = 0; a < MT; a++) {
+                    const s4 av = *(const s4*)&pl[w + 1 + dy][16 * a + m + dx + 1][ci0 + kq];
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc[a][t], 0, 0, 0);
+                }
+            }
+        }
+        const int64_t rowo = (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int co = t * 16 + m;
+            const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int a = 0; a < MT; a++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int xv = x0 + 16 * a + kq + i;
+                    if (xv >= g.N[0]) continue;
+                    const int64_t v = rowo + xv;
+                    float r = acc[a][t][i] + bb;
+                    if (relu) r = fmaxf(r, 0.f);
+                    if (OUT_F32) ((float*)out)[v * Cout + co] = r;
+                    else ((uint16_t*)out)[v * Cout + co] = f2bf(r);
+                }
+        }
     }
 }
 

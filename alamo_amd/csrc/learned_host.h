@@ -55,6 +55,12 @@ struct alamo_net {
     std::vector<void*> allocs;
 };
 
+// the LDS-staged convolution for 16 and 32 input channels (ALAMO_CONV_LDS=0 selects the plain kernel)
+static const bool g_conv_lds = [] {
+    const char* e = getenv("ALAMO_CONV_LDS");
+    return !(e && strcmp(e, "0") == 0);
+}();
+
 static int net_alloc(std::vector<void*>& list, void** p, size_t bytes) {
     HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 16)));
     list.push_back(*p);
@@ -115,21 +121,48 @@ static int net_bufs(alamo_net* n, const int* N0) {
 
 static inline unsigned nb256(int64_t n) { return (unsigned)std::min<int64_t>(65535 * 8, (n + 255) / 256); }
 
+// The bulk of a row runs in 64-voxel tiles; when the row is 64 k + r nodes long the last r
+// nodes run in 16-voxel tiles, so 4k+1-node rows waste one 16-voxel tile, not a 64-voxel one.
+template <int NT, int MT>
+static void conv_pass(alamo_net* n, const net::Grid3& g, int cin, int xb, int xe, const uint16_t* in,
+                      const uint16_t* W, const float* bias, void* out, bool out_f32, int relu) {
+    const int tx = (xe - xb + 16 * MT - 1) / (16 * MT);
+    if (tx <= 0) return;
+    // LDS staging pays for 16 input channels and for 32 with at most 32 outputs (with 64 outputs
+    // its register footprint leaves one wavefront per SIMD)
+    if ((cin == 16 || (cin == 32 && NT <= 2)) && g_conv_lds) {
+        dim3 grid(tx, (g.N[1] + net::CL_TY - 1) / net::CL_TY, (g.N[2] + net::CL_KZ - 1) / net::CL_KZ);
+        const unsigned blk = 64 * net::CL_TY;
+        if (cin == 16) {
+            if (out_f32) net::k_conv3_fwd_lds<16, NT, MT, true><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+            else net::k_conv3_fwd_lds<16, NT, MT, false><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+        } else {
+            if (out_f32) net::k_conv3_fwd_lds<32, NT, MT, true><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+            else net::k_conv3_fwd_lds<32, NT, MT, false><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+        }
+        return;
+    }
+    const int64_t tiles = (int64_t)tx * g.N[1] * g.N[2];
+    const unsigned grid = (unsigned)((tiles + net::CONV_WAVES - 1) / net::CONV_WAVES);
+    if (out_f32)
+        net::k_conv3_fwd<NT, MT, true><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, xb, tx, in, W, bias, out, relu);
+    else
+        net::k_conv3_fwd<NT, MT, false><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, xb, tx, in, W, bias, out, relu);
+}
+
 template <int NT>
 static void conv_launch_nt(alamo_net* n, const net::Grid3& g, int cin, const uint16_t* in, const uint16_t* W,
-                           const float* bias, void* out, bool out_f32, int relu, unsigned grid) {
-    if (out_f32) net::k_conv3_fwd<NT, true><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, in, W, bias, out, relu);
-    else net::k_conv3_fwd<NT, false><<<grid, 64 * net::CONV_WAVES, 0, n->s>>>(g, cin, in, W, bias, out, relu);
+                           const float* bias, void* out, bool out_f32, int relu) {
+    const int xm = (g.N[0] / 64) * 64;  // 64-voxel tiles over [0, xm), 16-voxel tiles over [xm, N0)
+    conv_pass<NT, 4>(n, g, cin, 0, xm, in, W, bias, out, out_f32, relu);
+    conv_pass<NT, 1>(n, g, cin, xm, g.N[0], in, W, bias, out, out_f32, relu);
 }
 
 static void conv_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, const uint16_t* in, const uint16_t* W,
                         const float* bias, void* out, bool out_f32, int relu) {
-    const int row = 16 * net::CONV_MT;
-    const int64_t tiles = (int64_t)((g.N[0] + row - 1) / row) * g.N[1] * g.N[2];
-    const unsigned grid = (unsigned)((tiles + net::CONV_WAVES - 1) / net::CONV_WAVES);
-    if (cout == 16) conv_launch_nt<1>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
-    else if (cout == 32) conv_launch_nt<2>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
-    else conv_launch_nt<4>(n, g, cin, in, W, bias, out, out_f32, relu, grid);
+    if (cout == 16) conv_launch_nt<1>(n, g, cin, in, W, bias, out, out_f32, relu);
+    else if (cout == 32) conv_launch_nt<2>(n, g, cin, in, W, bias, out, out_f32, relu);
+    else conv_launch_nt<4>(n, g, cin, in, W, bias, out, out_f32, relu);
 }
 
 static void wgrad_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, const uint16_t* in,
