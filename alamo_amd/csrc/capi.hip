@@ -152,6 +152,23 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
         }
         launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);
     } else if (sym == MAJORMINOR) {
+        if (mode != DIAG && g.n[0] >= 2 && g.n[1] >= 2 && g.n[2] >= 2) {
+            // interior rows slot-major (no spills), boundary rows by the face kernel
+            const int64_t ni = (int64_t)(g.N[0] - 2) * (g.N[1] - 2) * (g.N[2] - 2);
+            const int64_t nb = boundary_count(g);
+            const unsigned gi = (unsigned)((ni + 255) / 256), gb = (unsigned)((nb + 255) / 256);
+            if (mode == APPLY) {
+                k_mm_interior<T, APPLY><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                k_boundary<T, MAJORMINOR, APPLY><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
+            } else if (mode == RESID) {
+                k_mm_interior<T, RESID><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                k_boundary<T, MAJORMINOR, RESID><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
+            } else {
+                k_mm_interior<T, JACOBI><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                k_boundary<T, MAJORMINOR, JACOBI><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
+            }
+            return;
+        }
         launch_node_sym<T, MAJORMINOR>(mode, g, bc, C, u, b, out, omega, s);
     } else {
         launch_node_sym<T, DIAGONAL>(mode, g, bc, C, u, b, out, omega, s);

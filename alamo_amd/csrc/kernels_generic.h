@@ -253,6 +253,29 @@ __global__ void k_axpy(Ty* __restrict__ y, const Tx* __restrict__ x, int64_t n, 
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         y[q] = (Ty)((double)y[q] + alpha * (double)x[q]);
 }
+// MajorMinor interior rows, one work-item per interior node (boundary rows: k_boundary)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_mm_interior(Geom g, const T* __restrict__ C, const T* __restrict__ u,
+                                                     const T* __restrict__ b, T* __restrict__ out, T omega) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t I0 = g.N[0] - 2, I1 = g.N[1] - 2;
+    if (q >= I0 * I1 * (g.N[2] - 2)) return;
+    const int i = 1 + (int)(q % I0), j = 1 + (int)((q / I0) % I1), k = 1 + (int)(q / (I0 * I1));
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+    T f[3], D[3];
+    apply_interior_mm<T>(g, C, u, m, f, MODE == JACOBI ? D : nullptr);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const int64_t qq = p * g.tot + m;
+        if (MODE == APPLY) out[qq] = f[p];
+        else if (MODE == RESID) out[qq] = b[qq] - f[p];
+        else {
+            const T x = u[qq];
+            out[qq] = (T(1) - omega) * x + omega * (b[qq] - (f[p] - D[p] * x)) / D[p];
+        }
+    }
+}
+
 // x /= D (Operator<Grid::Node>::normalize)
 template <typename T>
 __global__ void k_divide(T* __restrict__ x, const T* __restrict__ D, int64_t n) {

@@ -170,4 +170,64 @@ __device__ __forceinline__ void apply_interior(const Geom& g, const T* __restric
     }
 }
 
+// MajorMinor interior row (Elastic.cpp:204-254 with Set/Matrix4_MajorMinor.H): the same sums as
+// apply_interior, reordered slot-major.  The 21 coefficient slots are streamed one at a time, each
+// with its six neighbours for the gradient term; every (i, j, k, l) that the slot stores adds
+// C u_k,lj and (d_j C) u_k,l to f_i (index matches resolve at compile time), so only the
+// derivatives of u stay in registers.  D (optional) receives the Jacobi diagonal.
+template <typename T>
+__device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __restrict__ C, const T* __restrict__ u,
+                                                  int64_t m, T f[3], T* D) {
+    const int64_t st[3] = {1, g.sy, g.sz};
+    const T dx[3] = {T(g.dx[0]), T(g.dx[1]), T(g.dx[2])};
+    const T w[3] = {T(-2.0 / (g.dx[0] * g.dx[0])), T(-2.0 / (g.dx[1] * g.dx[1])), T(-2.0 / (g.dx[2] * g.dx[2]))};
+    T gg[3][3][3], G[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* up = u + p * g.tot;
+        const T u0 = up[m];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T up1 = up[m + st[a]], um1 = up[m - st[a]];
+            gg[p][a][a] = (up1 - T(2) * u0 + um1) / dx[a] / dx[a];
+            G[p][a] = (up1 - um1) * T(0.5) / dx[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = a + 1; b < 3; b++) {
+                const T v = T(0.25) * (up[m + st[a] + st[b]] + up[m - st[a] - st[b]] - up[m + st[a] - st[b]] -
+                                       up[m - st[a] + st[b]]) / (dx[a] * dx[b]);
+                gg[p][a][b] = v;
+                gg[p][b][a] = v;
+            }
+    }
+    f[0] = f[1] = f[2] = T(0);
+    if (D) D[0] = D[1] = D[2] = T(0);
+#pragma unroll
+    for (int q = 0; q < 21; q++) {
+        const T* Cq = C + (int64_t)q * g.tot;
+        const T c = Cq[m];
+        T dc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) dc[d] = (Cq[m + st[d]] - Cq[m - st[d]]) * T(0.5) / dx[d];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int l = 0; l < 3; l++)
+                        if (mm_slot(i, j, k, l) == q) f[i] += c * gg[k][l][j] + dc[j] * G[k][l];
+        if (D) {
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++)
+                    if (mm_slot(p, jj, p, jj) == q) D[p] += c * w[jj];
+        }
+    }
+}
+
 }  // namespace alamo
