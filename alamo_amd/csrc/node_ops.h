@@ -8,6 +8,7 @@
 // psi is unset (psi_avg = 1) and m_uniform is false, the configuration
 // Integrator/Base/Mechanics.H:193-198 builds.
 #pragma once
+#include <type_traits>
 #include "common.h"
 
 namespace alamo {
@@ -170,17 +171,55 @@ __device__ __forceinline__ void apply_interior(const Geom& g, const T* __restric
     }
 }
 
+// The 6 index pairs of MajorMinor storage (00, 01, 02, 11, 12, 22; common.h::pair_of order).
+__host__ __device__ constexpr int mm_pa(int P) { return P < 3 ? 0 : (P < 5 ? 1 : 2); }
+__host__ __device__ constexpr int mm_pb(int P) { return P == 0 ? 0 : P == 1 ? 1 : P == 2 ? 2 : P == 3 ? 1 : 2; }
+
+// Terms of slot (pair I, pair J), I <= J: C_ijkl = c for every ordered (i, j) of one pair and
+// ordered (k, l) of the other; each adds c u_k,lj + (d_j C) u_k,l to f_i (and c w_j to D_i on the
+// diagonal slots).  All indices are template constants.
+template <int I, int J, typename T>
+__device__ __forceinline__ void mm_slot_terms(T c, const T dc[3], const T gg[3][3][3], const T G[3][3], T f[3],
+                                              T* D, const T w[3]) {
+    constexpr int ia = mm_pa(I), ib = mm_pb(I), ka = mm_pa(J), kb = mm_pb(J);
+    constexpr int ni = ia == ib ? 1 : 2, nk = ka == kb ? 1 : 2;
+#pragma unroll
+    for (int r = 0; r < ni; r++) {
+        const int i = r == 0 ? ia : ib, j = r == 0 ? ib : ia;
+#pragma unroll
+        for (int t = 0; t < nk; t++) {
+            const int k = t == 0 ? ka : kb, l = t == 0 ? kb : ka;
+            f[i] += c * gg[k][l][j] + dc[j] * G[k][l];
+            if (I != J) f[k] += c * gg[i][j][l] + dc[l] * G[i][j];
+        }
+        if (I == J && D) D[i] += c * w[j];
+    }
+}
+
+// pairs (I, J) with I <= J in slot order (common.h::mm_slot)
+template <int I, int J, typename F>
+__device__ __forceinline__ void for_slots(F& f) {
+    if constexpr (I < 6) {
+        if constexpr (J < 6) {
+            f(std::integral_constant<int, I>{}, std::integral_constant<int, J>{});
+            for_slots<I, J + 1>(f);
+        } else {
+            for_slots<I + 1, I + 1>(f);
+        }
+    }
+}
+
 // MajorMinor interior row (Elastic.cpp:204-254 with Set/Matrix4_MajorMinor.H): the same sums as
 // apply_interior, reordered slot-major.  The 21 coefficient slots are streamed one at a time, each
-// with its six neighbours for the gradient term; every (i, j, k, l) that the slot stores adds
-// C u_k,lj and (d_j C) u_k,l to f_i (index matches resolve at compile time), so only the
-// derivatives of u stay in registers.  D (optional) receives the Jacobi diagonal.
+// with its six neighbours for the gradient term, so only the derivatives of u stay in registers.
+// D (optional) receives the Jacobi diagonal.
 template <typename T>
 __device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __restrict__ C, const T* __restrict__ u,
                                                   int64_t m, T f[3], T* D) {
     const int64_t st[3] = {1, g.sy, g.sz};
-    const T dx[3] = {T(g.dx[0]), T(g.dx[1]), T(g.dx[2])};
-    const T w[3] = {T(-2.0 / (g.dx[0] * g.dx[0])), T(-2.0 / (g.dx[1] * g.dx[1])), T(-2.0 / (g.dx[2] * g.dx[2]))};
+    const T ih[3] = {T(0.5 / g.dx[0]), T(0.5 / g.dx[1]), T(0.5 / g.dx[2])};  // central first difference
+    const T i2[3] = {T(1.0 / (g.dx[0] * g.dx[0])), T(1.0 / (g.dx[1] * g.dx[1])), T(1.0 / (g.dx[2] * g.dx[2]))};
+    const T w[3] = {T(-2) * i2[0], T(-2) * i2[1], T(-2) * i2[2]};
     T gg[3][3][3], G[3][3];
 #pragma unroll
     for (int p = 0; p < 3; p++) {
@@ -189,45 +228,32 @@ __device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __rest
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             const T up1 = up[m + st[a]], um1 = up[m - st[a]];
-            gg[p][a][a] = (up1 - T(2) * u0 + um1) / dx[a] / dx[a];
-            G[p][a] = (up1 - um1) * T(0.5) / dx[a];
+            gg[p][a][a] = (up1 - T(2) * u0 + um1) * i2[a];
+            G[p][a] = (up1 - um1) * ih[a];
         }
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int b = a + 1; b < 3; b++) {
-                const T v = T(0.25) * (up[m + st[a] + st[b]] + up[m - st[a] - st[b]] - up[m + st[a] - st[b]] -
-                                       up[m - st[a] + st[b]]) / (dx[a] * dx[b]);
+                const T v = (up[m + st[a] + st[b]] + up[m - st[a] - st[b]] - up[m + st[a] - st[b]] -
+                             up[m - st[a] + st[b]]) * (ih[a] * ih[b]);
                 gg[p][a][b] = v;
                 gg[p][b][a] = v;
             }
     }
     f[0] = f[1] = f[2] = T(0);
     if (D) D[0] = D[1] = D[2] = T(0);
-#pragma unroll
-    for (int q = 0; q < 21; q++) {
+    auto slot = [&](auto ti, auto tj) {
+        constexpr int I = decltype(ti)::value, J = decltype(tj)::value;
+        constexpr int q = mm_slot(mm_pa(I), mm_pb(I), mm_pa(J), mm_pb(J));
         const T* Cq = C + (int64_t)q * g.tot;
         const T c = Cq[m];
         T dc[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) dc[d] = (Cq[m + st[d]] - Cq[m - st[d]]) * T(0.5) / dx[d];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-#pragma unroll
-                    for (int l = 0; l < 3; l++)
-                        if (mm_slot(i, j, k, l) == q) f[i] += c * gg[k][l][j] + dc[j] * G[k][l];
-        if (D) {
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int jj = 0; jj < 3; jj++)
-                    if (mm_slot(p, jj, p, jj) == q) D[p] += c * w[jj];
-        }
-    }
+        for (int d = 0; d < 3; d++) dc[d] = (Cq[m + st[d]] - Cq[m - st[d]]) * ih[d];
+        mm_slot_terms<I, J, T>(c, dc, gg, G, f, D, w);
+    };
+    for_slots<0, 0>(slot);
 }
 
 }  // namespace alamo

@@ -5,7 +5,7 @@ of the GPU solve against an FP64 reference solve.
   FP64 and mixed (FP32 cycles in FP64 refinement) against the oracle's FP64 solve.
 * 64-grain Voronoi polycrystal, rotated cubic grains (C11 1.68, C12 1.21, C44 0.75), u_z on top,
   bottom fixed, Neumann lateral faces: mixed V-cycle solve at 64^3 against the oracle; at 128^3
-  the oracle evaluates the GPU solution's true residual.
+  and at the configuration's 256^3 the oracle evaluates the GPU solution's true residual.
 * Two-phase 1:1000 field at 512^3 cells (513^3 nodes, correlation 16 cells, seed 5): mixed FGMRES
   against FP64 FGMRES on the GPU (full size: the oracle would take hours), both to 1e-10.
 Tolerance: relative L2 error 1e-8 (the target after refinement); true residuals 1e-9 of |b|
@@ -72,8 +72,8 @@ def test_config_polycrystal_64_vs_oracle(cuda, ora):
     assert st["converged"] == 1 and err <= 1e-8
 
 
-def test_config_polycrystal_128_true_residual(cuda, ora):
-    n = 128
+@pytest.mark.parametrize("n", [128, 256])
+def test_config_polycrystal_true_residual(cuda, ora, n):
     pr, N, b = _polycrystal(ora, n)
     C = pr["table"][pr["gid"]].ravel()
     rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.MAJORMINOR, C, np.zeros(9 * N), pr["bct"], np.zeros(3 * N), b)
@@ -81,7 +81,7 @@ def test_config_polycrystal_128_true_residual(cuda, ora):
     for prec in (0, PREC_MIXED):
         op = _poly_op(pr, prec)
         sol = to_dev(np.zeros(3 * N), cuda)
-        print("polycrystal 128^3 FGMRES precision", prec, flush=True)
+        print("polycrystal %d^3 FGMRES precision" % n, prec, flush=True)
         st = Krylov(op, restart=16, max_iter=400).solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
         print("iterations", st["iterations"], "seconds", st["seconds"])
         assert st["converged"] == 1
