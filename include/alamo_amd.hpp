@@ -141,7 +141,71 @@ private:
     ::Operator::Elastic<SYM>* linop_ = nullptr;
     alamo_solve_stats last_{};
 };
+
+// Outer flexible GMRES around the operator's cycle (no reference counterpart; same Define /
+// solve / Clear sequence as Linear, so a caller swaps one class name).  With Learned the
+// operator needs an attached network (alamo::Net::Attach).
+template <int SYM>
+class Krylov {
+public:
+    enum Precond { None = ALAMO_PRECOND_NONE, Cycle = ALAMO_PRECOND_MG, Learned = ALAMO_PRECOND_LEARNED };
+    Krylov() { alamo::check(alamo_krylov_opts_default(&opts_)); }
+    explicit Krylov(::Operator::Elastic<SYM>& op) : Krylov() { Define(op); }
+    void Define(::Operator::Elastic<SYM>& op) {
+        if (linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver cannot be re-defined");
+        linop_ = &op;
+    }
+    void Clear() {
+        if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver cannot be cleared if not defined");
+        linop_ = nullptr;
+    }
+    void setRestart(int m) { opts_.restart = m; }
+    void setMaxIter(int n) { opts_.max_iter = n; }
+    void setPrecond(Precond p) { opts_.precond = p; }
+    void setStagnationRatio(double s) { opts_.stagnation_ratio = s; }
+    double solve(double* sol, const double* rhs, double tol_rel, double tol_abs) {
+        if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver not defined");
+        alamo_solve_stats st{};
+        alamo::check(alamo_krylov_solve(linop_->handle(), sol, rhs, tol_rel, tol_abs, &opts_, &st));
+        last_ = st;
+        return st.final_res_inf;
+    }
+    const alamo_solve_stats& stats() const { return last_; }
+    alamo_krylov_opts& opts() { return opts_; }
+
+private:
+    ::Operator::Elastic<SYM>* linop_ = nullptr;
+    alamo_krylov_opts opts_{};
+    alamo_solve_stats last_{};
+};
 }  // namespace Nonlocal
 }  // namespace Solver
+
+namespace alamo {
+// The learned correction network (no reference counterpart): create, train on an operator's
+// own problem, save / load, attach to an operator for Krylov<SYM>::Learned.
+class Net {
+public:
+    explicit Net(uint64_t seed = 0) { check(alamo_net_create(&n_, seed)); }
+    ~Net() { if (n_) alamo_net_destroy(n_); }
+    Net(const Net&) = delete;
+    Net& operator=(const Net&) = delete;
+    template <int SYM>
+    alamo_train_stats Train(::Operator::Elastic<SYM>& op, const alamo_train_opts& o) {
+        alamo_train_stats st{};
+        check(alamo_net_train(n_, op.handle(), &o, &st));
+        return st;
+    }
+    template <int SYM>
+    void Attach(::Operator::Elastic<SYM>& op) { check(alamo_mg_set_net(op.handle(), n_)); }
+    void Save(const std::string& path) { check(alamo_net_save(n_, path.c_str())); }
+    void Load(const std::string& path) { check(alamo_net_load(n_, path.c_str())); }
+    int64_t NumParams() const { int64_t p = 0; check(alamo_net_num_params(n_, &p)); return p; }
+    alamo_net* handle() { return n_; }
+
+private:
+    alamo_net* n_ = nullptr;
+};
+}  // namespace alamo
 
 #endif

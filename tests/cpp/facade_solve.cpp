@@ -1,6 +1,8 @@
 // Drives the library through the C++ façade exactly as Integrator::Base::Mechanics drives the
 // reference (Base/Mechanics.H:193-208): build the operator, SetBC, SetModel, Define, solve, Clear.
-// Writes the solution (3 SoA planes, float64) to argv[1] and prints iterations and |r|inf.
+// Writes the solution (3 SoA planes, float64) to argv[1] and prints iterations and |r|inf; then
+// solves again from zero with the Krylov class (FGMRES around the same cycle) into argv[2], and
+// creates a network to check the Net class.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -9,7 +11,7 @@
 #include "alamo_amd.hpp"
 
 int main(int argc, char** argv) {
-    if (argc < 2) { fprintf(stderr, "usage: facade_solve out.bin\n"); return 2; }
+    if (argc < 3) { fprintf(stderr, "usage: facade_solve linear.bin krylov.bin\n"); return 2; }
     const int n = 16;
     const int N = (n + 1) * (n + 1) * (n + 1);
     BC::Operator::Elastic::Constant bc;  // Dirichlet 0 everywhere (Constant.H:36-45)
@@ -35,12 +37,27 @@ int main(int argc, char** argv) {
         double r = solver.solve(dx, db, 1e-12, 0.0);
         printf("iterations %d res_inf %.3e converged %d\n", solver.stats().iterations, r, solver.stats().converged);
         solver.Clear();
+        hipMemcpy(x.data(), dx, x.size() * 8, hipMemcpyDeviceToHost);
+        FILE* f = fopen(argv[1], "wb");
+        fwrite(x.data(), 8, x.size(), f);
+        fclose(f);
+
+        hipMemset(dx, 0, x.size() * 8);
+        Solver::Nonlocal::Krylov<Operator::Isotropic> krylov(op);
+        krylov.setRestart(16);
+        krylov.setPrecond(Solver::Nonlocal::Krylov<Operator::Isotropic>::Cycle);
+        r = krylov.solve(dx, db, 1e-12, 0.0);
+        printf("krylov iterations %d res_inf %.3e converged %d\n", krylov.stats().iterations, r,
+               krylov.stats().converged);
+        krylov.Clear();
+        alamo::Net net(7);
+        printf("net params %lld\n", (long long)net.NumParams());
     } catch (const alamo::Abort& e) {
         fprintf(stderr, "abort: %s\n", e.what());
         return 1;
     }
     hipMemcpy(x.data(), dx, x.size() * 8, hipMemcpyDeviceToHost);
-    FILE* f = fopen(argv[1], "wb");
+    FILE* f = fopen(argv[2], "wb");
     fwrite(x.data(), 8, x.size(), f);
     fclose(f);
     return 0;

@@ -1,5 +1,5 @@
 """The C++ façade (include/alamo_amd.hpp) compiles with the host compiler against the C ABI,
-and on a GPU drives a solve whose result matches the oracle."""
+and on a GPU drives a multigrid solve and a Krylov solve whose results match the oracle."""
 import os
 import subprocess
 
@@ -31,11 +31,12 @@ def test_facade_compiles():
 def test_facade_solve_matches_oracle(cuda, ora, tmp_path):
     if not os.path.exists(EXE):
         build_exe()
-    out = tmp_path / "x.bin"
-    res = subprocess.run([EXE, str(out)], capture_output=True, text=True, timeout=120)
+    out, outk = tmp_path / "x.bin", tmp_path / "xk.bin"
+    res = subprocess.run([EXE, str(out), str(outk)], capture_output=True, text=True, timeout=120)
     print(res.stdout, res.stderr)
-    assert res.returncode == 0 and "converged 1" in res.stdout
+    assert res.returncode == 0 and res.stdout.count("converged 1") == 2 and "net params" in res.stdout
     x = np.fromfile(out, dtype=np.float64)
+    xk = np.fromfile(outk, dtype=np.float64)
     n = 16
     N = (n + 1) ** 3
     i = np.arange(N) % (n + 1)
@@ -44,3 +45,4 @@ def test_facade_solve_matches_oracle(cuda, ora, tmp_path):
     b = ora.bc_init((n, n, n), bct, bcv)
     xref, it, *_ = ora.mg_solve((n, n, n), (1.0 / n,) * 3, P.ISO, C, bct, b, tol_rel=1e-12)
     assert np.linalg.norm(x - xref) / np.linalg.norm(xref) < 1e-9
+    assert np.linalg.norm(xk - xref) / np.linalg.norm(xref) < 1e-9
