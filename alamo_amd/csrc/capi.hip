@@ -98,7 +98,8 @@ struct alamo_mg {
     std::vector<void*> allocs;
 };
 
-static Geom make_geom(const int* n, const double* dx) {
+// align >= 2: rows of >= 4 align nodes get a pitch rounded up to a multiple of align nodes
+static Geom make_geom(const int* n, const double* dx, int align = 0) {
     Geom g{};
     for (int d = 0; d < 3; d++) {
         g.n[d] = n[d];
@@ -106,7 +107,8 @@ static Geom make_geom(const int* n, const double* dx) {
         g.dx[d] = dx[d];
     }
     g.sy = g.N[0];
-    g.sz = (int64_t)g.N[0] * g.N[1];
+    if (align >= 2 && g.N[0] >= 4 * align) g.sy = (g.N[0] + align - 1) / align * align;
+    g.sz = g.sy * g.N[1];
     g.tot = g.sz * g.N[2];
     return g;
 }
@@ -119,6 +121,9 @@ static int dalloc(alamo_mg* h, T** p, int64_t n) {
     void* q = nullptr;
     HIPCHK(hipMalloc(&q, std::max<int64_t>(1, n) * sizeof(T)));
     h->allocs.push_back(q);
+    // zero, so the pad entries of a pitched layout stay zero in every flat reduction
+    HIPCHK(hipMemset(q, 0, std::max<int64_t>(1, n) * sizeof(T)));
+    HIPCHK(hipDeviceSynchronize());  // complete before work on any stream uses the buffer
     *p = (T*)q;
     return ALAMO_OK;
 }
@@ -617,6 +622,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->bottom_max_iter = 200;
     d->bottom_tol_rel = 1e-4;
     d->verbose = 0;
+    d->row_align = 0;
     return ALAMO_OK;
 }
 
@@ -644,11 +650,12 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     int n[3] = {d->n_cell[0], d->n_cell[1], d->n_cell[2]};
     double dx[3] = {d->dx[0], d->dx[1], d->dx[2]};
     std::vector<Geom> gs;
-    gs.push_back(make_geom(n, dx));
+    if (d->row_align < 0 || d->row_align > 1024) FAIL(ALAMO_ERR_ARG, "row_align must be in 0..1024");
+    gs.push_back(make_geom(n, dx, d->row_align));
     while ((d->max_coarsening_level < 0 || (int)gs.size() <= d->max_coarsening_level) && n[0] % 2 == 0 &&
            n[1] % 2 == 0 && n[2] % 2 == 0 && n[0] / 2 >= 2 && n[1] / 2 >= 2 && n[2] / 2 >= 2) {
         for (int q = 0; q < 3; q++) { n[q] /= 2; dx[q] *= 2; }
-        gs.push_back(make_geom(n, dx));
+        gs.push_back(make_geom(n, dx, d->row_align));
     }
     const bool mixed = d->precision == ALAMO_PREC_MIXED;
     h->L.resize(gs.size());
@@ -726,6 +733,16 @@ int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3) {
     return ALAMO_OK;
 }
 
+int alamo_mg_level_layout(alamo_mg* h, int mglev, int64_t* pitch3) {
+    if (!h || !pitch3) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
+    const Geom& g = h->L[mglev].g;
+    pitch3[0] = g.sy;
+    pitch3[1] = g.sz;
+    pitch3[2] = g.tot;
+    return ALAMO_OK;
+}
+
 static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, bool host) {
     if (!h || !C) FAIL(ALAMO_ERR_ARG, "null argument");
     if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
@@ -734,12 +751,14 @@ static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, 
     const int64_t n = h->nc * l0.g.tot;
     const double* src = C;
     double* staged = nullptr;
+    const int64_t Vn = (int64_t)l0.g.N[0] * l0.g.N[1] * l0.g.N[2];
     if (host) {
-        HIPCHK(hipMalloc(&staged, n * sizeof(double)));
-        HIPCHK(hipMemcpy(staged, C, n * sizeof(double), hipMemcpyHostToDevice));
-        src = staged;
-    }
-    if (layout == ALAMO_LAYOUT_SOA) {
+        // host arrays are compact: stage them and scatter into the level's layout
+        HIPCHK(hipMalloc(&staged, h->nc * Vn * sizeof(double)));
+        HIPCHK(hipMemcpy(staged, C, h->nc * Vn * sizeof(double), hipMemcpyHostToDevice));
+        k_compact_to_soa<double><<<gs_blocks(Vn), 256, 0, h->stream>>>(l0.g, staged, l0.C, h->nc,
+                                                                       layout == ALAMO_LAYOUT_AOS);
+    } else if (layout == ALAMO_LAYOUT_SOA) {
         HIPCHK(hipMemcpyAsync(l0.C, src, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     } else {
         k_aos_to_soa<double><<<gs_blocks(l0.g.tot), 256, 0, h->stream>>>(src, l0.C, l0.g.tot, h->nc);
@@ -765,7 +784,8 @@ int alamo_mg_set_model_host(alamo_mg* h, int amrlev, const double* C, int layout
 int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const double* table, int ngrains) {
     if (!h || !gid || !table || ngrains <= 0) FAIL(ALAMO_ERR_ARG, "bad argument");
     if (h->sym != ALAMO_SYM_MAJORMINOR) FAIL(ALAMO_ERR_ARG, "grain tables need sym MAJORMINOR");
-    const int64_t tot = h->L[0].g.tot;
+    const Geom& g0 = h->L[0].g;
+    const int64_t tot = (int64_t)g0.N[0] * g0.N[1] * g0.N[2];  // compact host arrays
     std::vector<double> soa((size_t)21 * tot);
     for (int64_t m = 0; m < tot; m++) {
         const int gi = gid[m];
@@ -819,6 +839,7 @@ int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double
     LevelBuf& lb = h->L[mglev];
     double* tmp = nullptr;
     HIPCHK(hipMallocAsync((void**)&tmp, 3 * lb.g.tot * sizeof(double), (hipStream_t)stream));
+    if (lb.g.sy != lb.g.N[0]) vfill<double>(tmp, 3 * lb.g.tot, 0.0, (hipStream_t)stream);  // zero pads
     double* a = x;
     double* c = tmp;
     for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
@@ -932,6 +953,7 @@ __global__ void k_bc_init(Geom g, BCTable bc, BCVals vals, int homogeneous, doub
     if (m >= g.tot) return;
     int i, j, k;
     node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
     const int hi[3] = {g.n[0], g.n[1], g.n[2]};
     const int f = classify(i, j, k, hi);
     if (f == INT) return;
@@ -950,6 +972,7 @@ __global__ void k_newton_dw(Geom g, const double* __restrict__ C, const double* 
     if (m >= g.tot) return;
     int i, j, k;
     node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
     const int hi[3] = {g.n[0], g.n[1], g.n[2]};
     const int idx[3] = {i, j, k};
     const int64_t st[3] = {1, g.sy, g.sz};
@@ -972,6 +995,7 @@ __global__ void k_newton_rhs(Geom g, BCTable bc, const double* __restrict__ DW, 
     if (m >= g.tot) return;
     int i, j, k;
     node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
     const int hi[3] = {g.n[0], g.n[1], g.n[2]};
     const int idx[3] = {i, j, k};
     const int64_t st[3] = {1, g.sy, g.sz};
@@ -1029,10 +1053,11 @@ extern "C" int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, c
 // Learned correction network: C ABI
 // ===========================================================================
 namespace {
-__global__ void k_pack_target(int64_t V, const double* __restrict__ e, double inv_s, float* __restrict__ t) {
+__global__ void k_pack_target(Geom g, int64_t V, const double* __restrict__ e, double inv_s, float* __restrict__ t) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    for (int c = 0; c < 3; c++) t[v * 3 + c] = (float)(e[c * V + v] * inv_s);
+    const int64_t m = node_offset(g, v);
+    for (int c = 0; c < 3; c++) t[v * 3 + c] = (float)(e[c * g.tot + m] * inv_s);
 }
 __global__ void k_f32_to_bf16_host_layout(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
@@ -1197,7 +1222,8 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     TRY(net_level0(n, h));
     n->train_seed = o->seed;
     LevelBuf& l0 = h->L[0];
-    const int64_t V = l0.g.tot, n3 = 3 * V;
+    // V network voxels (compact); n3 elements of a level-0 vector (pitched rows, zero pads)
+    const int64_t V = (int64_t)l0.g.N[0] * l0.g.N[1] * l0.g.N[2], n3 = 3 * l0.g.tot;
     const int nlev = (int)h->L.size();
     std::vector<void*> tmp;
     auto talloc = [&](void** p, size_t b) -> int { return net_alloc(tmp, p, b); };
@@ -1248,10 +1274,11 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
         double el2 = 0;
         rc = norms_dev<double>(h, e, n3, nullptr, &el2);
         if (rc) break;
-        const double s = std::max(ql2 / std::sqrt((double)n3), 1e-300);
-        const double se = std::max(el2 / std::sqrt((double)n3), 1e-300);
-        net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(V, q, 1.0 / s, n->B.feat, X + (int64_t)p * V * 16);
-        k_pack_target<<<nb256(V), 256, 0, h->stream>>>(V, e, 1.0 / se, T + (int64_t)p * V * 3);
+        const double s = std::max(ql2 / std::sqrt(3.0 * (double)V), 1e-300);
+        const double se = std::max(el2 / std::sqrt(3.0 * (double)V), 1e-300);
+        net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, q, 1.0 / s, n->B.feat,
+                                                           X + (int64_t)p * V * 16);
+        k_pack_target<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, e, 1.0 / se, T + (int64_t)p * V * 3);
         if (o->verbose) fprintf(stderr, "alamo train: pair %d generated (solve %d its)\n", p, ss.iterations);
     }
     if (rc) { cleanup(); return rc; }

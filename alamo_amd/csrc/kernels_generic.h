@@ -8,11 +8,21 @@ namespace alamo {
 
 enum Mode { APPLY = 0, RESID = 1, JACOBI = 2, DIAG = 3 };
 
+// (i, j, k) of element m of a level's array; i >= N[0] marks a pad entry of a pitched row
 __device__ __forceinline__ void node_ijk(const Geom& g, int64_t m, int& i, int& j, int& k) {
     k = (int)(m / g.sz);
     int64_t r = m - (int64_t)k * g.sz;
     j = (int)(r / g.sy);
     i = (int)(r - (int64_t)j * g.sy);
+}
+// element offset of compact node index v (i fastest over the N[0] x N[1] x N[2] nodes)
+__device__ __forceinline__ int64_t node_offset(const Geom& g, int64_t v) {
+    const int64_t N01 = (int64_t)g.N[0] * g.N[1];
+    const int k = (int)(v / N01);
+    const int64_t r = v - (int64_t)k * N01;
+    const int j = (int)(r / g.N[0]);
+    const int i = (int)(r - (int64_t)j * g.N[0]);
+    return i + g.sy * j + g.sz * k;
 }
 
 // Row result of one node: Au (MODE APPLY/RESID/JACOBI) and/or the diagonal.
@@ -41,6 +51,7 @@ __global__ void __launch_bounds__(256) k_node(Geom g, BCTable bc, const T* __res
     if (m >= g.tot) return;
     int i, j, k;
     node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
     T f[3] = {T(0), T(0), T(0)}, D[3] = {T(1), T(1), T(1)};
     node_row<T, SYM, MODE == JACOBI || MODE == DIAG>(g, bc, C, MODE == DIAG ? nullptr : u, i, j, k, f, D);
 #pragma unroll
@@ -164,6 +175,7 @@ __global__ void __launch_bounds__(256) k_restrict(Geom gc, Geom gf, int np, cons
     if (mc >= gc.tot) return;
     int I, J, K;
     node_ijk(gc, mc, I, J, K);
+    if (I >= gc.N[0]) return;
     const bool bx = I == 0 || I == gc.n[0], by = J == 0 || J == gc.n[1], bz = K == 0 || K == gc.n[2];
     const int64_t mf = 2 * I + gf.sy * (2 * J) + gf.sz * (2 * K);
     const int64_t sx = 1, sy = gf.sy, sz = gf.sz;
@@ -211,6 +223,7 @@ __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* _
     if (m >= gf.tot) return;
     int i, j, k;
     node_ijk(gf, m, i, j, k);
+    if (i >= gf.N[0]) return;
     const int I = i / 2, J = j / 2, K = k / 2;
     const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
     const int64_t mc = I + gc.sy * J + gc.sz * K;
@@ -280,13 +293,22 @@ __global__ void __launch_bounds__(256) k_mm_interior(Geom g, const T* __restrict
 template <typename T>
 __global__ void k_divide(T* __restrict__ x, const T* __restrict__ D, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        x[q] /= D[q];
+        if (D[q] != T(0)) x[q] /= D[q];  // pad entries of a pitched layout hold D = 0
 }
 // y = a + b
 template <typename T>
 __global__ void k_add(T* __restrict__ y, const T* __restrict__ a, const T* __restrict__ b, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         y[q] = a[q] + b[q];
+}
+// compact nc-coefficient records (AoS [v][nc] or SoA [nc][v]) -> SoA planes of the level's layout
+template <typename T>
+__global__ void k_compact_to_soa(Geom g, const double* __restrict__ src, T* __restrict__ soa, int nc, int aos) {
+    const int64_t Vn = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < Vn; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = node_offset(g, v);
+        for (int q = 0; q < nc; q++) soa[q * g.tot + m] = (T)(aos ? src[v * nc + q] : src[q * Vn + v]);
+    }
 }
 // AoS (ncoef per node) -> SoA planes
 template <typename T>
@@ -309,6 +331,7 @@ __device__ void bottom_apply(const Geom& g, const BCTable& bc, const T* C, const
     for (int64_t m = threadIdx.x; m < g.tot; m += blockDim.x) {
         int i, j, k;
         node_ijk(g, m, i, j, k);
+        if (i >= g.N[0]) continue;
         T f[3], D[3];
         node_row<T, SYM, false>(g, bc, C, x, i, j, k, f, D);
         for (int p = 0; p < 3; p++) y[p * g.tot + m] = f[p];

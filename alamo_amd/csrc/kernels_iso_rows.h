@@ -17,6 +17,8 @@
 // When both x faces are Dirichlet in every component, their face rows (f = u) are written
 // here too, so the boundary kernel only visits the y and z faces.
 #pragma once
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "kernels_iso.h"
@@ -225,6 +227,207 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, cons
                 if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
                 else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
                 else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
+            }
+        }
+    };
+    RowPlane<T> qa, qb, qc, qd;  // plane p lives in record (p - k0 + 1) % 4
+    fetch(k0 - 1, qa);
+    for (int p = k0 - 1; p <= k1; p += 4) {
+        step(p, qc, qd, qa, qb);
+        if (p + 1 <= k1) step(p + 1, qd, qa, qb, qc);
+        if (p + 2 <= k1) step(p + 2, qa, qb, qc, qd);
+        if (p + 3 <= k1) step(p + 3, qb, qc, qd, qa);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Aligned-tile variant: a wavefront row is the 64 output nodes x0 .. x0 + 63 with x0 = 64 bx,
+// so when the row pitch is a multiple of 128 bytes every row load is whole 128-byte lines
+// (62-output tiles start at arbitrary offsets and touch one line more per row, which costs
+// about a fifth of the time of the access pattern).  The x neighbours of lanes 0 and 63
+// (x0 - 1, x0 + 64) are loaded by those two lanes alone and staged at entries 0 and 65 of a
+// 66-wide LDS row; the other lanes take their x neighbours by DPP as in k_iso_rows.  The edge
+// values of all staged rows (RW + 2 rows x 2 edges x 5 arrays) are fetched by one gather of
+// the first wavefront per plane, so the halo costs one load instruction per workgroup and
+// plane.  With both x faces Dirichlet, a face node x = n0 just past the last tile is written
+// by that tile's lane 63 from its halo value.
+// ---------------------------------------------------------------------------
+constexpr int R64_W = 66;  // LDS row: halo, 64 nodes, halo
+static_assert((ROWS_RW + 2) * 2 * 5 <= 64, "one gather covers the edge values of every staged row");
+
+template <typename T, int MODE, int FOLD>
+__global__ void ALAMO_ROWS_ATTR k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u,
+                                             const T* __restrict__ b, T* __restrict__ out, T omega) {
+    constexpr int NR = ROWS_RW + 2;  // staged rows: halo, RW rows, halo
+    constexpr int A = NR * R64_W;
+    __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 66 + 1 + lane]
+    const int lane = threadIdx.x, w = threadIdx.y;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (ALAMO_ROWS_XCD) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
+    const int x0 = 64 * bx;  // lane l <-> node x0 + l
+    const int y0 = 1 + ROWS_RW * by;
+    const int k0 = 1 + ROWS_KCH * bz;
+    const int k1 = min(k0 + ROWS_KCH, g.n[2]);  // output planes k0 .. k1 - 1; loads k0 - 1 .. k1
+    const int x = x0 + lane;
+    const int xl = min(x, g.n[0]);
+    const int y = y0 + w;
+    const bool row_out = y <= g.n[1] - 1;
+    const bool out_i = row_out && x >= 1 && x <= g.n[0] - 1;
+    const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
+    const bool out_fh = FOLD >= 1 && row_out && lane == 63 && x0 + 64 == g.n[0];
+    const int ee = lane == 0 ? 0 : R64_W - 1;  // LDS entry of an edge lane's halo node
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t szb = (uint32_t)g.sz * E;
+    const uint32_t colb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
+    const uint32_t ecolb = ((uint32_t)min(x0 + 64, g.n[0]) + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
+    const int hrow = w == 0 ? 0 : (w == ROWS_RW - 1 ? NR - 1 : -1);
+    const uint32_t hcolb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]))) * E;
+    // edge gather (first wavefront, lanes 0 .. 10 NR - 1): staged row gr, edge ge, array ga
+    const bool gl = w == 0 && lane < NR * 10;
+    const int gr = lane / 10, ge = (lane / 5) & 1, ga = lane % 5;
+    const int gy = gr == 0 ? y0 - 1 : min(y0 + gr - 1, g.n[1]);
+    const int gx = ge == 0 ? max(x0 - 1, 0) : min(x0 + 64, g.n[0]);
+    const T* __restrict__ gbase = ga < 3 ? u + (int64_t)ga * g.tot : C + (int64_t)(ga - 3) * g.tot;
+    const uint32_t gcolb = ((uint32_t)gx + (uint32_t)g.sy * (uint32_t)gy) * E;
+    const int gidx = gr * R64_W + (ge == 0 ? 0 : R64_W - 1);
+    const T* __restrict__ U[3] = {u, u + g.tot, u + 2 * g.tot};
+    const T* __restrict__ la = C;
+    const T* __restrict__ ma = C + g.tot;
+    const T* __restrict__ B[3] = {b, b + g.tot, b + 2 * g.tot};
+    T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
+    const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
+    const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
+    const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
+    const int rs = w * R64_W + 1 + lane, rc = (w + 1) * R64_W + 1 + lane, rn = (w + 2) * R64_W + 1 + lane;
+    const int rce = (w + 1) * R64_W + ee;
+
+    T ph[5] = {}, pg = T(0);  // prefetched halo row and edge gather of the next plane
+    auto fetch = [&](int p, RowPlane<T>& d) {
+        const uint32_t pz = szb * (uint32_t)p;
+        const uint32_t o = colb + pz;
+#pragma unroll
+        for (int v = 0; v < 3; v++) d.u[v] = ldb(U[v], o);
+        d.lam = ldb(la, o);
+        d.mu = ldb(ma, o);
+        if (gl) pg = ldb(gbase, gcolb + pz);
+        if (hrow >= 0) {
+            const uint32_t oh = hcolb + pz;
+#pragma unroll
+            for (int v = 0; v < 3; v++) ph[v] = ldb(U[v], oh);
+            ph[3] = ldb(la, oh);
+            ph[4] = ldb(ma, oh);
+        }
+    };
+    // x difference of a row value whose edge-lane neighbour is e (DPP in uniform flow)
+    auto dx_of = [&](T v, T e) {
+        const T r = from_right(v), l = from_left(v);
+        return ((lane == 63 ? e : r) - (lane == 0 ? e : l)) * hx;
+    };
+    auto step = [&](int p, const RowPlane<T>& qm, const RowPlane<T>& qc, RowPlane<T>& qn, RowPlane<T>& qx) {
+        const int sl = (p - k0 + 1) % 3;
+        T(*sp)[A] = lds[sl];
+#pragma unroll
+        for (int v = 0; v < 3; v++) sp[v][rc] = qn.u[v];
+        sp[3][rc] = qn.lam;
+        sp[4][rc] = qn.mu;
+        if (gl) sp[ga][gidx] = pg;
+        if (hrow >= 0) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) sp[v][hrow * R64_W + 1 + lane] = ph[v];
+        }
+        if (p < k1) fetch(p + 1, qx);
+        const uint32_t pc = szb * (uint32_t)(p - 1);
+        const uint32_t mcb = colb + pc;
+        T bq[3] = {T(0), T(0), T(0)}, bh[3] = {T(0), T(0), T(0)};
+        if (MODE != APPLY && p >= k0 + 1) {
+            if (out_i || out_f) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
+            }
+            if (out_fh) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) bh[v] = ldb(B[v], ecolb + pc);
+            }
+        }
+        __syncthreads();
+        qn.dxux = dx_of(qn.u[0], sp[0][rce]);
+        qn.dxuz = dx_of(qn.u[2], sp[2][rce]);
+        qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
+        qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
+        if (p < k0 + 1) return;
+        // output plane c = p - 1
+        const T(*sc)[A] = lds[sl == 0 ? 2 : sl - 1];
+        T ce[5];  // the centre plane's edge-halo values of this lane's row
+#pragma unroll
+        for (int v = 0; v < 5; v++) ce[v] = sc[v][rce];
+        T ue[3], uw[3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            const T r = from_right(qc.u[v]), l = from_left(qc.u[v]);
+            ue[v] = lane == 63 ? ce[v] : r;
+            uw[v] = lane == 0 ? ce[v] : l;
+        }
+        const T lx = dx_of(qc.lam, ce[3]);
+        const T mx = dx_of(qc.mu, ce[4]);
+        if (out_i) {
+            T ux[3], uy[3], uz[3], lap[3];
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                const T c0 = qc.u[v], e = ue[v], wv = uw[v];
+                const T nn = sc[v][rn], ss = sc[v][rs];
+                ux[v] = (e - wv) * hx;
+                uy[v] = (nn - ss) * hy;
+                uz[v] = (qn.u[v] - qm.u[v]) * hz;
+                lap[v] = (e - T(2) * c0 + wv) * ix2 + (nn - T(2) * c0 + ss) * iy2 + (qn.u[v] - T(2) * c0 + qm.u[v]) * iz2;
+            }
+            const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * ix2;
+            const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * iy2;
+            const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * iz2;
+            const T uxy0 = (sc[0][rn + 1] - sc[0][rn - 1] - sc[0][rs + 1] + sc[0][rs - 1]) * hxy;
+            const T uxy1 = (sc[1][rn + 1] - sc[1][rn - 1] - sc[1][rs + 1] + sc[1][rs - 1]) * hxy;
+            const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
+            const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
+            const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
+            const T lam = qc.lam, mu = qc.mu;
+            const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
+            const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
+            const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
+            const T tr = G[0][0] + G[1][1] + G[2][2];
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                T f = mu * (lap[v] + gd[v]) + lam * gd[v];
+                f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
+                     dm[2] * (G[v][2] + G[2][v]);
+                if (MODE == APPLY) stb(O[v], mcb, f);
+                else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
+                else {
+                    const T w2[3] = {ix2, iy2, iz2};
+                    const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
+                    stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
+                }
+            }
+        } else if (out_f) {
+            // Dirichlet face row f = u (diagonal 1)
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
+                else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
+                else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
+            }
+        }
+        if (out_fh) {
+            const uint32_t fcb = ecolb + pc;
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                if (MODE == APPLY) stb(O[v], fcb, ce[v]);
+                else if (MODE == RESID) stb(O[v], fcb, bh[v] - ce[v]);
+                else stb(O[v], fcb, (T(1) - omega) * ce[v] + omega * bh[v]);
             }
         }
     };
@@ -468,6 +671,11 @@ __global__ void __launch_bounds__(256) k_dirichlet_yz(Geom g, const T* __restric
 #ifndef ALAMO_ROWS_DMA
 #define ALAMO_ROWS_DMA 0
 #endif
+// 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons)
+static const bool g_rows_tile64 = [] {
+    const char* e = getenv("ALAMO_ROWS_TILE");
+    return !(e && strcmp(e, "62") == 0);
+}();
 template <typename T, int MODE, int FOLD>
 static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s) {
     if (ALAMO_ROWS_DMA) {
@@ -478,6 +686,12 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
         return;
     }
     dim3 blk(64, ROWS_RW);
+    if (g_rows_tile64) {
+        // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
+        dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW, (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
+        k_iso_rows64<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
+        return;
+    }
     dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,
               (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
     k_iso_rows<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "kernels_generic.h"
+
 namespace alamo {
 namespace net {
 
@@ -395,36 +397,43 @@ __global__ void k_prolong_T(Grid3 gf, Grid3 gc, int C, const float* __restrict__
     gcout[q] = s;
 }
 
+// Voxel v of the network (compact, V = N0 N1 N2 voxels) is node node_offset(g, v) of the
+// operator level g, whose rows may be pitched.
 // Network input: channels 0-2 = r / s (r FP64, 3 SoA planes), 3-4 = coefficient features,
 // 5-15 = 0.  feat holds 2 f32 per voxel.
-__global__ void k_pack_input(int64_t V, const double* __restrict__ r, double inv_s, const float* __restrict__ feat,
-                             uint16_t* __restrict__ x) {
+__global__ void k_pack_input(Geom g, int64_t V, const double* __restrict__ r, double inv_s,
+                             const float* __restrict__ feat, uint16_t* __restrict__ x) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
+    const int64_t m = node_offset(g, v);
     uint16_t* xv = x + v * 16;
-    xv[0] = f2bf((float)(r[v] * inv_s));
-    xv[1] = f2bf((float)(r[V + v] * inv_s));
-    xv[2] = f2bf((float)(r[2 * V + v] * inv_s));
+    xv[0] = f2bf((float)(r[m] * inv_s));
+    xv[1] = f2bf((float)(r[g.tot + m] * inv_s));
+    xv[2] = f2bf((float)(r[2 * g.tot + m] * inv_s));
     xv[3] = f2bf(feat[2 * v]);
     xv[4] = f2bf(feat[2 * v + 1]);
 #pragma unroll
     for (int c = 5; c < 16; c++) xv[c] = 0;
 }
 // z (FP64 SoA) = s * y[v][0..2]
-__global__ void k_unpack_output(int64_t V, const float* __restrict__ y, double s, double* __restrict__ z, int add) {
+__global__ void k_unpack_output(Geom g, int64_t V, const float* __restrict__ y, double s, double* __restrict__ z,
+                                int add) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
+    const int64_t m = node_offset(g, v);
     for (int c = 0; c < 3; c++) {
         const double val = s * (double)y[v * 16 + c];
-        z[c * V + v] = add ? z[c * V + v] + val : val;
+        z[c * g.tot + m] = add ? z[c * g.tot + m] + val : val;
     }
 }
 // feat = (log lambda, log mu) / fscale from the isotropic stiffness planes
-__global__ void k_coef_features(int64_t V, const double* __restrict__ C, double fscale, float* __restrict__ feat) {
+__global__ void k_coef_features(Geom g, int64_t V, const double* __restrict__ C, double fscale,
+                                float* __restrict__ feat) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    feat[2 * v] = (float)(log(fmax(C[v], 1e-300)) * fscale);
-    feat[2 * v + 1] = (float)(log(fmax(C[V + v], 1e-300)) * fscale);
+    const int64_t m = node_offset(g, v);
+    feat[2 * v] = (float)(log(fmax(C[m], 1e-300)) * fscale);
+    feat[2 * v + 1] = (float)(log(fmax(C[g.tot + m], 1e-300)) * fscale);
 }
 
 // Loss 0.5 * mean over voxels of |y - t|^2 (3 channels of y = [V][16], t = [V][3] f32) and its

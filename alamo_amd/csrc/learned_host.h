@@ -64,6 +64,8 @@ static const bool g_conv_lds = [] {
 static int net_alloc(std::vector<void*>& list, void** p, size_t bytes) {
     HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 16)));
     list.push_back(*p);
+    HIPCHK(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));  // pad entries of pitched vectors stay zero
+    HIPCHK(hipDeviceSynchronize());
     return ALAMO_OK;
 }
 #define NALLOC(list, ptr, bytes) TRY(net_alloc(list, (void**)&(ptr), (size_t)(bytes)))
@@ -346,9 +348,10 @@ __global__ void k_noise(int64_t n, uint64_t seed, double* __restrict__ x) {
 }
 
 static int net_features(alamo_net* n, alamo_mg* h) {
-    const int64_t V = h->L[0].g.tot;
+    const Geom& g = h->L[0].g;
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     if (h->sym != ISO) FAIL(ALAMO_ERR_ARG, "the learned preconditioner needs an isotropic operator");
-    net::k_coef_features<<<nb256(V), 256, 0, n->s>>>(V, h->L[0].C, 1.0 / 7.0, n->B.feat);
+    net::k_coef_features<<<nb256(V), 256, 0, n->s>>>(g, V, h->L[0].C, 1.0 / 7.0, n->B.feat);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -372,17 +375,18 @@ static int net_level0(alamo_net* n, alamo_mg* h) {
 // FP64 vectors: q, A y, y.
 static int net_correct(alamo_net* n, alamo_mg* h, const double* r, double* z, double* w) {
     LevelBuf& l0 = h->L[0];
-    const int64_t V = l0.g.tot, n3 = 3 * V;
+    // V network voxels (compact); n3 elements of a level-0 vector (pitched rows, zero pads)
+    const int64_t V = (int64_t)l0.g.N[0] * l0.g.N[1] * l0.g.N[2], n3 = 3 * l0.g.tot;
     double *q = w, *Ay = w + n3, *y = w + 2 * n3;
     TRY(net_level0(n, h));
     launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, z, r, q, 0.0, h->stream);
     double ql2 = 0;
     TRY(norms_dev<double>(h, q, n3, nullptr, &ql2));
-    const double s = ql2 / std::sqrt((double)n3);
+    const double s = ql2 / std::sqrt(3.0 * (double)V);
     if (!(s > 0.0) || !std::isfinite(s)) return ALAMO_OK;
-    net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(V, q, 1.0 / s, n->B.feat, n->B.x);
+    net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, q, 1.0 / s, n->B.feat, n->B.x);
     TRY(net_forward(n));
-    net::k_unpack_output<<<nb256(V), 256, 0, h->stream>>>(V, n->B.y, 1.0, y, 0);
+    net::k_unpack_output<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, n->B.y, 1.0, y, 0);
     launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, y, nullptr, Ay, 0.0, h->stream);
     double d[2];
     TRY(multidot(h, Ay, q, n3, 2, d));  // <Ay, q>, <Ay, Ay>

@@ -37,7 +37,7 @@ class Elastic:
 
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
-                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0):
+                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
         for q in range(3):
@@ -56,6 +56,7 @@ class Elastic:
         d.pre_smooth, d.post_smooth, d.sweeps_per_smooth = pre_smooth, post_smooth, sweeps_per_smooth
         d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
         d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
+        d.row_align = int(row_align)
         self.desc = d
         self.sym = sym
         self.ncell = tuple(int(n) for n in ncell)
@@ -87,6 +88,45 @@ class Elastic:
     def LevelNodes(self, mglev):
         n = self.LevelNCell(mglev)
         return (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
+
+    def LevelLayout(self, mglev=0):
+        """(row pitch, plane pitch, elements per component) of a device field on level mglev."""
+        a = (ctypes.c_int64 * 3)()
+        check(lib().alamo_mg_level_layout(self.h, mglev, a))
+        return tuple(int(v) for v in a)
+
+    def LevelSize(self, mglev=0):
+        """elements per component of a device field on level mglev (nodes plus row pads)"""
+        return self.LevelLayout(mglev)[2]
+
+    def to_layout(self, x, mglev=0, ncomp=3, aos=False):
+        """A compact field (``ncomp`` SoA planes, or AoS records of ``ncomp`` values, of the level's
+        nodes with i fastest; numpy or torch) in the level's device layout, pads zero.  Returns
+        ``x`` itself when the level is compact."""
+        N = [v + 1 for v in self.LevelNCell(mglev)]
+        sy = self.LevelLayout(mglev)[0]
+        if sy == N[0]:
+            return x
+        shape = (N[2], N[1], sy, ncomp) if aos else (ncomp, N[2], N[1], sy)
+        if isinstance(x, np.ndarray):
+            out = np.zeros(shape, dtype=x.dtype)
+        else:
+            import torch
+            out = torch.zeros(shape, dtype=x.dtype, device=x.device)
+        if aos:
+            out[:, :, :N[0], :] = x.reshape(N[2], N[1], N[0], ncomp)
+        else:
+            out[..., :N[0]] = x.reshape(ncomp, N[2], N[1], N[0])
+        return out.reshape(-1)
+
+    def from_layout(self, x, mglev=0, ncomp=3):
+        """The compact SoA field (``ncomp`` planes) of a field in the level's device layout."""
+        N = [v + 1 for v in self.LevelNCell(mglev)]
+        sy = self.LevelLayout(mglev)[0]
+        if sy == N[0]:
+            return x
+        y = x.reshape(ncomp, N[2], N[1], sy)[..., :N[0]]
+        return (np.ascontiguousarray(y) if isinstance(x, np.ndarray) else y.contiguous()).reshape(-1)
 
     # --- coefficients ----------------------------------------------------------
     def SetModel(self, C, layout=LAYOUT_AOS, amrlev=0):

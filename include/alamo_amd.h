@@ -59,6 +59,14 @@ typedef struct {
     int bottom_max_iter;       /* solver.bottom_max_iter */
     double bottom_tol_rel;     /* solver.bottom_tol_rel */
     int verbose;
+    int row_align;             /* 0 or 1: compact node layout on every level.  A >= 2: on each level
+                                  whose x rows hold >= 4 A nodes, the row pitch is rounded up to a
+                                  multiple of A nodes (32: every row starts on a 256-byte boundary),
+                                  which lets the row kernels load whole cache lines; every field
+                                  passed for the handle on such a level then uses that layout
+                                  (alamo_mg_level_layout), with zeros in the pad entries.  Host
+                                  arrays (the *_host entry points) are always compact.  No
+                                  reference counterpart (an amrex::FArrayBox is compact). */
 } alamo_mg_desc;
 
 typedef struct {
@@ -85,6 +93,9 @@ int alamo_mg_destroy(alamo_mg* h);
 int alamo_mg_set_stream(alamo_mg* h, void* stream);
 int alamo_mg_num_levels(alamo_mg* h, int* nlev);
 int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3);
+/* Node layout of level mglev: node (i,j,k) of component c is element i + pitch[0] j + pitch[1] k
+ * + pitch[2] c; pitch[2] elements per component (the size of one component's array). */
+int alamo_mg_level_layout(alamo_mg* h, int mglev, int64_t* pitch3);
 
 /* Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116): copy the per-node stiffness of
  * mglev 0.  `C` is a device pointer (alamo_mg_set_model_host: host pointer). */
