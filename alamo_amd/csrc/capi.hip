@@ -451,6 +451,27 @@ static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, in
     return ALAMO_OK;
 }
 
+// w -= sum_{i < nv} c_i V_i with c already uploaded (upload_coef); *nrm = |w|_2 afterwards
+static int multiaxpy_norm(alamo_mg* h, double* w, const double* V, int64_t n, int nv, double* nrm) {
+    const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
+    k_multiaxpy_nrm<<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kcoef, h->kpart);
+    k_sum_rows<<<1, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *nrm = std::sqrt(std::max(h->red_host[0], 0.0));
+    return ALAMO_OK;
+}
+
+// second Gram-Schmidt pass when |w'| < eta |w| after the first.  With the multigrid cycle as
+// preconditioner the first pass leaves |w'| / |w| = 0.3 .. 0.5 (A P v is close to v), so Kahan's
+// eta = 1/sqrt 2 re-orthogonalised every step without changing the iterations or the final
+// residual of any solve measured; eta = 0.1 keeps the second pass for severe cancellation only.
+// ALAMO_KRY_REORTH overrides eta (0: never, above 1: always).
+static const double g_reorth_eta = [] {
+    const char* e = getenv("ALAMO_KRY_REORTH");
+    return e ? atof(e) : 0.1;
+}();
+
 static int upload_coef(alamo_mg* h, const double* c, int nv) {
     TRY(krylov_scratch(h));
     for (int i = 0; i < nv; i++) h->red_host[i] = c[i];
@@ -485,7 +506,7 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
     const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
     double rinf = r0;
-    int it = 0, fallbacks = 0;
+    int it = 0, fallbacks = 0, reorths = 0;
     int kind = o->precond;
     double beta_prev = beta;
     double H[KRY_MAXV + 1][KRY_MAXV], cs[KRY_MAXV], sn[KRY_MAXV], gv[KRY_MAXV + 1], y[KRY_MAXV];
@@ -511,16 +532,22 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
                 return prc;
             }
             launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
-            // CGS2
-            TRY(multidot(h, w, V, n, j + 1, d1));
+            // classical Gram-Schmidt; w is vector j + 1 of the basis block, so the same multi-dot
+            // pass returns |w|^2 in d1[j + 1]
+            TRY(multidot(h, w, V, n, j + 2, d1));
+            const double wn = std::sqrt(std::max(d1[j + 1], 0.0));
             TRY(upload_coef(h, d1, j + 1));
-            k_multiaxpy<<<gs_blocks(n), 256, 0, h->stream>>>(w, V, n, n, j + 1, h->kcoef);
-            TRY(multidot(h, w, V, n, j + 1, d2));
-            TRY(upload_coef(h, d2, j + 1));
-            k_multiaxpy<<<gs_blocks(n), 256, 0, h->stream>>>(w, V, n, n, j + 1, h->kcoef);
             double hn = 0;
-            TRY(norms_dev<double>(h, w, n, nullptr, &hn));
-            for (int i = 0; i <= j; i++) H[i][j] = d1[i] + d2[i];
+            TRY(multiaxpy_norm(h, w, V, n, j + 1, &hn));
+            for (int i = 0; i <= j; i++) H[i][j] = d1[i];
+            if (hn < g_reorth_eta * wn) {
+                // the first pass cancelled most of w: re-orthogonalise (CGS2)
+                TRY(multidot(h, w, V, n, j + 1, d2));
+                TRY(upload_coef(h, d2, j + 1));
+                TRY(multiaxpy_norm(h, w, V, n, j + 1, &hn));
+                for (int i = 0; i <= j; i++) H[i][j] += d2[i];
+                reorths++;
+            }
             H[j + 1][j] = hn;
             if (hn > 0.0) k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / hn);
             for (int i = 0; i < j; i++) {
@@ -564,7 +591,9 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
         k_multiaxpy_add<<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
         TRY(norms_dev<double>(h, V, n, &rinf, &beta));
-        if (o->verbose) fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e\n", it, rinf, beta);
+        if (o->verbose)
+            fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e (%d second Gram-Schmidt passes)\n", it, rinf,
+                    beta, reorths);
         // the plain cycle judges the learned stage: a restart that does not cut |r|_2 by the
         // stagnation ratio switches to the plain cycle for good
         if (kind == ALAMO_PRECOND_LEARNED && !(beta <= o->stagnation_ratio * beta_prev)) {

@@ -3,10 +3,13 @@
 // Saad, "A flexible inner-outer preconditioned GMRES algorithm", SIAM J. Sci. Comput. 14 (1993).
 //
 // The operator is the nonsymmetric strong-form Operator::Elastic, so the outer Krylov method
-// is GMRES rather than CG.  Orthogonalisation is classical Gram-Schmidt with one
-// re-orthogonalisation pass (CGS2): each pass is one fused multi-dot kernel and one
-// multi-axpy kernel over the basis, so a step reads the basis twice instead of once per
-// vector as modified Gram-Schmidt would.
+// is GMRES rather than CG.  Orthogonalisation is classical Gram-Schmidt: one fused multi-dot
+// kernel (which also returns |w|^2, w being the next vector of the strided basis) and one
+// multi-axpy kernel that also returns |w'|^2, so a step reads the basis twice instead of once
+// per vector as modified Gram-Schmidt would.  A second pass (CGS2) runs when the first one
+// cancelled w by more than a set factor, |w'| < eta |w| (Kahan's criterion for eta = 1/sqrt 2;
+// Giraud, Langou, Rozloznik, "The loss of orthogonality in the Gram-Schmidt orthogonalization
+// process", Comput. Math. Appl. 50 (2005)).
 #pragma once
 #include "kernels_generic.h"
 
@@ -78,13 +81,26 @@ __global__ void __launch_bounds__(RED_BLOCK) k_sum_rows(const double* __restrict
     }
 }
 
-// w -= sum_{i < nv} c_i V_i
-__global__ void __launch_bounds__(256) k_multiaxpy(double* __restrict__ w, const double* __restrict__ V, int64_t n,
-                                                   int64_t ldv, int nv, const double* __restrict__ c) {
+// w -= sum_{i < nv} c_i V_i and partial[block] = the block's share of |w|^2 afterwards
+__global__ void __launch_bounds__(RED_BLOCK) k_multiaxpy_nrm(double* __restrict__ w, const double* __restrict__ V,
+                                                             int64_t n, int64_t ldv, int nv,
+                                                             const double* __restrict__ c,
+                                                             double* __restrict__ partial) {
+    double acc = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         double s = w[q];
         for (int i = 0; i < nv; i++) s -= c[i] * V[i * ldv + q];
         w[q] = s;
+        acc += s * s;
+    }
+    __shared__ double sh[RED_BLOCK / 64];
+    const double v = wave_sum(acc);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < RED_BLOCK / 64; q++) t += sh[q];
+        partial[blockIdx.x] = t;
     }
 }
 
