@@ -230,14 +230,49 @@ template <> struct LB<float> {
 };
 
 template <typename T>
-static void smooth(alamo_mg* h, int l, int ncalls) {
+static void sweeps(alamo_mg* h, int l, int count) {
     LevelBuf& lb = h->L[l];
-    for (int c = 0; c < ncalls; c++)
-        for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
-            launch_node<T>(h->sym, JACOBI, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::tmp(lb),
-                           T(h->d.omega), h->stream);
-            std::swap(LB<T>::cor(lb), LB<T>::tmp(lb));
-        }
+    for (int s = 0; s < count; s++) {
+        launch_node<T>(h->sym, JACOBI, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::tmp(lb),
+                       T(h->d.omega), h->stream);
+        std::swap(LB<T>::cor(lb), LB<T>::tmp(lb));
+    }
+}
+
+template <typename T>
+static void smooth(alamo_mg* h, int l, int ncalls) {
+    sweeps<T>(h, l, ncalls * h->d.sweeps_per_smooth);
+}
+
+// First damped-Jacobi sweep from a zero correction: A 0 = 0, so x = omega b / D pointwise (the
+// prepared diagonal; pads of a pitched layout hold D = 0 and stay 0).  b_out, when given,
+// receives b in the cycle precision, so the same pass stages the cycle's right-hand side.
+template <typename Ti, typename T>
+__global__ void k_jacobi_from_zero(const Ti* __restrict__ b_in, T* __restrict__ b_out, T* __restrict__ x,
+                                   const T* __restrict__ D, int64_t n, T omega) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const T bq = (T)b_in[q];
+        if (b_out) b_out[q] = bq;
+        const T d = D[q];
+        x[q] = d != T(0) ? omega * bq / d : T(0);
+    }
+}
+
+// pre-smoothing of a V-cycle level from a zero correction; first_done: the first sweep has
+// already been applied (by the caller's staging pass)
+template <typename T>
+static void presmooth_from_zero(alamo_mg* h, int l, bool first_done) {
+    LevelBuf& lb = h->L[l];
+    const int total = h->d.pre_smooth * h->d.sweeps_per_smooth;
+    const int64_t n = 3 * lb.g.tot;
+    if (total == 0) {
+        vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
+        return;
+    }
+    if (!first_done)
+        k_jacobi_from_zero<T, T><<<gs_blocks(n), 256, 0, h->stream>>>(LB<T>::res(lb), nullptr, LB<T>::cor(lb),
+                                                                      LB<T>::D(lb), n, T(h->d.omega));
+    sweeps<T>(h, l, total - 1);
 }
 
 template <typename T>
@@ -256,12 +291,11 @@ static void bottom(alamo_mg* h) {
 }
 
 template <typename T>
-static void vcycle(alamo_mg* h, int top) {
+static void vcycle(alamo_mg* h, int top, bool first_done = false) {
     const int bot = (int)h->L.size() - 1;
     for (int l = top; l < bot; l++) {
         LevelBuf& lb = h->L[l];
-        vfill<T>(LB<T>::cor(lb), 3 * lb.g.tot, T(0), h->stream);
-        smooth<T>(h, l, h->d.pre_smooth);
+        presmooth_from_zero<T>(h, l, first_done && l == top);
         launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
                        T(0), h->stream);
         LevelBuf& lc = h->L[l + 1];
@@ -417,13 +451,26 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     if (kind == ALAMO_PRECOND_NONE) {
         HIPCHK(hipMemcpyAsync(z, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     } else if (h->d.precision == ALAMO_PREC_MIXED) {
-        k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, h->L[0].resf, n);
-        one_cycle<float>(h);
-        k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(h->L[0].corf, z, n);
+        // a V-cycle with pre-smoothing stages its FP32 right-hand side in its first sweep's pass
+        if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
+            k_jacobi_from_zero<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, l0.corf, l0.Df, n,
+                                                                                  (float)h->d.omega);
+            vcycle<float>(h, 0, true);
+        } else {
+            k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, n);
+            one_cycle<float>(h);
+        }
+        k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(l0.corf, z, n);
     } else {
-        HIPCHK(hipMemcpyAsync(h->L[0].res, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-        one_cycle<double>(h);
-        HIPCHK(hipMemcpyAsync(z, h->L[0].cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
+            k_jacobi_from_zero<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.res, l0.cor, l0.D, n,
+                                                                                    h->d.omega);
+            vcycle<double>(h, 0, true);
+        } else {
+            HIPCHK(hipMemcpyAsync(l0.res, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            one_cycle<double>(h);
+        }
+        HIPCHK(hipMemcpyAsync(z, l0.cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
     return ALAMO_OK;
 }

@@ -32,11 +32,16 @@ def parse():
     p.add_argument("--contrast", type=float, default=1000.0)
     p.add_argument("--seed", type=int, default=256)
     p.add_argument("--fapply-reps", type=int, default=50)
+    p.add_argument("--restart", type=int, default=10, help="FGMRES restart length")
+    p.add_argument("--krylov-smooth", type=int, default=1,
+                   help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
+    p.add_argument("--krylov-sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
     p.add_argument("--skip-cpu", action="store_true")
     p.add_argument("--methods", default="fp64v,mixedv,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
                         "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps); "
-                        "fgmres-*: flexible GMRES(16) preconditioned by one such cycle; "
+                        "fgmres-*: flexible GMRES(--restart) preconditioned by one V-cycle with --krylov-smooth "
+                        "Fsmooth calls of --krylov-sweeps Jacobi sweeps each side; "
                         "fgmres-learned-*: the same with the learned correction stage after each cycle")
     p.add_argument("--train-n", type=int, default=128, help="cells per direction of the training fields")
     p.add_argument("--train-fields", type=int, default=2)
@@ -119,8 +124,10 @@ def main():
                                                 CYCLE_V if base.endswith("v") else CYCLE_F)
         prec = PREC_MIXED if base.startswith("mixed") else PREC_FP64
         vcyc = base.endswith("v")
+        nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
-                     pre_smooth=1 if vcyc else 2, post_smooth=1 if vcyc else 2, verbose=a.verbose)
+                     pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
+                     verbose=a.verbose)
         op.SetModel(pr["C"])
         op.prepareForSolve()
         if rhs is None:
@@ -132,7 +139,7 @@ def main():
         sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
         if learned:
             net.attach(op)
-        lin = (Krylov(op, restart=16, precond=PRECOND_LEARNED if learned else PRECOND_MG, verbose=a.verbose)
+        lin = (Krylov(op, restart=a.restart, precond=PRECOND_LEARNED if learned else PRECOND_MG, verbose=a.verbose)
                if krylov else Linear(op))
         ok = True
         for _ in range(a.warmup):
@@ -208,6 +215,8 @@ def main():
                         "nodes, Dirichlet box, eigenstrain load; solve to |b-Au|inf/|b|inf <= 1e-10"
                         % (a.corr, a.contrast, a.seed, n, n + 1),
             "method": best,
+            "fgmres": {"restart": a.restart, "vcycle_smooth_calls": a.krylov_smooth,
+                       "jacobi_sweeps_per_call": a.krylov_sweeps},
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
@@ -255,8 +264,9 @@ def train_network(a, P, Elastic, Net, precision, cycle):
         pr = P.two_phase_problem(n=a.train_n, corr_cells=a.corr * a.train_n / a.n, contrast=a.contrast,
                                  seed=a.seed + 1 + f)
         vc = cycle == 0
+        nsm = a.krylov_smooth if vc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
-                     pre_smooth=1 if vc else 2, post_smooth=1 if vc else 2)
+                     pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps)
         op.SetModel(pr["C"])
         stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
                                solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))
