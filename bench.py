@@ -144,12 +144,15 @@ def main():
         ok = True
         for _ in range(a.warmup):
             sol.zero_()
+            op.prepareForSolve()
             ok = lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False)["converged"] == 1
         torch.cuda.synchronize()
         stats = []
         ts = time.perf_counter()
         for _ in range(a.steps if ok else 1):
+            # a step: coefficient restriction to every level, diagonals, then the solve from u0 = 0
             sol.zero_()
+            op.prepareForSolve()
             stats.append(lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False))
         torch.cuda.synchronize()
         te = time.perf_counter()
@@ -176,7 +179,22 @@ def main():
             ms = e0.elapsed_time(e1) / a.fapply_reps
             bytes_per_launch = 64.0 * N  # (3 u + 3 f + 2 C) x 8 B per node, isotropic FP64
             fapply = dict(ms=ms, gbs=bytes_per_launch / (ms * 1e-3) / 1e9, bytes=bytes_per_launch)
-            del u, f
+            # Fsmooth (level 0, FP64): fused damped-Jacobi sweeps, the diagonal recomputed from
+            # lambda, mu, so (3 x + 3 b + 2 C + 3 x') x 8 B per node and sweep
+            b = torch.rand_like(u)
+            op.Fsmooth(0, 0, f, b)
+            torch.cuda.synchronize()
+            reps = max(1, a.fapply_reps // 5)
+            e0.record()
+            for _ in range(reps):
+                op.Fsmooth(0, 0, f, b)
+            e1.record()
+            torch.cuda.synchronize()
+            sw = op.desc.sweeps_per_smooth
+            ms_sweep = e0.elapsed_time(e1) / reps / sw
+            fapply["fsmooth_sweep_ms"] = ms_sweep
+            fapply["fsmooth_sweep_gbs"] = 88.0 * N / (ms_sweep * 1e-3) / 1e9
+            del u, f, b
         del op, lin, sol
         torch.cuda.empty_cache()
 
@@ -220,6 +238,8 @@ def main():
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
+            "fsmooth_sweep_ms": fapply["fsmooth_sweep_ms"] if fapply else None,
+            "fsmooth_sweep_gbs": fapply["fsmooth_sweep_gbs"] if fapply else None,
             "fapply_ms": fapply["ms"] if fapply else None,
         },
         "roofline": {
@@ -285,20 +305,36 @@ def cpu_baseline(pr, rhs, gpu_iters, cycle=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as ora
     from alamo_amd import problems as P
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=0)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=1, cycle=cycle,
-                 nu1=1 if cycle == 0 else 2, nu2=1 if cycle == 0 else 2)
-    t_one = time.perf_counter() - t0
-    per_cycle = max(t_one - t_setup, 1e-9)
-    est = t_setup + gpu_iters * per_cycle
-    return {"value": est, "unit": "s", "cores": cores, "kind": "port",
-            "sample": "oracle FP64 %s-cycle multigrid on the full 257^3 field: setup+residual (%.2fs) and one "
-                      "cycle (%.2fs) timed, scaled to the GPU solve's %d cycles"
-                      % ("V" if cycle == 0 else "F", t_setup, per_cycle, gpu_iters)}
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")  # the OpenMP runtime the oracle is linked against
+    cores = int(gomp.omp_get_max_threads())
+
+    def timed():
+        t0 = time.perf_counter()
+        ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=0)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=1, cycle=cycle,
+                     nu1=1 if cycle == 0 else 2, nu2=1 if cycle == 0 else 2)
+        t_one = time.perf_counter() - t0
+        per_cycle = max(t_one - t_setup, 1e-9)
+        return t_setup + gpu_iters * per_cycle, t_setup, per_cycle
+
+    def sample(ts, tc, nthr):
+        return ("oracle FP64 %s-cycle multigrid on the full 257^3 field, %d thread(s): setup+residual (%.2fs) and "
+                "one cycle (%.2fs) timed, scaled to the GPU solve's %d cycles"
+                % ("V" if cycle == 0 else "F", nthr, ts, tc, gpu_iters))
+
+    est, ts, tc = timed()
+    out = {"value": est, "unit": "s", "cores": cores, "kind": "port", "sample": sample(ts, tc, cores)}
+    # the reference's own loops run one rank per core without threads: the same sample on one core
+    gomp.omp_set_num_threads(1)
+    try:
+        est1, ts1, tc1 = timed()
+    finally:
+        gomp.omp_set_num_threads(cores)
+    out["single_core"] = {"value": est1, "unit": "s", "cores": 1, "sample": sample(ts1, tc1, 1)}
+    return out
 
 
 if __name__ == "__main__":
