@@ -90,6 +90,9 @@ struct alamo_mg {
     double* kV = nullptr;     // (m+1) basis vectors
     double* kZ = nullptr;     // m preconditioned vectors
     int kry_m = 0;
+    float* kVf = nullptr;     // FP32 basis and preconditioned vectors of the mixed-precision inner iterations
+    float* kZf = nullptr;
+    int kry_mf = 0;
     double* kcoef = nullptr;  // device copy of Gram-Schmidt / solution coefficients
     double* kpart = nullptr;  // multi-dot partials
     double* kdots = nullptr;  // multi-dot results (device)
@@ -486,7 +489,8 @@ static int krylov_scratch(alamo_mg* h) {
     return ALAMO_OK;
 }
 
-static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out) {
+template <typename T>
+static int multidot_t(alamo_mg* h, const T* w, const T* V, int64_t n, int nv, double* out) {
     if (nv < 1 || nv > KRY_MAXV) FAIL(ALAMO_ERR_ARG, "multidot: vector count out of range");
     TRY(krylov_scratch(h));
     const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
@@ -497,11 +501,15 @@ static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, in
     for (int i = 0; i < nv; i++) out[i] = h->red_host[i];
     return ALAMO_OK;
 }
+static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out) {
+    return multidot_t<double>(h, w, V, n, nv, out);
+}
 
 // w -= sum_{i < nv} c_i V_i with c already uploaded (upload_coef); *nrm = |w|_2 afterwards
-static int multiaxpy_norm(alamo_mg* h, double* w, const double* V, int64_t n, int nv, double* nrm) {
+template <typename T>
+static int multiaxpy_norm(alamo_mg* h, T* w, const T* V, int64_t n, int nv, double* nrm) {
     const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
-    k_multiaxpy_nrm<<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kcoef, h->kpart);
+    k_multiaxpy_nrm<T><<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kcoef, h->kpart);
     k_sum_rows<<<1, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
     HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -528,8 +536,32 @@ static int upload_coef(alamo_mg* h, const double* c, int nv) {
     return ALAMO_OK;
 }
 
-static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+// One FP32 preconditioner application of the mixed-precision inner iterations: z = cycle(v), FP32
+// in and out (the cycle's own FP32 level-0 buffers stage them).
+static int precond_f32(alamo_mg* h, const float* v, float* z) {
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
+        k_jacobi_from_zero<float, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, l0.corf, l0.Df, n,
+                                                                              (float)h->d.omega);
+        vcycle<float>(h, 0, true);
+    } else {
+        HIPCHK(hipMemcpyAsync(l0.resf, v, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+        one_cycle<float>(h);
+    }
+    HIPCHK(hipMemcpyAsync(z, l0.corf, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    return ALAMO_OK;
+}
+
+// FGMRES(m) with the Krylov vectors in precision TB.  TB = double: the FP64 iteration, any
+// preconditioner kind.  TB = float (mixed precision, cycle preconditioner): the basis, the
+// preconditioned vectors and A z are FP32 inside a restart cycle, a GMRES-based iterative
+// refinement (Carson and Higham, SIAM J. Sci. Comput. 40 (2018)): every restart recomputes the
+// FP64 true residual from the FP64 solution, and only that residual decides convergence.
+template <typename TB>
+static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        const alamo_krylov_opts* o, alamo_solve_stats* st) {
+    constexpr bool F32 = std::is_same<TB, float>::value;
     if (h->L.back().g.tot > 17 * 17 * 17 && o->precond != ALAMO_PRECOND_NONE)
         FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
     const int m = o->restart;
@@ -537,19 +569,32 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     if (!h->prepared) TRY(prepare(h));
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
-    if (h->kry_m < m) {
-        TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * n));
-        TRY(dalloc(h, &h->kZ, (int64_t)m * n));
-        TRY(krylov_scratch(h));
-        h->kry_m = m;
+    TRY(krylov_scratch(h));
+    TB *V, *Z;
+    if constexpr (F32) {
+        if (h->kry_mf < m) {
+            TRY(dalloc(h, &h->kVf, (int64_t)(m + 1) * n));
+            TRY(dalloc(h, &h->kZf, (int64_t)m * n));
+            h->kry_mf = m;
+        }
+        V = h->kVf;
+        Z = h->kZf;
+    } else {
+        if (h->kry_m < m) {
+            TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * n));
+            TRY(dalloc(h, &h->kZ, (int64_t)m * n));
+            h->kry_m = m;
+        }
+        V = h->kV;
+        Z = h->kZ;
     }
+    // FP64 true residual: its own buffer for FP32 inner vectors, else the first basis vector
+    double* r = F32 ? h->sol_res : (double*)V;
     HIPCHK(hipStreamSynchronize(h->stream));
     auto t0 = std::chrono::steady_clock::now();
-    double* V = h->kV;
-    double* Z = h->kZ;
-    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
+    launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, r, 0.0, h->stream);
     double r0 = 0, beta = 0, bn = 0;
-    TRY(norms_dev<double>(h, V, n, &r0, &beta));
+    TRY(norms_dev<double>(h, r, n, &r0, &beta));
     TRY(norms_dev<double>(h, rhs, n, &bn, nullptr));
     const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
     double rinf = r0;
@@ -561,42 +606,48 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     if (o->verbose) fprintf(stderr, "alamo fgmres: iter 0 |r|inf %.6e |r|2 %.6e target %.3e\n", r0, beta, target);
     while (rinf > target && it < o->max_iter && std::isfinite(rinf) && beta > 0.0) {
         const double ratio = rinf / beta;
-        k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(V, V, n, 1.0 / beta);
+        k_scale<double, TB><<<gs_blocks(n), 256, 0, h->stream>>>(V, r, n, 1.0 / beta);
         for (int i = 0; i <= m; i++) gv[i] = 0.0;
         gv[0] = beta;
         int jend = 0;
         for (int j = 0; j < m; j++) {
-            double* vj = V + (int64_t)j * n;
-            double* zj = Z + (int64_t)j * n;
-            double* w = V + (int64_t)(j + 1) * n;
-            const int prc = precond_apply(h, kind, vj, zj);
-            if (prc == NET_NONFINITE) {
-                // zj holds the plain cycle's result; the plain cycle takes over for good
-                kind = ALAMO_PRECOND_MG;
-                fallbacks++;
-                if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
-            } else if (prc != ALAMO_OK) {
-                return prc;
+            TB* vj = V + (int64_t)j * n;
+            TB* zj = Z + (int64_t)j * n;
+            TB* w = V + (int64_t)(j + 1) * n;
+            if constexpr (F32) {
+                TRY(precond_f32(h, vj, zj));
+                launch_node<float>(h->sym, APPLY, l0.g, h->bc, l0.Cf, zj, nullptr, w, 0.f, h->stream);
+            } else {
+                const int prc = precond_apply(h, kind, vj, zj);
+                if (prc == NET_NONFINITE) {
+                    // zj holds the plain cycle's result; the plain cycle takes over for good
+                    kind = ALAMO_PRECOND_MG;
+                    fallbacks++;
+                    if (o->verbose)
+                        fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
+                } else if (prc != ALAMO_OK) {
+                    return prc;
+                }
+                launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             }
-            launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             // classical Gram-Schmidt; w is vector j + 1 of the basis block, so the same multi-dot
             // pass returns |w|^2 in d1[j + 1]
-            TRY(multidot(h, w, V, n, j + 2, d1));
+            TRY(multidot_t<TB>(h, w, V, n, j + 2, d1));
             const double wn = std::sqrt(std::max(d1[j + 1], 0.0));
             TRY(upload_coef(h, d1, j + 1));
             double hn = 0;
-            TRY(multiaxpy_norm(h, w, V, n, j + 1, &hn));
+            TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &hn));
             for (int i = 0; i <= j; i++) H[i][j] = d1[i];
             if (hn < g_reorth_eta * wn) {
                 // the first pass cancelled most of w: re-orthogonalise (CGS2)
-                TRY(multidot(h, w, V, n, j + 1, d2));
+                TRY(multidot_t<TB>(h, w, V, n, j + 1, d2));
                 TRY(upload_coef(h, d2, j + 1));
-                TRY(multiaxpy_norm(h, w, V, n, j + 1, &hn));
+                TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &hn));
                 for (int i = 0; i <= j; i++) H[i][j] += d2[i];
                 reorths++;
             }
             H[j + 1][j] = hn;
-            if (hn > 0.0) k_scale<<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / hn);
+            if (hn > 0.0) k_scale<TB, TB><<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / hn);
             for (int i = 0; i < j; i++) {
                 const double a = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
                 H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
@@ -628,16 +679,16 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
             // a non-finite learned stage leaves the iterate untouched; the plain cycle takes over
             kind = ALAMO_PRECOND_MG;
             fallbacks++;
-            launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
-            TRY(norms_dev<double>(h, V, n, &rinf, &beta));
+            launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, r, 0.0, h->stream);
+            TRY(norms_dev<double>(h, r, n, &rinf, &beta));
             if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
             beta_prev = beta;
             continue;
         }
         TRY(upload_coef(h, y, jend));
-        k_multiaxpy_add<<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
-        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, V, 0.0, h->stream);
-        TRY(norms_dev<double>(h, V, n, &rinf, &beta));
+        k_multiaxpy_add<TB><<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
+        launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, r, 0.0, h->stream);
+        TRY(norms_dev<double>(h, r, n, &rinf, &beta));
         if (o->verbose)
             fprintf(stderr, "alamo fgmres: iter %d |r|inf %.6e |r|2 %.6e (%d second Gram-Schmidt passes)\n", it, rinf,
                     beta, reorths);
@@ -668,6 +719,20 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
     if (!std::isfinite(rinf)) FAIL(ALAMO_ERR_NAN, "alamo_krylov_solve: residual is not finite");
     if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_krylov_solve: max_iter reached before the tolerance");
     return ALAMO_OK;
+}
+
+// inner iterations in FP32 for mixed-precision handles with the cycle preconditioner
+// (ALAMO_KRY_F32=0 keeps them FP64, for comparisons)
+static const bool g_kry_f32 = [] {
+    const char* e = getenv("ALAMO_KRY_F32");
+    return !(e && strcmp(e, "0") == 0);
+}();
+
+static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                       const alamo_krylov_opts* o, alamo_solve_stats* st) {
+    if (g_kry_f32 && h->d.precision == ALAMO_PREC_MIXED && o->precond == ALAMO_PRECOND_MG)
+        return fgmres_core<float>(h, sol, rhs, tol_rel, tol_abs, o, st);
+    return fgmres_core<double>(h, sol, rhs, tol_rel, tol_abs, o, st);
 }
 
 // ===========================================================================

@@ -18,18 +18,20 @@ namespace alamo {
 constexpr int KRY_MAXV = 32;         // largest restart length + 1
 constexpr int KRY_DOT_BLOCKS = 4096;  // workgroups of the multi-dot pass (enough to saturate HBM)
 
+// Vectors are FP64, or FP32 for the inner iterations of the mixed-precision solve; sums and
+// coefficients are FP64 in both.
 // partial[i * nb + block] = sum over the block's share of w . V_i, i < NV.  NV is a
 // compile-time count so the NV accumulators stay in registers.
-template <int NV>
-__global__ void __launch_bounds__(RED_BLOCK) k_multidot(const double* __restrict__ w, const double* __restrict__ V,
-                                                        int64_t n, int64_t ldv, double* __restrict__ partial) {
+template <int NV, typename T>
+__global__ void __launch_bounds__(RED_BLOCK) k_multidot(const T* __restrict__ w, const T* __restrict__ V, int64_t n,
+                                                        int64_t ldv, double* __restrict__ partial) {
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const double wq = w[q];
+        const double wq = (double)w[q];
 #pragma unroll
-        for (int i = 0; i < NV; i++) acc[i] += wq * __builtin_nontemporal_load(V + i * ldv + q);
+        for (int i = 0; i < NV; i++) acc[i] += wq * (double)__builtin_nontemporal_load(V + i * ldv + q);
     }
     __shared__ double sh[NV][RED_BLOCK / 64];
 #pragma unroll
@@ -46,21 +48,22 @@ __global__ void __launch_bounds__(RED_BLOCK) k_multidot(const double* __restrict
 }
 
 // Launch k_multidot over nv <= KRY_MAXV vectors in register-sized groups of at most 8.
-inline void launch_multidot(const double* w, const double* V, int64_t n, int64_t ldv, int nv, int nb,
-                            double* partial, hipStream_t s) {
+template <typename T>
+inline void launch_multidot(const T* w, const T* V, int64_t n, int64_t ldv, int nv, int nb, double* partial,
+                            hipStream_t s) {
     for (int i0 = 0; i0 < nv; i0 += 8) {
         const int k = nv - i0 < 8 ? nv - i0 : 8;
-        const double* Vi = V + i0 * ldv;
+        const T* Vi = V + i0 * ldv;
         double* P = partial + (int64_t)i0 * nb;
         switch (k) {
-            case 1: k_multidot<1><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 2: k_multidot<2><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 3: k_multidot<3><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 4: k_multidot<4><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 5: k_multidot<5><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 6: k_multidot<6><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            case 7: k_multidot<7><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
-            default: k_multidot<8><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 1: k_multidot<1, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 2: k_multidot<2, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 3: k_multidot<3, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 4: k_multidot<4, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 5: k_multidot<5, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 6: k_multidot<6, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            case 7: k_multidot<7, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
+            default: k_multidot<8, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P); break;
         }
     }
 }
@@ -81,17 +84,19 @@ __global__ void __launch_bounds__(RED_BLOCK) k_sum_rows(const double* __restrict
     }
 }
 
-// w -= sum_{i < nv} c_i V_i and partial[block] = the block's share of |w|^2 afterwards
-__global__ void __launch_bounds__(RED_BLOCK) k_multiaxpy_nrm(double* __restrict__ w, const double* __restrict__ V,
-                                                             int64_t n, int64_t ldv, int nv,
-                                                             const double* __restrict__ c,
+// w -= sum_{i < nv} c_i V_i and partial[block] = the block's share of |w|^2 afterwards (of the
+// stored values)
+template <typename T>
+__global__ void __launch_bounds__(RED_BLOCK) k_multiaxpy_nrm(T* __restrict__ w, const T* __restrict__ V, int64_t n,
+                                                             int64_t ldv, int nv, const double* __restrict__ c,
                                                              double* __restrict__ partial) {
     double acc = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        double s = w[q];
-        for (int i = 0; i < nv; i++) s -= c[i] * V[i * ldv + q];
-        w[q] = s;
-        acc += s * s;
+        double s = (double)w[q];
+        for (int i = 0; i < nv; i++) s -= c[i] * (double)V[i * ldv + q];
+        const T t = (T)s;
+        w[q] = t;
+        acc += (double)t * (double)t;
     }
     __shared__ double sh[RED_BLOCK / 64];
     const double v = wave_sum(acc);
@@ -104,20 +109,22 @@ __global__ void __launch_bounds__(RED_BLOCK) k_multiaxpy_nrm(double* __restrict_
     }
 }
 
-// x += sum_{i < nv} c_i Z_i
-__global__ void __launch_bounds__(256) k_multiaxpy_add(double* __restrict__ x, const double* __restrict__ Z, int64_t n,
+// x += sum_{i < nv} c_i Z_i (x FP64)
+template <typename T>
+__global__ void __launch_bounds__(256) k_multiaxpy_add(double* __restrict__ x, const T* __restrict__ Z, int64_t n,
                                                        int64_t ldz, int nv, const double* __restrict__ c) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         double s = x[q];
-        for (int i = 0; i < nv; i++) s += c[i] * Z[i * ldz + q];
+        for (int i = 0; i < nv; i++) s += c[i] * (double)Z[i * ldz + q];
         x[q] = s;
     }
 }
 
-// y = a * x
-__global__ void k_scale(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a) {
+// y = a * x, converted to the precision of y
+template <typename Ti, typename To>
+__global__ void k_scale(To* __restrict__ y, const Ti* __restrict__ x, int64_t n, double a) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        y[q] = a * x[q];
+        y[q] = (To)(a * (double)x[q]);
 }
 
 }  // namespace alamo

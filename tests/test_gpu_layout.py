@@ -4,9 +4,11 @@ multiple of `row_align` nodes gives the compact handle's results through every e
 Operator rows, boundary values, the Newton right-hand side and the grid transfers run the same
 kernels with the same arithmetic on both layouts, so they must agree bit for bit (after
 dropping the pads); the pads themselves must stay zero.  Solves differ only in the order of
-the flat reductions (the pads move the block boundaries), so solutions agree to 1e-10 relative
-with the same iteration counts; the learned stage rounds its input to bf16 after a scale that
-differs in the last bits, so its output agrees to 1e-3 relative.
+the flat reductions (the pads move the block boundaries): FP64 solves agree to 1e-10 relative
+with iteration counts within one; mixed-precision Krylov solves, whose FP32 inner vectors round
+differently once a coefficient differs in its last bits, agree to 1e-8 (the bar of every
+mixed-vs-FP64 comparison) with counts within three.  The learned stage rounds its input to bf16
+after a scale that differs in the last bits, so its output agrees to 1e-3 relative.
 """
 import numpy as np
 import pytest
@@ -171,10 +173,10 @@ def test_layout_solve(cuda, precision, krylov):
         st = s.solve(sol, b, 1e-10, 0.0)
         assert st["converged"] == 1
         out.append((st["iterations"], op.from_layout(to_host(sol))))
-    assert abs(out[0][0] - out[1][0]) <= 1
     err = np.linalg.norm(out[0][1] - out[1][1]) / np.linalg.norm(out[0][1])
     print("iterations", out[0][0], out[1][0], "rel diff", err)
-    assert err <= 1e-10
+    assert abs(out[0][0] - out[1][0]) <= (3 if precision == PREC_MIXED else 1)
+    assert err <= (1e-8 if precision == PREC_MIXED else 1e-10)
 
 
 def test_layout_learned_stage(cuda):
