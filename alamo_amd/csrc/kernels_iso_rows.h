@@ -44,6 +44,17 @@ static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row
 #endif
 #define ALAMO_ROWS_ATTR __launch_bounds__(64 * ROWS_RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
 
+// rows per workgroup and planes per chunk of the aligned-tile kernel, per precision (the FP32
+// kernel needs about half the registers, which leaves room for taller workgroups)
+#ifndef ALAMO_ROWS64_RW_F32
+#define ALAMO_ROWS64_RW_F32 4
+#endif
+#ifndef ALAMO_ROWS64_KCH_F32
+#define ALAMO_ROWS64_KCH_F32 16
+#endif
+template <typename T> struct Rows64Cfg { static constexpr int RW = ROWS_RW, KCH = ROWS_KCH; };
+template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_F32, KCH = ALAMO_ROWS64_KCH_F32; };
+
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
 
@@ -253,12 +264,13 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, cons
 // by that tile's lane 63 from its halo value.
 // ---------------------------------------------------------------------------
 constexpr int R64_W = 66;  // LDS row: halo, 64 nodes, halo
-static_assert((ROWS_RW + 2) * 2 * 5 <= 64, "one gather covers the edge values of every staged row");
-
-template <typename T, int MODE, int FOLD>
-__global__ void ALAMO_ROWS_ATTR k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u,
-                                             const T* __restrict__ b, T* __restrict__ out, T omega) {
-    constexpr int NR = ROWS_RW + 2;  // staged rows: halo, RW rows, halo
+template <typename T, int MODE, int FOLD, int RW = Rows64Cfg<T>::RW, int KCH = Rows64Cfg<T>::KCH>
+__global__ void __launch_bounds__(64 * RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
+k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* __restrict__ b, T* __restrict__ out,
+             T omega) {
+    static_assert(RW >= 2, "the first and the last wavefront stage one halo row each");
+    constexpr int NR = RW + 2;  // staged rows: halo, RW rows, halo
+    static_assert(NR * 10 <= 64 * RW, "the wavefronts' gathers cover the edge values of every staged row");
     constexpr int A = NR * R64_W;
     __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 66 + 1 + lane]
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -271,9 +283,9 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows64(Geom g, const T* __restrict__ C, co
         bz = t / (gx * gy);
     }
     const int x0 = 64 * bx;  // lane l <-> node x0 + l
-    const int y0 = 1 + ROWS_RW * by;
-    const int k0 = 1 + ROWS_KCH * bz;
-    const int k1 = min(k0 + ROWS_KCH, g.n[2]);  // output planes k0 .. k1 - 1; loads k0 - 1 .. k1
+    const int y0 = 1 + RW * by;
+    const int k0 = 1 + KCH * bz;
+    const int k1 = min(k0 + KCH, g.n[2]);  // output planes k0 .. k1 - 1; loads k0 - 1 .. k1
     const int x = x0 + lane;
     const int xl = min(x, g.n[0]);
     const int y = y0 + w;
@@ -286,11 +298,12 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows64(Geom g, const T* __restrict__ C, co
     const uint32_t szb = (uint32_t)g.sz * E;
     const uint32_t colb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
     const uint32_t ecolb = ((uint32_t)min(x0 + 64, g.n[0]) + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
-    const int hrow = w == 0 ? 0 : (w == ROWS_RW - 1 ? NR - 1 : -1);
-    const uint32_t hcolb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]))) * E;
-    // edge gather (first wavefront, lanes 0 .. 10 NR - 1): staged row gr, edge ge, array ga
-    const bool gl = w == 0 && lane < NR * 10;
-    const int gr = lane / 10, ge = (lane / 5) & 1, ga = lane % 5;
+    const int hrow = w == 0 ? 0 : (w == RW - 1 ? NR - 1 : -1);
+    const uint32_t hcolb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)(w == 0 ? y0 - 1 : min(y0 + RW, g.n[1]))) * E;
+    // edge gather, entry gi = 64 w + lane < 10 NR: staged row gr, edge ge, array ga
+    const int gi = 64 * w + lane;
+    const bool gl = gi < NR * 10;
+    const int gr = gi / 10, ge = (gi / 5) & 1, ga = gi % 5;
     const int gy = gr == 0 ? y0 - 1 : min(y0 + gr - 1, g.n[1]);
     const int gx = ge == 0 ? max(x0 - 1, 0) : min(x0 + 64, g.n[0]);
     const T* __restrict__ gbase = ga < 3 ? u + (int64_t)ga * g.tot : C + (int64_t)(ga - 3) * g.tot;
@@ -688,8 +701,9 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
     dim3 blk(64, ROWS_RW);
     if (g_rows_tile64) {
         // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
-        dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW, (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
-        k_iso_rows64<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
+        constexpr int RW = Rows64Cfg<T>::RW, KCH = Rows64Cfg<T>::KCH;
+        dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
+        k_iso_rows64<T, MODE, FOLD><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
         return;
     }
     dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,

@@ -54,7 +54,51 @@ struct Level {
     inline bool bnd(int i, int j, int k) const {
         return i == 0 || j == 0 || k == 0 || i == n[0] || j == n[1] || k == n[2];
     }
+    // Elastic::m_psi_mf of this level (cell field, n[0] n[1] n[2] values, i fastest) and
+    // m_psi_small; psi == nullptr is the reference's m_psi_set == false
+    const double* psi = nullptr;
+    double psi_small = 1e-8;
 };
+
+// ---------------------------------------------------------------------------
+// psi (Operator/Elastic.cpp:118-137,191-192,255-260; Numeric/Stencil.H:699-715,1385-1406).
+// The operator fills its whole psi field, ghost cells included, with 1 at define time
+// (Elastic.cpp:49-52) and SetPsi copies only the domain's cells (:122-133), so a cell outside
+// the non-periodic domain reads 1.
+// ---------------------------------------------------------------------------
+static inline double psi_cell(const Level& L, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= L.n[0] || j >= L.n[1] || k >= L.n[2]) return 1.0;
+    return L.psi[i + (int64_t)L.n[0] * (j + (int64_t)L.n[1] * k)];
+}
+// Interpolate::CellToNodeAverage: mean of the 8 cells around node (i,j,k)
+static inline double psi_node_mean(const Level& L, int i, int j, int k) {
+    double s = 0.0;
+    for (int c = 0; c < 8; c++) s += psi_cell(L, i - (c & 1), j - ((c >> 1) & 1), k - (c >> 2));
+    return s * 0.125;
+}
+// psi_avg = (1 - psi_small) * CellToNodeAverage(psi) + psi_small   (Elastic.cpp:191-192)
+static inline double psi_avg(const Level& L, int i, int j, int k) {
+    return L.psi ? (1.0 - L.psi_small) * psi_node_mean(L, i, j, k) + L.psi_small : 1.0;
+}
+// Numeric::CellGradientOnNode (Stencil.H:709-712): for d, the mean over the 4 cell pairs that
+// straddle the node in d of (cell on the + side - cell on the - side) / dx_d
+static inline void psi_grad(const Level& L, int i, int j, int k, double g[3]) {
+    const int idx[3] = {i, j, k};
+    for (int d = 0; d < 3; d++) {
+        const int a = (d + 1) % 3, b = (d + 2) % 3;
+        double s = 0.0;
+        for (int qa = 0; qa < 2; qa++)
+            for (int qb = 0; qb < 2; qb++) {
+                int p[3], q[3];
+                p[d] = idx[d];
+                q[d] = idx[d] - 1;
+                p[a] = q[a] = idx[a] - qa;
+                p[b] = q[b] = idx[b] - qb;
+                s += psi_cell(L, p[0], p[1], p[2]) - psi_cell(L, q[0], q[1], q[2]);
+            }
+        g[d] = 0.25 * s / L.dx[d];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Set::Matrix4 element access.  Pinned by tests/test_oracle_tensor.py (dense
@@ -217,8 +261,8 @@ static inline void grad_u(const Level& L, const double* u, int i, int j, int k, 
 
 // ---------------------------------------------------------------------------
 // a1: Operator::Elastic<SYM>::Fapply (Operator/Elastic.cpp:139-292), single
-// box covering the whole non-periodic domain, psi unset (psi_avg = 1),
-// m_uniform = false (Integrator/Base/Mechanics.H:194).
+// box covering the whole non-periodic domain, m_uniform = false
+// (Integrator/Base/Mechanics.H:194); with psi set, the psi terms of :191-195,243-260.
 //   interior f_i = C_ijkl u_k,lj + (dC/dx_d)_idkl u_k,l          (:211-254)
 //   boundary f   = BC(u, grad u, C:grad u)                        (:195-203)
 // Pinned by tests/test_oracle_fapply.py: patch test, quadratic fields with
@@ -232,11 +276,13 @@ static void fapply_node(const Level& L, int sym, const double* C, const int* bct
     const double* c = C + m * nc;
     double G[3][3];
     grad_u(L, u, i, j, k, G);
+    const double pa = psi_avg(L, i, j, k);
     if (L.bnd(i, j, k)) {
         int lo[3] = {0, 0, 0}, hi[3] = {L.n[0], L.n[1], L.n[2]};
         int face = classify(i, j, k, lo, hi);
         double S[3][3], uu[3];
         c_dot_mat(sym, c, G, S);
+        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) S[a][b] *= pa;  // sig = (DDW grad u) psi_avg (:195)
         for (int p = 0; p < 3; p++) uu[p] = u[p * L.tot + m];
         bc_row(bct, face, uu, G, S, f);
         return;
@@ -272,6 +318,17 @@ static void fapply_node(const Level& L, int sym, const double* C, const int* bct
             double s = 0.0;
             for (int kk = 0; kk < 3; kk++) for (int ll = 0; ll < 3; ll++) s += C4(sym, dc, ii, d, kk, ll) * G[kk][ll];
             f[ii] += s;
+        }
+    }
+    if (L.psi) {
+        // both terms above scale by psi_avg (:243,:253); then f += (DDW grad u) grad psi (1 - psi_small) (:255-260)
+        double S[3][3], gp[3];
+        c_dot_mat(sym, c, G, S);
+        psi_grad(L, i, j, k, gp);
+        for (int ii = 0; ii < 3; ii++) {
+            double s = 0.0;
+            for (int jj = 0; jj < 3; jj++) s += S[ii][jj] * gp[jj] * (1.0 - L.psi_small);
+            f[ii] = f[ii] * pa + s;
         }
     }
 }
@@ -315,6 +372,7 @@ static void diagonal(const Level& L, int sym, const double* C, const int* bct, d
                         double G[3][3] = {{0}}, S[3][3], uu[3] = {0, 0, 0}, f[3];
                         for (int d = 0; d < 3; d++) G[p][d] = cd[d];
                         c_dot_mat(sym, c, G, S);
+                        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) S[a][b] *= psi_avg(L, i, j, k);
                         uu[p] = 1.0;
                         bc_row(bct, face, uu, G, S, f);
                         D[p * L.tot + m] = f[p];
@@ -323,7 +381,7 @@ static void diagonal(const Level& L, int sym, const double* C, const int* bct, d
                     for (int p = 0; p < 3; p++) {
                         double s = 0.0;
                         for (int jj = 0; jj < 3; jj++) s += C4(sym, c, p, jj, p, jj) * (-2.0 / L.dx[jj] / L.dx[jj]);
-                        D[p * L.tot + m] = s;
+                        D[p * L.tot + m] = s * psi_avg(L, i, j, k);  // Elastic.cpp:363
                     }
                 }
             }
@@ -479,14 +537,58 @@ struct Hier {
     int sym;
     const int* bct;
     std::vector<Level> L;
-    std::vector<std::vector<double>> C, D;
+    std::vector<std::vector<double>> C, D, psi;
 };
 
+// Restriction of psi to the next multigrid level (Operator/Elastic.cpp:839-907): coarse cell
+// (I,J,K) from fine cells around (2I,2J,2K) with the nodal full-weighting weights.  The
+// branch bounds are the coarse level's nodal box [0, n], which a cell index reaches only on the
+// lo side, so only I == 0 (J == 0, K == 0) takes the face / edge / corner stencils.
+static void restrict_psi(const Level& Lc, const Level& Lf, const double* fine, double* crse) {
+    auto F = [&](int i, int j, int k) { return fine[i + (int64_t)Lf.n[0] * (j + (int64_t)Lf.n[1] * k)]; };
+    for (int K = 0; K < Lc.n[2]; K++)
+        for (int J = 0; J < Lc.n[1]; J++)
+            for (int I = 0; I < Lc.n[0]; I++) {
+                const int i = 2 * I, j = 2 * J, k = 2 * K;
+                const bool xl = I == 0, yl = J == 0, zl = K == 0;
+                double v;
+                if (xl && yl && zl) v = F(i, j, k);
+                else if (yl && zl) v = F(i - 1, j, k) * 0.25 + F(i, j, k) * 0.5 + F(i + 1, j, k) * 0.25;
+                else if (zl && xl) v = F(i, j - 1, k) * 0.25 + F(i, j, k) * 0.5 + F(i, j + 1, k) * 0.25;
+                else if (xl && yl) v = F(i, j, k - 1) * 0.25 + F(i, j, k) * 0.5 + F(i, j, k + 1) * 0.25;
+                else if (xl)
+                    v = (F(i, j - 1, k - 1) + F(i, j, k - 1) * 2.0 + F(i, j + 1, k - 1) + F(i, j - 1, k) * 2.0 +
+                         F(i, j, k) * 4.0 + F(i, j + 1, k) * 2.0 + F(i, j - 1, k + 1) + F(i, j, k + 1) * 2.0 +
+                         F(i, j + 1, k + 1)) / 16.0;
+                else if (yl)
+                    v = (F(i - 1, j, k - 1) + F(i - 1, j, k) * 2.0 + F(i - 1, j, k + 1) + F(i, j, k - 1) * 2.0 +
+                         F(i, j, k) * 4.0 + F(i, j, k + 1) * 2.0 + F(i + 1, j, k - 1) + F(i + 1, j, k) * 2.0 +
+                         F(i + 1, j, k + 1)) / 16.0;
+                else if (zl)
+                    v = (F(i - 1, j - 1, k) + F(i, j - 1, k) * 2.0 + F(i + 1, j - 1, k) + F(i - 1, j, k) * 2.0 +
+                         F(i, j, k) * 4.0 + F(i + 1, j, k) * 2.0 + F(i - 1, j + 1, k) + F(i, j + 1, k) * 2.0 +
+                         F(i + 1, j + 1, k)) / 16.0;
+                else {
+                    double c8 = 0, c12 = 0, c6 = 0;
+                    for (int a = -1; a <= 1; a++)
+                        for (int b = -1; b <= 1; b++)
+                            for (int c = -1; c <= 1; c++) {
+                                const int nz = (a != 0) + (b != 0) + (c != 0);
+                                if (nz == 3) c8 += F(i + a, j + b, k + c);
+                                else if (nz == 2) c12 += F(i + a, j + b, k + c);
+                                else if (nz == 1) c6 += F(i + a, j + b, k + c);
+                            }
+                    v = c8 / 64.0 + c12 / 32.0 + c6 / 16.0 + F(i, j, k) / 8.0;
+                }
+                crse[I + (int64_t)Lc.n[0] * (J + (int64_t)Lc.n[1] * K)] = v;
+            }
+}
+
 static void build_hier(Hier& H, const int* ncell, const double* dx, int sym, const int* bct, const double* C0,
-                       int max_coarsening) {
+                       int max_coarsening, const double* psi0 = nullptr, double psi_small = 1e-8) {
     H.sym = sym;
     H.bct = bct;
-    H.L.clear(); H.C.clear(); H.D.clear();
+    H.L.clear(); H.C.clear(); H.D.clear(); H.psi.clear();
     int n[3] = {ncell[0], ncell[1], ncell[2]};
     double h[3] = {dx[0], dx[1], dx[2]};
     H.L.push_back(Level(n, h));
@@ -503,6 +605,18 @@ static void build_hier(Hier& H, const int* ncell, const double* dx, int sym, con
     for (size_t l = 1; l < H.L.size(); l++) {
         H.C[l].resize(H.L[l].tot * nc);
         restrict_fw(H.L[l], H.L[l - 1], nc, 0, H.C[l - 1].data(), H.C[l].data());   // Elastic.cpp:746-836
+    }
+    if (psi0) {
+        H.psi.resize(H.L.size());
+        H.psi[0].assign(psi0, psi0 + (int64_t)H.L[0].n[0] * H.L[0].n[1] * H.L[0].n[2]);
+        for (size_t l = 1; l < H.L.size(); l++) {
+            H.psi[l].resize((int64_t)H.L[l].n[0] * H.L[l].n[1] * H.L[l].n[2]);
+            restrict_psi(H.L[l], H.L[l - 1], H.psi[l - 1].data(), H.psi[l].data());
+        }
+        for (size_t l = 0; l < H.L.size(); l++) {
+            H.L[l].psi = H.psi[l].data();
+            H.L[l].psi_small = psi_small;
+        }
     }
     for (size_t l = 0; l < H.L.size(); l++) {
         H.D[l].resize(3 * H.L[l].tot);
@@ -639,8 +753,8 @@ static int mg_solve(Hier& H, const MGOpts& o, double* x, const double* b, double
 }
 
 // ---------------------------------------------------------------------------
-// a17: Solver::Nonlocal::Newton<T>::prepareForSolve, vector/no-psi path
-// (Solver/Nonlocal/Newton.H:144-231) for a linear or affine model with
+// a17: Solver::Nonlocal::Newton<T>::prepareForSolve, vector path
+// (Solver/Nonlocal/Newton.H:144-231; with L.psi set, the psi variant :234-266) for a linear or affine model with
 // DW = C : (grad u - F0) (Model/Solid/Affine/Isotropic.H:38-49,
 // Model/Solid/Linear/Cubic.H:34-41):
 //   boundary: rhs = b - BC(u, grad u, DW)      (:220-225)
@@ -671,6 +785,7 @@ static void newton_rhs(const Level& L, int sym, const double* C, const double* F
                 int64_t m = L.id(i, j, k);
                 double f[3];
                 if (L.bnd(i, j, k)) {
+                    // with psi the boundary rows keep psiavg = 1: the node is on the box's edge (:247-252)
                     double G[3][3], S[3][3], uu[3];
                     grad_u(L, u, i, j, k, G);
                     for (int a = 0; a < 3; a++) for (int c = 0; c < 3; c++) S[a][c] = DW[9 * m + 3 * a + c];
@@ -682,6 +797,17 @@ static void newton_rhs(const Level& L, int sym, const double* C, const double* F
                         for (int d = 0; d < 3; d++)
                             s += (DW[9 * (m + st[d]) + 3 * a + d] - DW[9 * (m - st[d]) + 3 * a + d]) / 2. / L.dx[d];
                         f[a] = s;
+                    }
+                    if (L.psi) {
+                        // divdw *= CellToNodeAverage(psi); divdw += dw grad psi  (Newton.H:258-263; no psi_small)
+                        const double pm = psi_node_mean(L, i, j, k);
+                        double gp[3];
+                        psi_grad(L, i, j, k, gp);
+                        for (int a = 0; a < 3; a++) {
+                            double s = 0.0;
+                            for (int d = 0; d < 3; d++) s += DW[9 * m + 3 * a + d] * gp[d];
+                            f[a] = f[a] * pm + s;
+                        }
                     }
                 }
                 for (int p = 0; p < 3; p++) rhs[p * L.tot + m] = b[p * L.tot + m] - f[p];
@@ -806,6 +932,43 @@ void ora_newton_rhs(const int* ncell, const double* dx, int sym, const double* C
                     const double* u, const double* b, double* rhs) {
     Level L(ncell, dx);
     newton_rhs(L, sym, C, F0, bct, u, b, rhs, nullptr);
+}
+// psi variants: psi is the level-0 cell field (ncell[0] ncell[1] ncell[2], i fastest)
+void ora_fapply_psi(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* psi,
+                    double psi_small, const double* u, double* f) {
+    Level L(ncell, dx);
+    L.psi = psi;
+    L.psi_small = psi_small;
+    fapply(L, sym, C, bct, u, f);
+}
+void ora_diagonal_psi(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* psi,
+                      double psi_small, double* D) {
+    Level L(ncell, dx);
+    L.psi = psi;
+    L.psi_small = psi_small;
+    diagonal(L, sym, C, bct, D);
+}
+void ora_newton_rhs_psi(const int* ncell, const double* dx, int sym, const double* C, const double* F0,
+                        const int* bct, const double* psi, const double* u, const double* b, double* rhs) {
+    Level L(ncell, dx);
+    L.psi = psi;
+    newton_rhs(L, sym, C, F0, bct, u, b, rhs, nullptr);
+}
+// restricted psi of multigrid level `level` (cells of that level)
+void ora_psi_level(const int* ncell, const double* dx, const double* psi, int level, double* out) {
+    Hier H;
+    int dummy[78] = {0};
+    std::vector<double> C((int64_t)(ncell[0] + 1) * (ncell[1] + 1) * (ncell[2] + 1) * 2, 1.0);
+    build_hier(H, ncell, dx, 0, dummy, C.data(), level, psi, 1e-8);
+    if (level < (int)H.L.size()) std::copy(H.psi[level].begin(), H.psi[level].end(), out);
+}
+int ora_mg_solve_psi(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* psi,
+                     double psi_small, double* x, const double* b, double tol_rel, double tol_abs, int cycle, int nu1,
+                     int nu2, int sweeps, double omega, int max_iter, double* final_res) {
+    Hier H;
+    build_hier(H, ncell, dx, sym, bct, C, -1, psi, psi_small);
+    MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, 1e-4};
+    return mg_solve(H, o, x, b, tol_rel, tol_abs, nullptr, 0, final_res);
 }
 void ora_bc_init(const int* ncell, const int* bct, const double* bcv, int homogeneous, double* b) {
     double dx[3] = {1, 1, 1};

@@ -136,6 +136,50 @@ def newton_rhs(ncell, dx, sym, C, F0, bct, u, b):
     return rhs
 
 
+# psi variants (Operator/Elastic.cpp psi terms): psi is the level-0 cell field, prod(ncell) values
+def fapply_psi(ncell, dx, sym, C, bct, psi, psi_small, u):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); u = _c(u); psi = _c(psi)
+    f = np.zeros_like(u)
+    lib().ora_fapply_psi(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(psi), ctypes.c_double(psi_small), _d(u), _d(f))
+    return f
+
+
+def diagonal_psi(ncell, dx, sym, C, bct, psi, psi_small):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); psi = _c(psi)
+    Dg = np.zeros(3 * int(np.prod(nodes(ncell))))
+    lib().ora_diagonal_psi(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(psi), ctypes.c_double(psi_small), _d(Dg))
+    return Dg
+
+
+def newton_rhs_psi(ncell, dx, sym, C, F0, bct, psi, u, b):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); u = _c(u); b = _c(b)
+    psi = _c(psi)
+    rhs = np.zeros_like(b)
+    f0p = None if F0 is None else _c(F0)
+    lib().ora_newton_rhs_psi(_i(ncell), _d(dx), sym, _d(C), None if f0p is None else _d(f0p), _i(bct), _d(psi),
+                             _d(u), _d(b), _d(rhs))
+    return rhs
+
+
+def psi_level(ncell, dx, psi, level):
+    ncell = _c(ncell, np.int32); dx = _c(dx); psi = _c(psi)
+    nc = [int(n) // (2 ** level) for n in ncell]
+    out = np.zeros(int(np.prod(nc)))
+    lib().ora_psi_level(_i(ncell), _d(dx), _d(psi), level, _d(out))
+    return out
+
+
+def mg_solve_psi(ncell, dx, sym, C, bct, psi, psi_small, b, tol_rel=1e-10, tol_abs=0.0, cycle=1, nu1=2, nu2=2,
+                 sweeps=2, omega=2.0 / 3.0, max_iter=200):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); b = _c(b); psi = _c(psi)
+    x = np.zeros_like(b)
+    fr = ctypes.c_double(0)
+    it = lib().ora_mg_solve_psi(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(psi), ctypes.c_double(psi_small), _d(x),
+                                _d(b), ctypes.c_double(tol_rel), ctypes.c_double(tol_abs), cycle, nu1, nu2, sweeps,
+                                ctypes.c_double(omega), max_iter, ctypes.byref(fr))
+    return x, it, fr.value
+
+
 def bc_init(ncell, bct, bcv, homogeneous=False, b=None):
     ncell = _c(ncell, np.int32); bct = _c(bct, np.int32); bcv = _c(bcv)
     N = int(np.prod(nodes(ncell)))
