@@ -62,6 +62,8 @@ SIGNATURES = {
     "alamo_mg_num_levels": (c_int, [c_void_p, P(c_int)]),
     "alamo_mg_level_ncell": (c_int, [c_void_p, c_int, P(c_int)]),
     "alamo_mg_level_layout": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),
+    "alamo_mg_set_psi_host": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_model": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "alamo_mg_set_model_host": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "alamo_mg_set_model_grains_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),

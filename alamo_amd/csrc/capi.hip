@@ -67,6 +67,8 @@ struct LevelBuf {
     float* Cf = nullptr;
     float* Df = nullptr;
     float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
+    double* psi = nullptr;  // Elastic::m_psi_mf of the level (cells), when set
+    float* psif = nullptr;
 };
 
 }  // namespace
@@ -93,6 +95,8 @@ struct alamo_mg {
     float* kVf = nullptr;     // FP32 basis and preconditioned vectors of the mixed-precision inner iterations
     float* kZf = nullptr;
     int kry_mf = 0;
+    bool psi_set = false;     // Elastic::m_psi_set
+    double psi_small = 1e-8;  // Elastic::m_psi_small
     double* kcoef = nullptr;  // device copy of Gram-Schmidt / solution coefficients
     double* kpart = nullptr;  // multi-dot partials
     double* kdots = nullptr;  // multi-dot results (device)
@@ -149,6 +153,13 @@ static void launch_node_sym(int mode, const Geom& g, const BCTable& bc, const T*
 template <typename T>
 static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b,
                         T* out, T omega, hipStream_t s) {
+    if (g.psi) {
+        // the psi terms live in the node-parallel rows
+        if (sym == ISO) launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);
+        else if (sym == MAJORMINOR) launch_node_sym<T, MAJORMINOR>(mode, g, bc, C, u, b, out, omega, s);
+        else launch_node_sym<T, DIAGONAL>(mode, g, bc, C, u, b, out, omega, s);
+        return;
+    }
     if (sym == ISO) {
         // large isotropic levels take the LDS-tiled interior kernel plus the boundary kernel
         if (mode != DIAG && iso_tiled_ok(g)) {
@@ -354,6 +365,20 @@ static int prepare(alamo_mg* h) {
         LevelBuf& lf = h->L[l - 1];
         LevelBuf& lc = h->L[l];
         k_restrict<double><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lf.g, h->nc, lf.C, lc.C);
+    }
+    if (h->psi_set) {
+        // Elastic::averageDownCoeffs restricts psi to every level too (Elastic.cpp:839-907)
+        for (int l = 0; l < nl; l++) {
+            LevelBuf& lb = h->L[l];
+            const int64_t nc = (int64_t)lb.g.n[0] * lb.g.n[1] * lb.g.n[2];
+            if (l > 0)
+                k_restrict_psi<<<nblocks(nc), 256, 0, h->stream>>>(lb.g, h->L[l - 1].g, h->L[l - 1].psi, lb.psi);
+            if (h->d.precision == ALAMO_PREC_MIXED)
+                k_convert<double, float><<<gs_blocks(nc), 256, 0, h->stream>>>(lb.psi, lb.psif, nc);
+            lb.g.psi = lb.psi;
+            lb.g.psif = lb.psif;
+            lb.g.psi_small = h->psi_small;
+        }
     }
     for (int l = 0; l < nl; l++) {
         LevelBuf& lb = h->L[l];
@@ -936,6 +961,40 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
     return set_model_impl(h, 0, soa.data(), ALAMO_LAYOUT_SOA, true);
 }
 
+static int set_psi_impl(alamo_mg* h, int amrlev, const double* psi, double psi_small, bool host) {
+    if (!h || !psi) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
+    if (!(psi_small >= 0.0 && psi_small < 1.0)) FAIL(ALAMO_ERR_ARG, "psi_small must be in [0, 1)");
+    const bool mixed = h->d.precision == ALAMO_PREC_MIXED;
+    for (LevelBuf& lb : h->L) {
+        const int64_t nc = (int64_t)lb.g.n[0] * lb.g.n[1] * lb.g.n[2];
+        if (!lb.psi) TRY(dalloc(h, &lb.psi, nc));
+        if (mixed && !lb.psif) TRY(dalloc(h, &lb.psif, nc));
+    }
+    LevelBuf& l0 = h->L[0];
+    const int64_t nc0 = (int64_t)l0.g.n[0] * l0.g.n[1] * l0.g.n[2];
+    HIPCHK(hipMemcpyAsync(l0.psi, psi, nc0 * sizeof(double), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                          h->stream));
+    if (mixed) k_convert<double, float><<<gs_blocks(nc0), 256, 0, h->stream>>>(l0.psi, l0.psif, nc0);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->psi_set = true;
+    h->psi_small = psi_small;
+    // the finest level applies psi at once; the coarse levels get theirs at prepare
+    l0.g.psi = l0.psi;
+    l0.g.psif = l0.psif;
+    l0.g.psi_small = psi_small;
+    h->prepared = false;
+    h->model_version = ++g_model_version;
+    return ALAMO_OK;
+}
+
+int alamo_mg_set_psi(alamo_mg* h, int amrlev, const double* psi, double psi_small) {
+    return set_psi_impl(h, amrlev, psi, psi_small, false);
+}
+int alamo_mg_set_psi_host(alamo_mg* h, int amrlev, const double* psi, double psi_small) {
+    return set_psi_impl(h, amrlev, psi, psi_small, true);
+}
+
 int alamo_mg_prepare(alamo_mg* h) {
     if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
     return prepare(h);
@@ -1158,6 +1217,18 @@ __global__ void k_newton_rhs(Geom g, BCTable bc, const double* __restrict__ DW, 
                 s += (P[m + st[d]] - P[m - st[d]]) / 2. / g.dx[d];
             }
             f[a] = s;
+        }
+        if (g.psi) {
+            // divdw *= CellToNodeAverage(psi); divdw += dw grad psi (Newton.H:258-263, no psi_small);
+            // the boundary rows above keep psiavg = 1 (the node lies on the box's edge, :247-252)
+            const double pm = psi_node_mean(g.psi, g, i, j, k);
+            double gp[3];
+            psi_grad(g.psi, g, i, j, k, gp);
+            for (int a = 0; a < 3; a++) {
+                double s = 0.0;
+                for (int d = 0; d < 3; d++) s += DW[(3 * a + d) * g.tot + m] * gp[d];
+                f[a] = f[a] * pm + s;
+            }
         }
     }
     for (int p = 0; p < 3; p++) rhs[p * g.tot + m] = b[p * g.tot + m] - f[p];

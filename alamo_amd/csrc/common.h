@@ -39,6 +39,11 @@ struct Geom {
     int64_t sy, sz; // strides of j and k
     int64_t tot;    // nodes per component plane
     double dx[3];
+    // Elastic's psi on this level (cell field, n[0] n[1] n[2] values, i fastest), FP64 and FP32
+    // copies; null when psi is not set
+    const double* psi;
+    const float* psif;
+    double psi_small;
 };
 
 // Boundary types per region and component, packed for kernel arguments.

@@ -39,6 +39,7 @@ __device__ __forceinline__ void node_row(const Geom& g, const BCTable& bc, const
         if (u) apply_interior<T, SYM>(g, C, u, m, f, c);
         else load_coef<T, SYM>(C, g.tot, m, c);
         if (NEED_DIAG) diag_interior<T, SYM>(g, c, D);
+        psi_interior<T, SYM>(g, C, u, i, j, k, f, NEED_DIAG ? D : nullptr);
     }
 }
 
@@ -213,6 +214,47 @@ __global__ void __launch_bounds__(256) k_restrict(Geom gc, Geom gf, int np, cons
         }
         crse[p * gc.tot + mc] = v;
     }
+}
+
+// Restriction of the psi cell field (Operator/Elastic.cpp:839-907): coarse cell (I,J,K) from the
+// fine cells around (2I,2J,2K) with the nodal full-weighting weights; the branch bounds are the
+// coarse nodal box [0, n], which a cell index reaches only on the lo side.
+__global__ void __launch_bounds__(256) k_restrict_psi(Geom gc, Geom gf, const double* __restrict__ fine,
+                                                      double* __restrict__ crse) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n01 = (int64_t)gc.n[0] * gc.n[1];
+    if (q >= n01 * gc.n[2]) return;
+    const int K = (int)(q / n01), J = (int)((q / gc.n[0]) % gc.n[1]), I = (int)(q % gc.n[0]);
+    const int i = 2 * I, j = 2 * J, k = 2 * K;
+    auto F = [&](int a, int b, int c) { return fine[(i + a) + (int64_t)gf.n[0] * ((j + b) + (int64_t)gf.n[1] * (k + c))]; };
+    const bool xl = I == 0, yl = J == 0, zl = K == 0;
+    double v;
+    if (xl && yl && zl) v = F(0, 0, 0);
+    else if (yl && zl) v = F(-1, 0, 0) * 0.25 + F(0, 0, 0) * 0.5 + F(1, 0, 0) * 0.25;
+    else if (zl && xl) v = F(0, -1, 0) * 0.25 + F(0, 0, 0) * 0.5 + F(0, 1, 0) * 0.25;
+    else if (xl && yl) v = F(0, 0, -1) * 0.25 + F(0, 0, 0) * 0.5 + F(0, 0, 1) * 0.25;
+    else if (xl)
+        v = (F(0, -1, -1) + F(0, 0, -1) * 2.0 + F(0, 1, -1) + F(0, -1, 0) * 2.0 + F(0, 0, 0) * 4.0 + F(0, 1, 0) * 2.0 +
+             F(0, -1, 1) + F(0, 0, 1) * 2.0 + F(0, 1, 1)) / 16.0;
+    else if (yl)
+        v = (F(-1, 0, -1) + F(-1, 0, 0) * 2.0 + F(-1, 0, 1) + F(0, 0, -1) * 2.0 + F(0, 0, 0) * 4.0 + F(0, 0, 1) * 2.0 +
+             F(1, 0, -1) + F(1, 0, 0) * 2.0 + F(1, 0, 1)) / 16.0;
+    else if (zl)
+        v = (F(-1, -1, 0) + F(0, -1, 0) * 2.0 + F(1, -1, 0) + F(-1, 0, 0) * 2.0 + F(0, 0, 0) * 4.0 + F(1, 0, 0) * 2.0 +
+             F(-1, 1, 0) + F(0, 1, 0) * 2.0 + F(1, 1, 0)) / 16.0;
+    else {
+        double c8 = 0, c12 = 0, c6 = 0;
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++)
+                for (int c = -1; c <= 1; c++) {
+                    const int nz = (a != 0) + (b != 0) + (c != 0);
+                    if (nz == 3) c8 += F(a, b, c);
+                    else if (nz == 2) c12 += F(a, b, c);
+                    else if (nz == 1) c6 += F(a, b, c);
+                }
+        v = c8 / 64.0 + c12 / 32.0 + c6 / 16.0 + F(0, 0, 0) / 8.0;
+    }
+    crse[q] = v;
 }
 
 // fine += I(crse), trilinear nodal prolongation by parity (Operator/Operator.cpp:399-428).

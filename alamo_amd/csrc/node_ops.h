@@ -5,8 +5,8 @@
 //                                                                   BC/Operator/Elastic/Constant.H:344-425)
 //   diagonal        Operator/Elastic.cpp:296-379 with Numeric/Stencil.H:906-963
 //
-// psi is unset (psi_avg = 1) and m_uniform is false, the configuration
-// Integrator/Base/Mechanics.H:193-198 builds.
+// m_uniform is false, the configuration Integrator/Base/Mechanics.H:193-198 builds; with psi set
+// (Elastic::SetPsi) the psi terms of Elastic.cpp:191-195,243-260,340-363 apply.
 #pragma once
 #include <type_traits>
 #include "common.h"
@@ -17,6 +17,55 @@ template <typename T, int SYM>
 __device__ __forceinline__ void load_coef(const T* __restrict__ C, int64_t tot, int64_t m, T* c) {
 #pragma unroll
     for (int q = 0; q < NCoef<SYM>::value; q++) c[q] = C[q * tot + m];
+}
+
+// ---------------------------------------------------------------------------
+// psi.  The operator fills its whole psi field with 1 (Elastic.cpp:49-52) and SetPsi copies the
+// domain's cells only (:122-133), so a cell outside the non-periodic domain reads 1.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ const T* psi_field(const Geom& g);
+template <> __device__ __forceinline__ const double* psi_field<double>(const Geom& g) { return g.psi; }
+template <> __device__ __forceinline__ const float* psi_field<float>(const Geom& g) { return g.psif; }
+
+template <typename T>
+__device__ __forceinline__ T psi_cell(const T* __restrict__ ps, const Geom& g, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) return T(1);
+    return ps[i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k)];
+}
+// Interpolate::CellToNodeAverage (Numeric/Stencil.H:1385-1406): mean of the node's 8 cells
+template <typename T>
+__device__ __forceinline__ T psi_node_mean(const T* __restrict__ ps, const Geom& g, int i, int j, int k) {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < 8; c++) s += psi_cell(ps, g, i - (c & 1), j - ((c >> 1) & 1), k - (c >> 2));
+    return s * T(0.125);
+}
+// psi_avg = (1 - psi_small) CellToNodeAverage(psi) + psi_small (Elastic.cpp:191-192); 1 unset
+template <typename T>
+__device__ __forceinline__ T psi_avg(const Geom& g, int i, int j, int k) {
+    const T* ps = psi_field<T>(g);
+    if (!ps) return T(1);
+    return (T(1) - T(g.psi_small)) * psi_node_mean(ps, g, i, j, k) + T(g.psi_small);
+}
+// Numeric::CellGradientOnNode (Numeric/Stencil.H:709-712)
+template <typename T>
+__device__ __forceinline__ void psi_grad(const T* __restrict__ ps, const Geom& g, int i, int j, int k, T gp[3]) {
+    const int idx[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int a = (d + 1) % 3, b = (d + 2) % 3;
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int p[3], m[3];
+            p[d] = idx[d];
+            m[d] = idx[d] - 1;
+            p[a] = m[a] = idx[a] - (q & 1);
+            p[b] = m[b] = idx[b] - (q >> 1);
+            s += psi_cell(ps, g, p[0], p[1], p[2]) - psi_cell(ps, g, m[0], m[1], m[2]);
+        }
+        gp[d] = T(0.25) * s / T(g.dx[d]);
+    }
 }
 
 // First derivative with Numeric::GetStencil semantics (Numeric/Stencil.H:41-60, 70-191).
@@ -62,6 +111,14 @@ __device__ __forceinline__ void apply_boundary(const Geom& g, const BCTable& bc,
     }
     load_coef<T, SYM>(C, g.tot, m, c);
     c_dot<SYM>(c, G, S);
+    if (psi_field<T>(g)) {
+        // sig = (DDW grad u) psi_avg (Elastic.cpp:195)
+        const T pa = psi_avg<T>(g, i, j, k);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) S[a][b] *= pa;
+    }
     bc_row(bc, classify(i, j, k, hi), uu, G, S, f);
 }
 
@@ -82,6 +139,7 @@ __device__ __forceinline__ void diag_boundary(const Geom& g, const BCTable& bc, 
         cd[d] = (T(mx ? 1 : 0) - T(mn ? 1 : 0)) / (T(mn || mx ? 1 : 2) * T(g.dx[d]));
     }
     const int face = classify(i, j, k, hi);
+    const T pa = psi_avg<T>(g, i, j, k);  // Elastic.cpp:340-355
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         T G[3][3], S[3][3], uu[3] = {T(0), T(0), T(0)}, f[3];
@@ -90,6 +148,10 @@ __device__ __forceinline__ void diag_boundary(const Geom& g, const BCTable& bc, 
 #pragma unroll
             for (int d = 0; d < 3; d++) G[a][d] = (a == p) ? cd[d] : T(0);
         c_dot<SYM>(c, G, S);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) S[a][b] *= pa;
         uu[p] = T(1);
         bc_row(bc, face, uu, G, S, f);
         D[p] = f[p];
@@ -254,6 +316,36 @@ __device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __rest
         mm_slot_terms<I, J, T>(c, dc, gg, G, f, D, w);
     };
     for_slots<0, 0>(slot);
+}
+
+// psi terms of an interior row (Elastic.cpp:243-260): f = psi_avg f + (C : grad u) . grad psi (1 - s),
+// applied to the row f of apply_interior; D (may be null) is scaled by psi_avg (:363)
+template <typename T, int SYM>
+__device__ __forceinline__ void psi_interior(const Geom& g, const T* __restrict__ C, const T* __restrict__ u, int i,
+                                             int j, int k, T f[3], T* D) {
+    const T* ps = psi_field<T>(g);
+    if (!ps) return;
+    const T pa = psi_avg<T>(g, i, j, k);
+    if (D) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) D[p] *= pa;
+    }
+    if (!u) return;
+    const int64_t m = i + g.sy * j + g.sz * k;
+    const int64_t st[3] = {1, g.sy, g.sz};
+    T G[3][3], S[3][3], c[NCoef<SYM>::value], gp[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* up = u + p * g.tot;
+#pragma unroll
+        for (int d = 0; d < 3; d++) G[p][d] = (up[m + st[d]] - up[m - st[d]]) * T(0.5) / T(g.dx[d]);
+    }
+    load_coef<T, SYM>(C, g.tot, m, c);
+    c_dot<SYM>(c, G, S);
+    psi_grad(ps, g, i, j, k, gp);
+    const T w = T(1) - T(g.psi_small);
+#pragma unroll
+    for (int a = 0; a < 3; a++) f[a] = f[a] * pa + w * (S[a][0] * gp[0] + S[a][1] * gp[1] + S[a][2] * gp[2]);
 }
 
 }  // namespace alamo

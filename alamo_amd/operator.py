@@ -146,6 +146,17 @@ class Elastic:
         check(lib().alamo_mg_set_model_grains_host(self.h, gid.ctypes.data_as(ctypes.c_void_p),
                                                    table.ctypes.data_as(ctypes.c_void_p), table.shape[0]))
 
+    def SetPsi(self, psi, psi_small=1e-8, amrlev=0):
+        """Elastic::SetPsi(amrlev, psi, psi_small) (Elastic.H:55-57, Elastic.cpp:118-137): the cell
+        field psi (prod(n_cell) values, i fastest), numpy (host) or tensor (device)."""
+        if isinstance(psi, np.ndarray):
+            psi = np.ascontiguousarray(psi, dtype=np.float64).ravel()
+            if psi.size != int(np.prod(self.ncell)):
+                raise _lib.AlamoError(1, "psi needs one value per cell")
+            check(lib().alamo_mg_set_psi_host(self.h, amrlev, psi.ctypes.data_as(ctypes.c_void_p), float(psi_small)))
+        else:
+            check(lib().alamo_mg_set_psi(self.h, amrlev, _ptr(psi), float(psi_small)))
+
     def prepareForSolve(self):
         check(lib().alamo_mg_prepare(self.h))
 

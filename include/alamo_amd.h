@@ -109,6 +109,14 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
 /* Operator<Grid::Node>::prepareForSolve (Operator.cpp:253-260): coefficient restriction
  * to every mglev (Elastic::averageDownCoeffs, Elastic.cpp:605-626, 746-836) and the
  * diagonal (Operator::Diagonal(true)). */
+/* Elastic::SetPsi(amrlev, psi, psi_small) (src/Operator/Elastic.H:55-57, Elastic.cpp:118-137): the
+ * cell field psi (n_cell[0] n_cell[1] n_cell[2] doubles, i fastest; compact also on pitched
+ * handles).  From then on every row carries the psi terms of Elastic.cpp:191-195,243-260 and the
+ * diagonal its psi_avg (:340-363); prepare restricts psi to the coarse levels (:839-907) and
+ * alamo_newton_rhs uses the psi variant of Newton.H:234-266.  psi_small in [0, 1) (the
+ * reference's default is 1e-8).  Device pointer; _host copies from host memory. */
+int alamo_mg_set_psi(alamo_mg* h, int amrlev, const double* psi, double psi_small);
+int alamo_mg_set_psi_host(alamo_mg* h, int amrlev, const double* psi, double psi_small);
 int alamo_mg_prepare(alamo_mg* h);
 
 /* Copy the stiffness of level mglev out (SoA planes, device pointer). */

@@ -83,6 +83,11 @@ public:
         ensure();
         alamo::check(alamo_mg_set_model(h_, amrlev, C, layout));
     }
+    // Elastic::SetPsi (Elastic.H:55-57): psi a device cell field, n_cell product values
+    void SetPsi(int amrlev, const double* psi, double psi_small = 1e-8) {
+        ensure();
+        alamo::check(alamo_mg_set_psi(h_, amrlev, psi, psi_small));
+    }
     void prepareForSolve() { ensure(); alamo::check(alamo_mg_prepare(h_)); }
     // Operator<Grid::Node>::Apply -> Elastic::Fapply (Operator.H:55, Elastic.cpp:139-292)
     void Apply(int amrlev, int mglev, double* out, const double* in) const {
