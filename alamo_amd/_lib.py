@@ -20,7 +20,7 @@ class MGDesc(ctypes.Structure):
                 ("max_coarsening_level", c_int), ("precision", c_int), ("cycle", c_int),
                 ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
                 ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
-                ("bottom_tol_rel", c_double), ("verbose", c_int), ("row_align", c_int)]
+                ("bottom_tol_rel", c_double), ("verbose", c_int), ("row_align", c_int), ("fixed_iter", c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -62,6 +62,7 @@ SIGNATURES = {
     "alamo_mg_num_levels": (c_int, [c_void_p, P(c_int)]),
     "alamo_mg_level_ncell": (c_int, [c_void_p, c_int, P(c_int)]),
     "alamo_mg_level_layout": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "alamo_mg_update_opts": (c_int, [c_void_p, P(MGDesc)]),
     "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_psi_host": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_model": (c_int, [c_void_p, c_int, c_void_p, c_int]),

@@ -427,7 +427,9 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
     double rn = r0;
     int it = 0;
     if (h->d.verbose) fprintf(stderr, "alamo: iter 0 |r|inf %.6e target %.3e\n", r0, target);
-    while (rn > target && it < h->d.max_iter && std::isfinite(rn)) {
+    // solver.fixed_iter: exactly that many iterations, no early exit and no convergence failure
+    const int fixed = h->d.fixed_iter > 0 ? h->d.fixed_iter : 0;
+    while ((fixed ? it < fixed : (rn > target && it < h->d.max_iter)) && std::isfinite(rn)) {
         if (mixed) {
             k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(res0(), h->L[0].resf, n);
             one_cycle<float>(h);
@@ -457,7 +459,7 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
         st->fallback_events = 0;
     }
     if (!std::isfinite(rn)) FAIL(ALAMO_ERR_NAN, "alamo_linear_solve: residual is not finite (diverged)");
-    if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_linear_solve: max_iter reached before the tolerance");
+    if (!conv && !fixed) FAIL(ALAMO_ERR_NOCONV, "alamo_linear_solve: max_iter reached before the tolerance");
     return ALAMO_OK;
 }
 
@@ -789,6 +791,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->bottom_tol_rel = 1e-4;
     d->verbose = 0;
     d->row_align = 0;
+    d->fixed_iter = 0;
     return ALAMO_OK;
 }
 
@@ -806,6 +809,7 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
             if (t == ALAMO_BC_PERIODIC) FAIL(ALAMO_ERR_ARG, "periodic boundaries are not supported by this handle");
         }
     if (d->sweeps_per_smooth < 1 || d->pre_smooth < 0 || d->post_smooth < 0) FAIL(ALAMO_ERR_ARG, "bad smoothing");
+    if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -896,6 +900,32 @@ int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3) {
     if (!h || !ncell3) FAIL(ALAMO_ERR_ARG, "null argument");
     if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
     for (int d = 0; d < 3; d++) ncell3[d] = h->L[mglev].g.n[d];
+    return ALAMO_OK;
+}
+
+int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
+    if (!h || !d) FAIL(ALAMO_ERR_ARG, "null argument");
+    const alamo_mg_desc& c = h->d;
+    bool same = d->sym == c.sym && d->precision == c.precision && d->max_coarsening_level == c.max_coarsening_level &&
+                d->row_align == c.row_align;
+    for (int q = 0; q < 3; q++) same = same && d->n_cell[q] == c.n_cell[q] && d->dx[q] == c.dx[q];
+    for (int f = 0; f < 26; f++)
+        for (int q = 0; q < 3; q++)
+            same = same && d->bc_type[f][q] == c.bc_type[f][q] && d->bc_val[f][q] == c.bc_val[f][q];
+    if (!same) FAIL(ALAMO_ERR_ARG, "alamo_mg_update_opts: geometry, boundary, symmetry or precision changed");
+    if (d->sweeps_per_smooth < 1 || d->pre_smooth < 0 || d->post_smooth < 0) FAIL(ALAMO_ERR_ARG, "bad smoothing");
+    if (d->cycle != ALAMO_CYCLE_V && d->cycle != ALAMO_CYCLE_F) FAIL(ALAMO_ERR_ARG, "unknown cycle");
+    if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
+    h->d.cycle = d->cycle;
+    h->d.pre_smooth = d->pre_smooth;
+    h->d.post_smooth = d->post_smooth;
+    h->d.sweeps_per_smooth = d->sweeps_per_smooth;
+    h->d.omega = d->omega;
+    h->d.max_iter = d->max_iter;
+    h->d.bottom_max_iter = d->bottom_max_iter;
+    h->d.bottom_tol_rel = d->bottom_tol_rel;
+    h->d.fixed_iter = d->fixed_iter;
+    h->d.verbose = d->verbose;
     return ALAMO_OK;
 }
 

@@ -37,7 +37,7 @@ class Elastic:
 
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
-                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0):
+                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0, fixed_iter=0):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
         for q in range(3):
@@ -57,6 +57,7 @@ class Elastic:
         d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
         d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
         d.row_align = int(row_align)
+        d.fixed_iter = int(fixed_iter)
         self.desc = d
         self.sym = sym
         self.ncell = tuple(int(n) for n in ncell)
@@ -88,6 +89,21 @@ class Elastic:
     def LevelNodes(self, mglev):
         n = self.LevelNCell(mglev)
         return (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
+
+    def UpdateOpts(self, **kw):
+        """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
+        sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose)."""
+        allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
+                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose"}
+        for k, v in kw.items():
+            if k not in allowed:
+                raise ValueError("not a solver option: " + k)
+            setattr(self.desc, k, v)
+        check(lib().alamo_mg_update_opts(self.h, ctypes.byref(self.desc)))
+
+    def SetOmega(self, omega):
+        """Operator::SetOmega (Operator.H:57)"""
+        self.UpdateOpts(omega=float(omega))
 
     def LevelLayout(self, mglev=0):
         """(row pitch, plane pitch, elements per component) of a device field on level mglev."""
@@ -224,6 +240,41 @@ class Linear:
         if self.linop is None:
             raise _lib.AlamoError(3, "Solver cannot be cleared if not defined")
         self.linop = None
+
+    # Linear::setMaxIter / setBottomMaxIter / setFixedIter / setVerbose / setPreSmooth /
+    # setPostSmooth (Linear.H:178-184)
+    def _opt(self, **kw):
+        if self.linop is None:
+            raise _lib.AlamoError(3, "Solver not defined")
+        self.linop.UpdateOpts(**kw)
+
+    def setMaxIter(self, n):
+        self._opt(max_iter=int(n))
+
+    def setBottomMaxIter(self, n):
+        self._opt(bottom_max_iter=int(n))
+
+    def setFixedIter(self, n):
+        self._opt(fixed_iter=int(n))
+
+    def setVerbose(self, v):
+        self._opt(verbose=int(v))
+
+    def setPreSmooth(self, n):
+        self._opt(pre_smooth=int(n))
+
+    def setPostSmooth(self, n):
+        self._opt(post_smooth=int(n))
+
+    def apply(self, rhs, sol):
+        """Linear::apply (Linear.H:169-174): rhs = A sol on the finest level"""
+        self.linop.Fapply(0, 0, rhs, sol)
+
+    def solveaffine(self, sol, rhs, tol_rel, tol_abs, raise_on_fail=True):
+        """Linear::solveaffine (Linear.H:53-130): the solve of rhs minus A 0 evaluated with
+        inhomogeneous boundaries.  Elastic's homogeneity flag is stored but never read
+        (Elastic.H:48,172), so A 0 = 0 on every node and this is the plain solve."""
+        return self.solve(sol, rhs, tol_rel, tol_abs, raise_on_fail)
 
     def solve(self, sol, rhs, tol_rel, tol_abs, raise_on_fail=True):
         """Returns the stats dict; sol holds the initial guess on entry (Linear.H:131-147)."""

@@ -67,6 +67,9 @@ typedef struct {
                                   (alamo_mg_level_layout), with zeros in the pad entries.  Host
                                   arrays (the *_host entry points) are always compact.  No
                                   reference counterpart (an amrex::FArrayBox is compact). */
+    int fixed_iter;            /* solver.fixed_iter (Linear.H:181,267): > 0 runs exactly that many
+                                  multigrid iterations and returns without a convergence failure
+                                  (amrex::MLMG::setFixedIter); 0 off */
 } alamo_mg_desc;
 
 typedef struct {
@@ -109,6 +112,13 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
 /* Operator<Grid::Node>::prepareForSolve (Operator.cpp:253-260): coefficient restriction
  * to every mglev (Elastic::averageDownCoeffs, Elastic.cpp:605-626, 746-836) and the
  * diagonal (Operator::Diagonal(true)). */
+/* Linear::setMaxIter / setBottomMaxIter / setFixedIter / setVerbose / setPreSmooth /
+ * setPostSmooth and Operator::SetOmega (Linear.H:178-184, Operator.H:57) on a built handle: copies
+ * cycle, pre_smooth, post_smooth, sweeps_per_smooth, omega, max_iter, bottom_max_iter,
+ * bottom_tol_rel, fixed_iter and verbose from d; the fields that shape the hierarchy (n_cell, dx,
+ * sym, bc, precision, max_coarsening_level, row_align) must be unchanged. */
+int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d);
+
 /* Elastic::SetPsi(amrlev, psi, psi_small) (src/Operator/Elastic.H:55-57, Elastic.cpp:118-137): the
  * cell field psi (n_cell[0] n_cell[1] n_cell[2] doubles, i fastest; compact also on pitched
  * handles).  From then on every row carries the psi terms of Elastic.cpp:191-195,243-260 and the

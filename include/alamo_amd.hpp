@@ -76,7 +76,7 @@ public:
         for (int f = 0; f < 26; f++)
             for (int d = 0; d < 3; d++) { desc_.bc_type[f][d] = bc->type[f][d]; desc_.bc_val[f][d] = bc->val[f][d]; }
     }
-    void SetOmega(double omega) { desc_.omega = omega; }                  // Operator.H:57
+    void SetOmega(double omega) { desc_.omega = omega; UpdateOpts(); }    // Operator.H:57
     void SetPrecision(int p) { desc_.precision = p; }
     // Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116); C: device, AoS Matrix4 records
     void SetModel(int amrlev, const double* C, int layout = ALAMO_LAYOUT_AOS) {
@@ -100,6 +100,10 @@ public:
         alamo::check(alamo_mg_diagonal(h_, amrlev, mglev, diag, nullptr));
     }
     int NMGLevels() const { int n = 0; alamo::check(alamo_mg_num_levels(h_, &n)); return n; }
+    // push changed solver options of desc() to a built handle (before the build they are read at creation)
+    void UpdateOpts() {
+        if (h_) alamo::check(alamo_mg_update_opts(h_, &desc_));
+    }
     alamo_mg* handle() { ensure(); return h_; }
     alamo_mg_desc& desc() { return desc_; }
 
@@ -129,9 +133,22 @@ public:
         if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver cannot be cleared if not defined");
         linop_ = nullptr;
     }
-    void setMaxIter(int n) { linop_->desc().max_iter = n; }
-    void setPreSmooth(int n) { linop_->desc().pre_smooth = n; }
-    void setPostSmooth(int n) { linop_->desc().post_smooth = n; }
+    // Linear::setMaxIter / ... (Linear.H:178-184): take effect on the operator's built handle too
+    void setMaxIter(int n) { opt().max_iter = n; push(); }
+    void setBottomMaxIter(int n) { opt().bottom_max_iter = n; push(); }
+    void setFixedIter(int n) { opt().fixed_iter = n; push(); }
+    void setVerbose(int v) { opt().verbose = v; push(); }
+    void setPreSmooth(int n) { opt().pre_smooth = n; push(); }
+    void setPostSmooth(int n) { opt().post_smooth = n; push(); }
+    void setBottomTolerance(double t) { opt().bottom_tol_rel = t; push(); }
+    // Linear::apply (Linear.H:169-174): rhs = A sol on the finest level
+    void apply(double* rhs, const double* sol) { defined().Apply(0, 0, rhs, sol); }
+    // Linear::solveaffine (Linear.H:53-130) subtracts A 0 under inhomogeneous boundaries first;
+    // Elastic stores its homogeneity flag without reading it (Elastic.H:48,172), so A 0 = 0 and
+    // this is solve
+    double solveaffine(double* sol, const double* rhs, double tol_rel, double tol_abs) {
+        return solve(sol, rhs, tol_rel, tol_abs);
+    }
     // Linear::solve (Linear.H:131-147): sol (initial guess in, solution out), rhs; returns |r|_inf
     double solve(double* sol, const double* rhs, double tol_rel, double tol_abs) {
         if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver not defined");
@@ -143,6 +160,12 @@ public:
     const alamo_solve_stats& stats() const { return last_; }
 
 private:
+    ::Operator::Elastic<SYM>& defined() {
+        if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver not defined");
+        return *linop_;
+    }
+    alamo_mg_desc& opt() { return defined().desc(); }
+    void push() { linop_->UpdateOpts(); }
     ::Operator::Elastic<SYM>* linop_ = nullptr;
     alamo_solve_stats last_{};
 };

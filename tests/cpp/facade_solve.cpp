@@ -34,6 +34,13 @@ int main(int argc, char** argv) {
         alamo::check(alamo_bc_init(op.handle(), db, 0));  // BC::Operator::Elastic::Constant::Init
         Solver::Nonlocal::Linear<Operator::Isotropic> solver;
         solver.Define(op);
+        // options set after the operator is built take effect (Linear::setFixedIter, setMaxIter)
+        solver.setFixedIter(2);
+        solver.solve(dx, db, 1e-12, 0.0);
+        printf("fixed iterations %d\n", solver.stats().iterations);
+        solver.setFixedIter(0);
+        solver.setMaxIter(300);
+        hipMemset(dx, 0, x.size() * 8);
         double r = solver.solve(dx, db, 1e-12, 0.0);
         printf("iterations %d res_inf %.3e converged %d\n", solver.stats().iterations, r, solver.stats().converged);
         solver.Clear();

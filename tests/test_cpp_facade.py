@@ -35,6 +35,7 @@ def test_facade_solve_matches_oracle(cuda, ora, tmp_path):
     res = subprocess.run([EXE, str(out), str(outk)], capture_output=True, text=True, timeout=120)
     print(res.stdout, res.stderr)
     assert res.returncode == 0 and res.stdout.count("converged 1") == 2 and "net params" in res.stdout
+    assert "fixed iterations 2" in res.stdout
     x = np.fromfile(out, dtype=np.float64)
     xk = np.fromfile(outk, dtype=np.float64)
     n = 16

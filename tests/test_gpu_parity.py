@@ -246,3 +246,29 @@ def test_solid_linear_cubic_golden(cuda, ora):
     assert st["converged"] == 1
     trac = ora.trac_xhi(ncell, dx, P.MAJORMINOR, C, None, to_host(sol))
     assert abs(trac[0] - 0.0168293) / 0.0168293 < 1e-4
+
+
+def test_fixed_iter_and_update_opts(cuda, ora):
+    """Linear::setFixedIter runs exactly that many cycles without a failure; options changed on a
+    built operator take effect (Linear.H:178-184)."""
+    pr, bc, rhs = eshelby_rhs(ora, 16)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, max_iter=100)
+    op.SetModel(pr["C"])
+    lin = Linear(op)
+    lin.setFixedIter(3)
+    sol = to_dev(np.zeros_like(rhs), cuda)
+    st = lin.solve(sol, to_dev(rhs, cuda), 1e-14, 0.0)
+    assert st["rc"] == 0 and st["iterations"] == 3
+    lin.setFixedIter(0)
+    lin.setMaxIter(2)
+    sol.zero_()
+    st = lin.solve(sol, to_dev(rhs, cuda), 1e-12, 0.0, raise_on_fail=False)
+    assert st["iterations"] == 2 and st["converged"] == 0 and st["rc"] != 0
+    lin.setMaxIter(200)
+    sol.zero_()
+    st = lin.solve(sol, to_dev(rhs, cuda), 1e-12, 0.0)
+    assert st["converged"] == 1
+    ref = to_dev(np.zeros_like(rhs), cuda)
+    lin.apply(ref, sol)
+    assert np.abs(to_host(ref) - rhs).max() <= 1e-11 * np.abs(rhs).max()
+
