@@ -63,6 +63,10 @@ SIGNATURES = {
     "alamo_mg_level_ncell": (c_int, [c_void_p, c_int, P(c_int)]),
     "alamo_mg_level_layout": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_int64)]),
     "alamo_mg_update_opts": (c_int, [c_void_p, P(MGDesc)]),
+    "alamo_mg_strain": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "alamo_mg_stress": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "alamo_mg_energy": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_blend_models": (c_int, [ctypes.c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_psi_host": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_model": (c_int, [c_void_p, c_int, c_void_p, c_int]),

@@ -23,6 +23,7 @@
 #include "kernels_iso_rows.h"
 #include "krylov.h"
 #include "amr.h"
+#include "post.h"
 
 using namespace alamo;
 
@@ -765,6 +766,26 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
 // ===========================================================================
 // C ABI
 // ===========================================================================
+template <int KIND>
+static void launch_post(int sym, const Geom& g, const double* C, const double* u, double* out, int voigt,
+                        hipStream_t s) {
+    const unsigned nb = nblocks(g.tot);
+    if (sym == ISO) k_post<ISO, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
+    else if (sym == MAJORMINOR) k_post<MAJORMINOR, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
+    else k_post<DIAGONAL, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
+}
+
+static int post_impl(alamo_mg* h, int amrlev, const double* u, double* out, int voigt, int kind, void* stream) {
+    if (!h || !u || !out) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
+    if (kind != POST_STRAIN && !h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    LevelBuf& l0 = h->L[0];
+    if (kind == POST_STRAIN) launch_post<POST_STRAIN>(h->sym, l0.g, l0.C, u, out, voigt, (hipStream_t)stream);
+    else if (kind == POST_STRESS) launch_post<POST_STRESS>(h->sym, l0.g, l0.C, u, out, voigt, (hipStream_t)stream);
+    else launch_post<POST_ENERGY>(h->sym, l0.g, l0.C, u, out, 0, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
 extern "C" {
 
 const char* alamo_last_error(void) { return g_err.c_str(); }
@@ -1023,6 +1044,33 @@ int alamo_mg_set_psi(alamo_mg* h, int amrlev, const double* psi, double psi_smal
 }
 int alamo_mg_set_psi_host(alamo_mg* h, int amrlev, const double* psi, double psi_small) {
     return set_psi_impl(h, amrlev, psi, psi_small, true);
+}
+
+int alamo_mg_strain(alamo_mg* h, int amrlev, const double* u, double* eps, int voigt, void* stream) {
+    return post_impl(h, amrlev, u, eps, voigt, POST_STRAIN, stream);
+}
+int alamo_mg_stress(alamo_mg* h, int amrlev, const double* u, double* sigma, int voigt, void* stream) {
+    return post_impl(h, amrlev, u, sigma, voigt, POST_STRESS, stream);
+}
+int alamo_mg_energy(alamo_mg* h, int amrlev, const double* u, double* energy, void* stream) {
+    return post_impl(h, amrlev, u, energy, 0, POST_ENERGY, stream);
+}
+
+int alamo_blend_models(int64_t nnodes, int nmodels, int nrec, const double* eta, const double* table, double* out,
+                       int layout, void* stream) {
+    if (!eta || !table || !out || nnodes < 1 || nmodels < 1 || nrec < 1) FAIL(ALAMO_ERR_ARG, "bad argument");
+    if (layout != ALAMO_LAYOUT_AOS && layout != ALAMO_LAYOUT_SOA) FAIL(ALAMO_ERR_ARG, "unknown layout");
+    hipStream_t s = (hipStream_t)stream;
+    double* dt = nullptr;
+    const size_t tb = (size_t)nmodels * nrec * sizeof(double);
+    HIPCHK(hipMallocAsync((void**)&dt, tb, s));
+    HIPCHK(hipMemcpyAsync(dt, table, tb, hipMemcpyHostToDevice, s));
+    k_blend<<<gs_blocks(nnodes), 256, 0, s>>>(nnodes, nmodels, nrec, eta, dt, out, layout == ALAMO_LAYOUT_AOS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(dt, s));
+    // the host table may be reused by the caller as soon as this returns
+    HIPCHK(hipStreamSynchronize(s));
+    return ALAMO_OK;
 }
 
 int alamo_mg_prepare(alamo_mg* h) {

@@ -213,6 +213,19 @@ class Elastic:
         check(lib().alamo_mg_norms(self.h, mglev, _ptr(x), ctypes.byref(a), ctypes.byref(b), stream))
         return a.value, b.value
 
+    # --- post-processing (Elastic.cpp:430-603) -----------------------------------
+    def Strain(self, amrlev, eps, u, voigt=False, stream=None):
+        """eps = sym(grad u): 9 planes row-major, or 6 Voigt planes 00, 11, 22, 12, 20, 01"""
+        check(lib().alamo_mg_strain(self.h, amrlev, _ptr(u), _ptr(eps), int(voigt), stream))
+
+    def Stress(self, amrlev, sigma, u, voigt=False, stream=None):
+        """sigma = (C : grad u) psi_avg"""
+        check(lib().alamo_mg_stress(self.h, amrlev, _ptr(u), _ptr(sigma), int(voigt), stream))
+
+    def Energy(self, amrlev, energy, u, stream=None):
+        """energy += (C : grad u) : sym(grad u) / 2"""
+        check(lib().alamo_mg_energy(self.h, amrlev, _ptr(u), _ptr(energy), stream))
+
     # --- right-hand sides ------------------------------------------------------
     def BCInit(self, rhs, homogeneous=False):
         """BC::Operator::Elastic::Constant::Init (Constant.H:197-286)."""
@@ -287,6 +300,16 @@ class Linear:
         out = st.as_dict()
         out["rc"] = rc
         return out
+
+
+def blend_models(eta, table, out, layout=LAYOUT_AOS, stream=None):
+    """Integrator::Mechanics::UpdateModel blending (Mechanics.H:242-254): out = sum_n eta_n table_n.
+    eta: device tensor of nmodels planes; table: numpy (nmodels, nrec); out: device tensor."""
+    table = np.ascontiguousarray(table, dtype=np.float64)
+    nmodels, nrec = table.shape
+    nnodes = eta.numel() // nmodels
+    check(lib().alamo_blend_models(ctypes.c_int64(nnodes), nmodels, nrec, _ptr(eta),
+                                   table.ctypes.data_as(ctypes.c_void_p), _ptr(out), layout, stream))
 
 
 def precond_apply(op, kind, r, z):

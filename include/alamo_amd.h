@@ -112,6 +112,24 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
 /* Operator<Grid::Node>::prepareForSolve (Operator.cpp:253-260): coefficient restriction
  * to every mglev (Elastic::averageDownCoeffs, Elastic.cpp:605-626, 746-836) and the
  * diagonal (Operator::Diagonal(true)). */
+/* Post-processing on the finest level (device fields, work queued on `stream`):
+ * Elastic::Strain (src/Operator/Elastic.cpp:430-484): eps = (grad u + grad u^T)/2;
+ * Elastic::Stress (:487-550): sigma = (C : grad u) psi_avg;
+ * Elastic::Energy (:552-603): energy += (C : grad u) : eps / 2 (accumulates, as the reference).
+ * GetStencil gradients (one-sided on the domain's faces).  Tensors: 9 planes row-major, or with
+ * voigt = 1 the 6 planes 00, 11, 22, 12, 20, 01. */
+int alamo_mg_strain(alamo_mg* h, int amrlev, const double* u, double* eps, int voigt, void* stream);
+int alamo_mg_stress(alamo_mg* h, int amrlev, const double* u, double* sigma, int voigt, void* stream);
+int alamo_mg_energy(alamo_mg* h, int amrlev, const double* u, double* energy, void* stream);
+
+/* Integrator::Mechanics::UpdateModel blending (src/Integrator/Mechanics.H:242-254): per node
+ * out = sum_n eta_n table_n over nmodels models of nrec values (a stiffness record: 2, 21 or 9;
+ * an affine model's F0: 9).  eta: nmodels device planes of nnodes values; table: host,
+ * nmodels x nrec; out: device, ALAMO_LAYOUT_AOS (nrec per node: SetModel's and
+ * alamo_newton_rhs's F0 records) or ALAMO_LAYOUT_SOA (nrec planes). */
+int alamo_blend_models(int64_t nnodes, int nmodels, int nrec, const double* eta, const double* table, double* out,
+                       int layout, void* stream);
+
 /* Linear::setMaxIter / setBottomMaxIter / setFixedIter / setVerbose / setPreSmooth /
  * setPostSmooth and Operator::SetOmega (Linear.H:178-184, Operator.H:57) on a built handle: copies
  * cycle, pre_smooth, post_smooth, sweeps_per_smooth, omega, max_iter, bottom_max_iter,

@@ -99,6 +99,16 @@ public:
     void Diagonal(int amrlev, int mglev, double* diag) const {
         alamo::check(alamo_mg_diagonal(h_, amrlev, mglev, diag, nullptr));
     }
+    // Elastic::Strain / Stress / Energy (Elastic.H:71-89, Elastic.cpp:430-603); device fields
+    void Strain(int amrlev, double* eps, const double* u, bool voigt = false) const {
+        alamo::check(alamo_mg_strain(h_, amrlev, u, eps, voigt ? 1 : 0, nullptr));
+    }
+    void Stress(int amrlev, double* sigma, const double* u, bool voigt = false) const {
+        alamo::check(alamo_mg_stress(h_, amrlev, u, sigma, voigt ? 1 : 0, nullptr));
+    }
+    void Energy(int amrlev, double* energy, const double* u) const {
+        alamo::check(alamo_mg_energy(h_, amrlev, u, energy, nullptr));
+    }
     int NMGLevels() const { int n = 0; alamo::check(alamo_mg_num_levels(h_, &n)); return n; }
     // push changed solver options of desc() to a built handle (before the build they are read at creation)
     void UpdateOpts() {

@@ -815,6 +815,41 @@ static void newton_rhs(const Level& L, int sym, const double* C, const double* F
     if (DWout) std::copy(DW.begin(), DW.end(), DWout);
 }
 
+// ---------------------------------------------------------------------------
+// Elastic::Strain / Stress / Energy (Operator/Elastic.cpp:430-603) on every node with the
+// GetStencil gradient: kind 0 eps = sym(grad u), 1 sig = (C : grad u) psi_avg, 2 energy +=
+// (C : grad u) : eps / 2.  Tensors row-major (9 planes) or Voigt 00,11,22,12,20,01 (6 planes).
+// Pinned by tests/test_oracle_post.py (linear fields, closed forms).
+// ---------------------------------------------------------------------------
+static void post(const Level& L, int sym, const double* C, const double* u, int kind, int voigt, double* out) {
+    const int nc = ncoef(sym);
+    const int va[6] = {0, 1, 2, 1, 2, 0}, vb[6] = {0, 1, 2, 2, 0, 1};
+    for (int k = 0; k < L.N[2]; k++)
+        for (int j = 0; j < L.N[1]; j++)
+            for (int i = 0; i < L.N[0]; i++) {
+                const int64_t m = L.id(i, j, k);
+                double G[3][3], T[3][3], E[3][3];
+                grad_u(L, u, i, j, k, G);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) E[a][b] = 0.5 * (G[a][b] + G[b][a]);
+                if (kind == 0) {
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) T[a][b] = E[a][b];
+                } else {
+                    c_dot_mat(sym, C + m * nc, G, T);
+                    if (kind == 2) {
+                        double w = 0.0;
+                        for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) w += 0.5 * T[a][b] * E[a][b];
+                        out[m] += w;
+                        continue;
+                    }
+                    const double pa = psi_avg(L, i, j, k);
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) T[a][b] *= pa;
+                }
+                if (voigt) for (int q = 0; q < 6; q++) out[q * L.tot + m] = T[va[q]][vb[q]];
+                else for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) out[(3 * a + b) * L.tot + m] = T[a][b];
+            }
+}
+
 // BC::Operator::Elastic::Constant::Init (BC/Operator/Elastic/Constant.H:197-286),
 // non-periodic: boundary nodes of b take the prescribed values.
 static void bc_init(const Level& L, const int* bct, const double* bcv, int homogeneous, double* b) {
@@ -969,6 +1004,22 @@ int ora_mg_solve_psi(const int* ncell, const double* dx, int sym, const double* 
     build_hier(H, ncell, dx, sym, bct, C, -1, psi, psi_small);
     MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, 1e-4};
     return mg_solve(H, o, x, b, tol_rel, tol_abs, nullptr, 0, final_res);
+}
+void ora_post(const int* ncell, const double* dx, int sym, const double* C, const double* psi, double psi_small,
+              const double* u, int kind, int voigt, double* out) {
+    Level L(ncell, dx);
+    L.psi = psi;
+    L.psi_small = psi_small;
+    post(L, sym, C, u, kind, voigt, out);
+}
+// Integrator::Mechanics::UpdateModel blending (Integrator/Mechanics.H:242-254)
+void ora_blend(int64_t nnodes, int nmodels, int nrec, const double* eta, const double* table, double* out, int aos) {
+    for (int64_t v = 0; v < nnodes; v++)
+        for (int q = 0; q < nrec; q++) {
+            double s = 0.0;
+            for (int n = 0; n < nmodels; n++) s += eta[n * nnodes + v] * table[n * nrec + q];
+            out[aos ? v * nrec + q : q * nnodes + v] = s;
+        }
 }
 void ora_bc_init(const int* ncell, const int* bct, const double* bcv, int homogeneous, double* b) {
     double dx[3] = {1, 1, 1};

@@ -180,6 +180,29 @@ def mg_solve_psi(ncell, dx, sym, C, bct, psi, psi_small, b, tol_rel=1e-10, tol_a
     return x, it, fr.value
 
 
+def post(ncell, dx, sym, C, u, kind, voigt=False, psi=None, psi_small=1e-8, energy=None):
+    """kind 0 strain, 1 stress, 2 energy (accumulates into `energy`)"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); u = _c(u)
+    N = int(np.prod(nodes(ncell)))
+    if kind == 2:
+        out = np.zeros(N) if energy is None else _c(energy).copy()
+    else:
+        out = np.zeros((6 if voigt else 9) * N)
+    ps = None if psi is None else _c(psi)
+    lib().ora_post(_i(ncell), _d(dx), sym, _d(C), None if ps is None else _d(ps), ctypes.c_double(psi_small), _d(u),
+                   kind, int(voigt), _d(out))
+    return out
+
+
+def blend(eta, table, aos=True):
+    eta = _c(eta); table = _c(table)
+    nmodels, nrec = table.shape
+    nnodes = eta.size // nmodels
+    out = np.zeros(nnodes * nrec)
+    lib().ora_blend(ctypes.c_int64(nnodes), nmodels, nrec, _d(eta), _d(table), _d(out), int(aos))
+    return out
+
+
 def bc_init(ncell, bct, bcv, homogeneous=False, b=None):
     ncell = _c(ncell, np.int32); bct = _c(bct, np.int32); bcv = _c(bcv)
     N = int(np.prod(nodes(ncell)))
