@@ -66,6 +66,7 @@ SIGNATURES = {
     "alamo_mg_strain": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_stress": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_energy": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_trac_xhi": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),
     "alamo_blend_models": (c_int, [ctypes.c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),
     "alamo_mg_set_psi_host": (c_int, [c_void_p, c_int, c_void_p, c_double]),

@@ -1046,6 +1046,23 @@ int alamo_mg_set_psi_host(alamo_mg* h, int amrlev, const double* psi, double psi
     return set_psi_impl(h, amrlev, psi, psi_small, true);
 }
 
+int alamo_mg_trac_xhi(alamo_mg* h, const double* u, const double* F0, double* trac3) {
+    if (!h || !u || !trac3) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    TRY(krylov_scratch(h));
+    LevelBuf& l0 = h->L[0];
+    const int64_t ncell = (int64_t)l0.g.n[1] * l0.g.n[2];
+    const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (ncell + RED_BLOCK - 1) / RED_BLOCK);
+    if (h->sym == ISO) k_trac_xhi<ISO><<<nb, RED_BLOCK, 0, h->stream>>>(l0.g, l0.C, F0, u, h->kpart);
+    else if (h->sym == MAJORMINOR) k_trac_xhi<MAJORMINOR><<<nb, RED_BLOCK, 0, h->stream>>>(l0.g, l0.C, F0, u, h->kpart);
+    else k_trac_xhi<DIAGONAL><<<nb, RED_BLOCK, 0, h->stream>>>(l0.g, l0.C, F0, u, h->kpart);
+    k_sum_rows<<<3, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int a = 0; a < 3; a++) trac3[a] = h->red_host[a];
+    return ALAMO_OK;
+}
+
 int alamo_mg_strain(alamo_mg* h, int amrlev, const double* u, double* eps, int voigt, void* stream) {
     return post_impl(h, amrlev, u, eps, voigt, POST_STRAIN, stream);
 }

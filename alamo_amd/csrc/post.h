@@ -57,6 +57,48 @@ __global__ void __launch_bounds__(256) k_post(Geom g, const double* __restrict__
     }
 }
 
+// Integrator::Base::Mechanics::Integrate, trac_hi[0] (Integrator/Base/Mechanics.H:391-448, 3-D):
+// sum over the x-hi face's cells of the mean of sigma e_x at the cell's 4 nodes, times dy dz, with
+// sigma = C : (grad u - F0) (the model's DW; F0 may be null).  partial[c * gridDim.x + block] =
+// the block's share of component c.
+template <int SYM>
+__global__ void __launch_bounds__(RED_BLOCK) k_trac_xhi(Geom g, const double* __restrict__ C,
+                                                        const double* __restrict__ F0, const double* __restrict__ u,
+                                                        double* __restrict__ partial) {
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int64_t st[3] = {1, g.sy, g.sz};
+    const int64_t ncell = (int64_t)g.n[1] * g.n[2];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += (int64_t)gridDim.x * blockDim.x) {
+        const int j0 = (int)(q % g.n[1]), k0 = (int)(q / g.n[1]);
+        for (int c = 0; c < 4; c++) {
+            const int i = g.n[0], j = j0 + (c & 1), k = k0 + (c >> 1);
+            const int idx[3] = {i, j, k};
+            const int64_t m = i + g.sy * j + g.sz * k;
+            double G[3][3], S[3][3], cf[NCoef<SYM>::value];
+            for (int p = 0; p < 3; p++)
+                for (int d = 0; d < 3; d++) {
+                    G[p][d] = d1(u + p * g.tot, m, st[d], idx[d], hi[d], g.dx[d]);
+                    if (F0) G[p][d] -= F0[9 * m + 3 * p + d];
+                }
+            load_coef<double, SYM>(C, g.tot, m, cf);
+            c_dot<SYM>(cf, G, S);
+            for (int a = 0; a < 3; a++) acc[a] += 0.25 * S[a][0] * g.dx[1] * g.dx[2];
+        }
+    }
+    __shared__ double sh[3][RED_BLOCK / 64];
+    for (int a = 0; a < 3; a++) {
+        const double v = wave_sum(acc[a]);
+        if (threadIdx.x % 64 == 0) sh[a][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < RED_BLOCK / 64; w++) t += sh[threadIdx.x][w];
+        partial[threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // out[node] = sum_n eta[n][node] table[n][q] for nrec values per model; eta holds nmodels SoA
 // planes of nnodes values; out is AoS (nrec per node) or SoA (nrec planes)
 __global__ void __launch_bounds__(256) k_blend(int64_t nnodes, int nmodels, int nrec, const double* __restrict__ eta,

@@ -226,6 +226,13 @@ class Elastic:
         """energy += (C : grad u) : sym(grad u) / 2"""
         check(lib().alamo_mg_energy(self.h, amrlev, _ptr(u), _ptr(energy), stream))
 
+    def TractionXhi(self, u, F0=None):
+        """Base::Mechanics::Integrate trac_hi[0] (Mechanics.H:391-448): the x-hi face integral of
+        sigma e_x with sigma = C : (grad u - F0); returns a 3-vector"""
+        out = (ctypes.c_double * 3)()
+        check(lib().alamo_mg_trac_xhi(self.h, _ptr(u), _ptr(F0), out))
+        return np.array(list(out))
+
     # --- right-hand sides ------------------------------------------------------
     def BCInit(self, rhs, homogeneous=False):
         """BC::Operator::Elastic::Constant::Init (Constant.H:197-286)."""

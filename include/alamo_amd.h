@@ -122,6 +122,12 @@ int alamo_mg_strain(alamo_mg* h, int amrlev, const double* u, double* eps, int v
 int alamo_mg_stress(alamo_mg* h, int amrlev, const double* u, double* sigma, int voigt, void* stream);
 int alamo_mg_energy(alamo_mg* h, int amrlev, const double* u, double* energy, void* stream);
 
+/* Integrator::Base::Mechanics::Integrate, trac_hi[0] (src/Integrator/Base/Mechanics.H:391-448, the
+ * thermo output of tension tests): the x-hi face integral of sigma e_x, each face cell taking the mean
+ * of its 4 nodes' sigma = C : (grad u - F0) times dy dz.  u, F0 (9 AoS per node, or NULL) device;
+ * trac3 host; synchronous on the handle's stream. */
+int alamo_mg_trac_xhi(alamo_mg* h, const double* u, const double* F0, double* trac3);
+
 /* Integrator::Mechanics::UpdateModel blending (src/Integrator/Mechanics.H:242-254): per node
  * out = sum_n eta_n table_n over nmodels models of nrec values (a stiffness record: 2, 21 or 9;
  * an affine model's F0: 9).  eta: nmodels device planes of nnodes values; table: host,

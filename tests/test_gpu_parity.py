@@ -244,7 +244,8 @@ def test_solid_linear_cubic_golden(cuda, ora):
     sol = to_dev(np.zeros(3 * N), cuda)
     st = Linear(op).solve(sol, rhs, 1e-12, 0.0)
     assert st["converged"] == 1
-    trac = ora.trac_xhi(ncell, dx, P.MAJORMINOR, C, None, to_host(sol))
+    trac = op.TractionXhi(sol)  # the integral on the GPU (Base/Mechanics.H:391-448)
+    assert np.abs(trac - ora.trac_xhi(ncell, dx, P.MAJORMINOR, C, None, to_host(sol))).max() <= 1e-14
     assert abs(trac[0] - 0.0168293) / 0.0168293 < 1e-4
 
 

@@ -55,3 +55,20 @@ def test_blend_models(cuda, ora, layout):
     f0 = to_dev(np.zeros(nn * 9), cuda)
     blend_models(to_dev(eta.ravel(), cuda), F0, f0)
     close(to_host(f0), ora.blend(eta.ravel(), F0), 1e-15)
+
+
+@pytest.mark.parametrize("sym", [P.ISO, P.MAJORMINOR])
+def test_traction_xhi(cuda, ora, sym):
+    ncell = (14, 11, 9)
+    nodes = int(np.prod([n + 1 for n in ncell]))
+    dx = (0.1, 0.07, 0.12)
+    C = random_iso(nodes, 131) if sym == P.ISO else random_mm(nodes, 131)
+    op = Elastic(ncell, dx, sym=sym, bc=random_bc(132))
+    op.SetModel(C, LAYOUT_AOS)
+    u = rng(133).uniform(-1, 1, 3 * nodes)
+    F0 = rng(134).uniform(-0.1, 0.1, 9 * nodes)
+    for f0 in (None, F0):
+        t = op.TractionXhi(to_dev(u, cuda), None if f0 is None else to_dev(f0, cuda))
+        ref = ora.trac_xhi(ncell, dx, sym, C, f0, u)
+        assert np.abs(t - ref).max() <= 1e-12 * np.abs(ref).max()
+
