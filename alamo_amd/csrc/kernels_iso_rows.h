@@ -454,195 +454,6 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Variant with the planes staged by direct-to-LDS loads (global_load_lds_dword), DMA_DEPTH
-// planes ahead in a ring of DMA_DEPTH + 2 slots, so no registers hold prefetched data and
-// several planes per workgroup are in flight.  Each wavefront issues the same number DMA_L of
-// DMA loads per plane (FP32: one dword per value; FP64: two).  Before reading plane p it waits
-// until at most the loads issued after plane p's (DMA_L for each later plane already issued)
-// are outstanding: vector memory operations retire in order, so plane p's have landed; then
-// the barrier makes every wavefront's loads visible.
-// ---------------------------------------------------------------------------
-#ifndef ALAMO_DMA_RW
-#define ALAMO_DMA_RW 4
-#endif
-#ifndef ALAMO_DMA_DEPTH
-#define ALAMO_DMA_DEPTH 3
-#endif
-constexpr int DMA_RW = ALAMO_DMA_RW;
-constexpr int DMA_DEPTH = ALAMO_DMA_DEPTH;
-constexpr int DMA_SLOTS = DMA_DEPTH + 2;
-constexpr int DMA_NR = DMA_RW + 2;
-template <typename T>
-constexpr int dma_jobs() { return DMA_NR * 5 * (int)(sizeof(T) / 4); }  // dword row pieces per plane
-template <typename T>
-constexpr int dma_l() { return (dma_jobs<T>() + DMA_RW - 1) / DMA_RW; }  // per wavefront (padded)
-constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
-static_assert((DMA_DEPTH - 1) * dma_l<double>() <= 63, "vmcnt counts at most 63 operations");
-
-template <int L>
-__device__ __forceinline__ void dma_wait(int later) {
-    // wait until at most `later` planes' loads (L each) are still in flight
-    switch (later) {
-        case 0: __builtin_amdgcn_s_waitcnt(vmcnt_imm(0)); break;
-        case 1: __builtin_amdgcn_s_waitcnt(vmcnt_imm(L)); break;
-        case 2: __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * L < 63 ? 2 * L : 63)); break;
-        case 3: __builtin_amdgcn_s_waitcnt(vmcnt_imm(3 * L < 63 ? 3 * L : 63)); break;
-        default: __builtin_amdgcn_s_waitcnt(vmcnt_imm(4 * L < 63 ? 4 * L : 63)); break;
-    }
-}
-
-template <typename T, int MODE, int FOLD>
-__global__ void __launch_bounds__(64 * DMA_RW) k_iso_rows_dma(Geom g, const T* __restrict__ C,
-                                                              const T* __restrict__ u, const T* __restrict__ b,
-                                                              T* __restrict__ out, T omega) {
-    constexpr int NR = DMA_NR;
-    constexpr int A = NR * 64;
-    constexpr int WPV = sizeof(T) / 4;  // dwords per value
-    constexpr int JOBS = dma_jobs<T>();
-    constexpr int L = dma_l<T>();
-    static_assert(DMA_DEPTH - 1 <= 4, "dma_wait covers up to four later planes");
-    __shared__ T lds[DMA_SLOTS][5][A];
-    const int lane = threadIdx.x;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-    if (ALAMO_ROWS_XCD) {
-        const int gx = gridDim.x, gy = gridDim.y;
-        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
-        bx = t % gx;
-        by = (t / gx) % gy;
-        bz = t / (gx * gy);
-    }
-    const int x0 = 1 + ROWS_OUT * bx;
-    const int y0 = 1 + DMA_RW * by;
-    const int k0 = 1 + ROWS_KCH * bz;
-    const int k1 = min(k0 + ROWS_KCH, g.n[2]);
-    const int x = x0 - 1 + lane;
-    const int y = y0 + w;
-    const bool row_out = y <= g.n[1] - 1;
-    const bool out_i = row_out && lane >= 1 && lane <= ROWS_OUT && x <= g.n[0] - 1;
-    const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
-    constexpr uint32_t E = sizeof(T);
-    const uint32_t szb = (uint32_t)g.sz * E;
-    const uint32_t colb = ((uint32_t)min(x, g.n[0]) + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
-    const T* __restrict__ B3[3] = {b, b + g.tot, b + 2 * g.tot};
-    T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
-    const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
-    const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
-    const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
-    const int ln = lane < 63 ? lane + 1 : 63, lw = lane > 0 ? lane - 1 : 0;
-    const int rs = w * 64 + lane, rc = (w + 1) * 64 + lane, rn = (w + 2) * 64 + lane;
-    // the dword this lane moves in piece h of a row-array: value (64 h + lane) / WPV of the
-    // 64-value row, word (64 h + lane) % WPV of that value
-    uint32_t pofs[WPV];
-#pragma unroll
-    for (int h = 0; h < WPV; h++) {
-        const int d = (64 * h + lane) / WPV;
-        pofs[h] = (uint32_t)min(x0 - 1 + d, g.n[0]) * E + 4u * (uint32_t)((64 * h + lane) % WPV);
-    }
-    auto issue = [&](int p) {
-        const uint32_t pz = szb * (uint32_t)p;
-        const int sl = (p - k0 + 1) % DMA_SLOTS;
-#pragma unroll
-        for (int q = 0; q < L; q++) {
-            // job j = (row r, array v, piece h); padded jobs repeat early ones (same data, same place)
-            const int j = (w + DMA_RW * q) % JOBS;
-            const int h = j % WPV, v = (j / WPV) % 5, r = (j / WPV) / 5;
-            const uint32_t yr = (uint32_t)min(max(y0 - 1 + r, 0), g.n[1]);
-            const T* base = v < 3 ? u + (int64_t)v * g.tot : C + (int64_t)(v - 3) * g.tot;
-            const char* src = (const char*)base + (pofs[h] + (uint32_t)g.sy * yr * E + pz);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)((char*)&lds[sl][v][r * 64] + 256 * h),
-                                             4, 0, 0);
-        }
-    };
-    auto step = [&](int p, const RowPlane<T>& qm, const RowPlane<T>& qc, RowPlane<T>& qn) {
-        const int sl = (p - k0 + 1) % DMA_SLOTS;
-        dma_wait<L>(min(DMA_DEPTH - 1, k1 - p));
-        __builtin_amdgcn_s_barrier();
-        const uint32_t mcb = colb + szb * (uint32_t)(p - 1);
-        T bq[3] = {T(0), T(0), T(0)};
-        if (MODE != APPLY && p >= k0 + 1 && (out_i || out_f)) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) bq[v] = ldb(B3[v], mcb);
-        }
-        // plane p + DMA_DEPTH goes into the slot of plane p - 2, last read before this barrier
-        if (p + DMA_DEPTH <= k1) issue(p + DMA_DEPTH);
-        const T(*sp)[A] = lds[sl];
-#pragma unroll
-        for (int v = 0; v < 3; v++) qn.u[v] = sp[v][rc];
-        qn.lam = sp[3][rc];
-        qn.mu = sp[4][rc];
-        qn.dxux = (from_right(qn.u[0]) - from_left(qn.u[0])) * hx;
-        qn.dxuz = (from_right(qn.u[2]) - from_left(qn.u[2])) * hx;
-        qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
-        qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
-        if (p < k0 + 1) return;
-        const T(*sc)[A] = lds[sl == 0 ? DMA_SLOTS - 1 : sl - 1];
-        T ue[3], uw[3];
-#pragma unroll
-        for (int v = 0; v < 3; v++) {
-            ue[v] = from_right(qc.u[v]);
-            uw[v] = from_left(qc.u[v]);
-        }
-        const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
-        const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
-        if (out_i) {
-            T ux[3], uy[3], uz[3], lap[3];
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                const T c0 = qc.u[v], e = ue[v], wv = uw[v];
-                const T nn = sc[v][rn], ss = sc[v][rs];
-                ux[v] = (e - wv) * hx;
-                uy[v] = (nn - ss) * hy;
-                uz[v] = (qn.u[v] - qm.u[v]) * hz;
-                lap[v] = (e - T(2) * c0 + wv) * ix2 + (nn - T(2) * c0 + ss) * iy2 + (qn.u[v] - T(2) * c0 + qm.u[v]) * iz2;
-            }
-            const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * ix2;
-            const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * iy2;
-            const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * iz2;
-            const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] + sc[0][rs - lane + lw]) * hxy;
-            const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] + sc[1][rs - lane + lw]) * hxy;
-            const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
-            const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
-            const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
-            const T lam = qc.lam, mu = qc.mu;
-            const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
-            const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
-            const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
-            const T tr = G[0][0] + G[1][1] + G[2][2];
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                T f = mu * (lap[v] + gd[v]) + lam * gd[v];
-                f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
-                     dm[2] * (G[v][2] + G[2][v]);
-                if (MODE == APPLY) stb(O[v], mcb, f);
-                else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
-                else {
-                    const T w2[3] = {ix2, iy2, iz2};
-                    const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
-                    stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
-                }
-            }
-        } else if (out_f) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
-                else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
-                else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
-            }
-        }
-    };
-    for (int d = 0; d < DMA_DEPTH; d++)
-        if (k0 - 1 + d <= k1) issue(k0 - 1 + d);
-    RowPlane<T> qa, qb, qd;  // plane p lives in record (p - k0 + 1) % 3
-    for (int p = k0 - 1; p <= k1; p += 3) {
-        step(p, qb, qd, qa);
-        if (p + 1 <= k1) step(p + 1, qd, qa, qb);
-        if (p + 2 <= k1) step(p + 2, qa, qb, qd);
-    }
-}
-
 // true when both x faces (without their edges) are Dirichlet in all three components
 inline bool x_faces_dirichlet(const BCTable& bc) {
     for (int c = 0; c < 3; c++)
@@ -681,9 +492,6 @@ __global__ void __launch_bounds__(256) k_dirichlet_yz(Geom g, const T* __restric
     }
 }
 
-#ifndef ALAMO_ROWS_DMA
-#define ALAMO_ROWS_DMA 0
-#endif
 // 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons)
 static const bool g_rows_tile64 = [] {
     const char* e = getenv("ALAMO_ROWS_TILE");
@@ -691,13 +499,6 @@ static const bool g_rows_tile64 = [] {
 }();
 template <typename T, int MODE, int FOLD>
 static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s) {
-    if (ALAMO_ROWS_DMA) {
-        dim3 blk(64, DMA_RW);
-        dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + DMA_RW - 1) / DMA_RW,
-                  (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
-        k_iso_rows_dma<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
-        return;
-    }
     dim3 blk(64, ROWS_RW);
     if (g_rows_tile64) {
         // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
