@@ -1,0 +1,113 @@
+"""The two learned-preconditioner configurations at their stated sizes, on one GPU.
+
+* training: 2000 (residual, error) pairs from the plain solver on 128^3-cell (129^3-node)
+  two-phase fields (Gaussian random field, correlation length 8 cells, thresholded at 0,
+  lambda = mu per phase, contrast 1:100), 200 pairs on each of 10 fields, one Adam step per pair,
+  by the library's own BF16 forward and backward kernels (alamo_net_train);
+* evaluation: a 512^3-cell (513^3-node) two-phase field (correlation length 16, contrast 1:1000,
+  seed 5): plain FP64 multigrid (V-cycles), mixed-precision FGMRES, and the learned-preconditioned
+  mixed FGMRES with the network above, all to |b - A u|_inf / |b|_inf <= 1e-10 in FP64; iterations,
+  wall time and the relative L2 difference to the FP64 FGMRES solution.
+
+    python -m alamo_amd.config_runs [--fields 10 --pairs 200] [--eval-n 512]
+
+prints progress on stderr and one JSON line on stdout.  It takes minutes of GPU time, so it is
+separate from bench.py.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--train-n", type=int, default=128)
+    p.add_argument("--fields", type=int, default=10)
+    p.add_argument("--pairs", type=int, default=200, help="pairs per field")
+    p.add_argument("--contrast-train", type=float, default=100.0)
+    p.add_argument("--eval-n", type=int, default=512)
+    p.add_argument("--corr-eval", type=float, default=16.0)
+    p.add_argument("--restart", type=int, default=10)
+    p.add_argument("--sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    a = p.parse_args()
+
+    import torch
+    from . import problems as P
+    from .operator import Elastic, Linear, Krylov, Net, PREC_MIXED, PREC_FP64, CYCLE_V, PRECOND_MG, PRECOND_LEARNED
+
+    dev = torch.device("cuda:0")
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)
+    out = {"training": {}, "evaluation": {}}
+
+    # ---- training ---------------------------------------------------------------------------
+    t0 = time.time()
+    net = Net(seed=2024)
+    losses = []
+    for f in range(a.fields):
+        pr = P.two_phase_problem(n=a.train_n, corr_cells=8, contrast=a.contrast_train, seed=1 + f)
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=PREC_MIXED,
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps)
+        op.SetModel(pr["C"])
+        st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f, solve_max_iter=100, solve_tol=1e-6)
+        losses.append((st["first_loss"], st["last_loss"]))
+        log("field %d: pairs %d (converged %d), loss %.4f -> %.4f, data %.1fs, train %.1fs"
+            % (f, st["pairs"], st["pairs_converged"], st["first_loss"], st["last_loss"], st["seconds_data"],
+               st["seconds_train"]))
+        del op
+    out["training"] = {"fields": a.fields, "pairs": a.fields * a.pairs, "cells": a.train_n,
+                       "contrast": a.contrast_train, "first_loss": losses[0][0], "last_loss": losses[-1][1],
+                       "seconds": time.time() - t0}
+
+    # ---- evaluation -------------------------------------------------------------------------
+    n = a.eval_n
+    log("generating the %d^3-cell evaluation field" % n)
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr_eval, contrast=1000.0, seed=5)
+    N = (n + 1) ** 3
+    C = torch.from_numpy(pr["C"]).to(dev)
+    F0 = torch.from_numpy(pr["F0"]).to(dev)
+    del pr["C"], pr["F0"], pr["eta"]
+    rhs, xs = None, {}
+    methods = [("fp64-mg", PREC_FP64, None), ("fp64-fgmres", PREC_FP64, PRECOND_MG),
+               ("mixed-fgmres", PREC_MIXED, PRECOND_MG), ("learned-mixed-fgmres", PREC_MIXED, PRECOND_LEARNED)]
+    for name, prec, kind in methods:
+        krylov = kind is not None
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_V,
+                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=400)
+        op.SetModel(C, layout=0)
+        if rhs is None:
+            b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+            op.BCInit(b)
+            rhs = torch.zeros_like(b)
+            op.NewtonRHS(torch.zeros_like(b), b, F0, rhs)
+            del b, F0
+        if kind == PRECOND_LEARNED:
+            net.attach(op)
+        sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        op.prepareForSolve()
+        s = Krylov(op, restart=a.restart, precond=kind, max_iter=400) if krylov else Linear(op)
+        st = s.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - ts
+        out["evaluation"][name] = {"seconds": sec, "iterations": st["iterations"], "converged": st["converged"],
+                                   "final_rel_res": st["final_res_inf"] / st["rhs_inf"],
+                                   "fallback_events": st["fallback_events"]}
+        log("%s: %.3f s, %d iterations, converged %d" % (name, sec, st["iterations"], st["converged"]))
+        xs[name] = sol
+        del op, s
+        torch.cuda.empty_cache()
+    ref = xs["fp64-fgmres"]
+    for name in xs:
+        out["evaluation"][name]["rel_l2_vs_fp64_fgmres"] = float(torch.linalg.norm(xs[name] - ref) /
+                                                                 torch.linalg.norm(ref))
+    out["evaluation"]["workload"] = ("two-phase %d^3 cells, corr %g, contrast 1:1000, seed 5, Dirichlet box, "
+                                     "eigenstrain load, 1e-10" % (n, a.corr_eval))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
