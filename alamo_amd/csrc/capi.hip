@@ -29,6 +29,9 @@ using namespace alamo;
 
 static thread_local std::string g_err;
 static std::atomic<uint64_t> g_model_version{0};
+// largest grain table (21 values per grain) a MajorMinor workgroup stages in LDS: 48 KB (292 grains
+// in FP64, 585 in FP32); larger tables use the coefficient planes
+constexpr size_t MM_GRAIN_LDS_MAX = 48 * 1024;
 // interior kernel of large isotropic levels: 0 = row-wavefront kernel (default), 1 = 64x4
 // LDS-tile kernel; ALAMO_ISO_KERNEL=tiles selects the latter (for comparisons)
 static const int g_iso_kernel = [] {
@@ -70,6 +73,10 @@ struct LevelBuf {
     float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
     double* psi = nullptr;  // Elastic::m_psi_mf of the level (cells), when set
     float* psif = nullptr;
+    uint16_t* gid = nullptr;  // per-grain model (finest level): grain ids and the grain table
+    double* gtab = nullptr;
+    float* gtabf = nullptr;
+    int ngrains_cap = 0;
 };
 
 }  // namespace
@@ -177,14 +184,20 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
             const int64_t ni = (int64_t)(g.N[0] - 2) * (g.N[1] - 2) * (g.N[2] - 2);
             const int64_t nb = boundary_count(g);
             const unsigned gi = (unsigned)((ni + 255) / 256), gb = (unsigned)((nb + 255) / 256);
+            // a per-grain model whose table fits the LDS budget reads it from there
+            const size_t tabb = (size_t)21 * g.ngrains * sizeof(T);
+            const bool grains = g.gid && tabb <= MM_GRAIN_LDS_MAX;
             if (mode == APPLY) {
-                k_mm_interior<T, APPLY><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                if (grains) k_mm_interior_gid<T, APPLY><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else k_mm_interior<T, APPLY><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, APPLY><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             } else if (mode == RESID) {
-                k_mm_interior<T, RESID><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                if (grains) k_mm_interior_gid<T, RESID><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else k_mm_interior<T, RESID><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, RESID><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             } else {
-                k_mm_interior<T, JACOBI><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
+                if (grains) k_mm_interior_gid<T, JACOBI><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else k_mm_interior<T, JACOBI><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, JACOBI><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             }
             return;
@@ -988,6 +1001,10 @@ static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, 
     h->model_set = true;
     h->prepared = false;
     h->model_version = ++g_model_version;
+    l0.g.gid = nullptr;  // a full model replaces a per-grain one
+    l0.g.gtab = nullptr;
+    l0.g.gtabf = nullptr;
+    l0.g.ngrains = 0;
     return ALAMO_OK;
 }
 
@@ -1009,7 +1026,29 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
         if (gi >= ngrains) FAIL(ALAMO_ERR_ARG, "grain id out of range");
         for (int q = 0; q < 21; q++) soa[(size_t)q * tot + m] = table[21 * gi + q];
     }
-    return set_model_impl(h, 0, soa.data(), ALAMO_LAYOUT_SOA, true);
+    // the full records serve the boundary rows and the coefficient restriction
+    TRY(set_model_impl(h, 0, soa.data(), ALAMO_LAYOUT_SOA, true));
+    // the interior rows of the finest level read the grain ids and the table
+    LevelBuf& l0 = h->L[0];
+    if (!l0.gid) TRY(dalloc(h, &l0.gid, l0.g.tot));
+    if (l0.ngrains_cap < ngrains) {
+        TRY(dalloc(h, &l0.gtab, (int64_t)21 * ngrains));
+        TRY(dalloc(h, &l0.gtabf, (int64_t)21 * ngrains));
+        l0.ngrains_cap = ngrains;
+    }
+    uint16_t* staged = nullptr;
+    HIPCHK(hipMalloc(&staged, tot * sizeof(uint16_t)));
+    HIPCHK(hipMemcpy(staged, gid, tot * sizeof(uint16_t), hipMemcpyHostToDevice));
+    k_scatter_u16<<<gs_blocks(tot), 256, 0, h->stream>>>(l0.g, staged, l0.gid);
+    HIPCHK(hipMemcpyAsync(l0.gtab, table, (size_t)21 * ngrains * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    k_convert<double, float><<<1, 256, 0, h->stream>>>(l0.gtab, l0.gtabf, (int64_t)21 * ngrains);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(staged));
+    l0.g.gid = l0.gid;
+    l0.g.gtab = l0.gtab;
+    l0.g.gtabf = l0.gtabf;
+    l0.g.ngrains = ngrains;
+    return ALAMO_OK;
 }
 
 static int set_psi_impl(alamo_mg* h, int amrlev, const double* psi, double psi_small, bool host) {

@@ -44,6 +44,12 @@ struct Geom {
     const double* psi;
     const float* psif;
     double psi_small;
+    // MajorMinor stiffness given per grain (the finest level of a SetModelGrains model): the node's
+    // grain id and the grains' 21-slot records, FP64 and FP32; null otherwise
+    const uint16_t* gid;
+    const double* gtab;
+    const float* gtabf;
+    int ngrains;
 };
 
 // Boundary types per region and component, packed for kernel arguments.

@@ -331,6 +331,53 @@ __global__ void __launch_bounds__(256) k_mm_interior(Geom g, const T* __restrict
     }
 }
 
+// The same rows for a model given per grain: the grain table is staged in LDS once per workgroup
+// and each slot's seven reads come from it by the grain ids of the node and its six neighbours
+// (2-byte loads) instead of from 21 coefficient planes.  Same arithmetic in the same order as
+// k_mm_interior, so both give the same bits.
+template <typename T> __device__ __forceinline__ const T* grain_table(const Geom& g);
+template <> __device__ __forceinline__ const double* grain_table<double>(const Geom& g) { return g.gtab; }
+template <> __device__ __forceinline__ const float* grain_table<float>(const Geom& g) { return g.gtabf; }
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_mm_interior_gid(Geom g, const T* __restrict__ u, const T* __restrict__ b,
+                                                         T* __restrict__ out, T omega) {
+    extern __shared__ unsigned char smem[];
+    T* tab = (T*)smem;
+    const T* gt = grain_table<T>(g);
+    for (int e = threadIdx.x; e < 21 * g.ngrains; e += blockDim.x) tab[e] = gt[e];
+    __syncthreads();
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t I0 = g.N[0] - 2, I1 = g.N[1] - 2;
+    if (q >= I0 * I1 * (g.N[2] - 2)) return;
+    const int i = 1 + (int)(q % I0), j = 1 + (int)((q / I0) % I1), k = 1 + (int)(q / (I0 * I1));
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+    const int64_t off[7] = {0, 1, -1, g.sy, -g.sy, g.sz, -g.sz};
+    CoefGrains<T> cs;
+    cs.tab = tab;
+#pragma unroll
+    for (int d = 0; d < 7; d++) cs.row[d] = 21 * (int)g.gid[m + off[d]];
+    T f[3], D[3];
+    apply_interior_mm_cs<T>(g, cs, u, m, f, MODE == JACOBI ? D : nullptr);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const int64_t qq = p * g.tot + m;
+        if (MODE == APPLY) out[qq] = f[p];
+        else if (MODE == RESID) out[qq] = b[qq] - f[p];
+        else {
+            const T x = u[qq];
+            out[qq] = (T(1) - omega) * x + omega * (b[qq] - (f[p] - D[p] * x)) / D[p];
+        }
+    }
+}
+
+// compact node values (i fastest) -> the level's layout
+__global__ void k_scatter_u16(Geom g, const uint16_t* __restrict__ src, uint16_t* __restrict__ dst) {
+    const int64_t Vn = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < Vn; v += (int64_t)gridDim.x * blockDim.x)
+        dst[node_offset(g, v)] = src[v];
+}
+
 // x /= D (Operator<Grid::Node>::normalize)
 template <typename T>
 __global__ void k_divide(T* __restrict__ x, const T* __restrict__ D, int64_t n) {

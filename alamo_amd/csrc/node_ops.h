@@ -275,9 +275,37 @@ __device__ __forceinline__ void for_slots(F& f) {
 // apply_interior, reordered slot-major.  The 21 coefficient slots are streamed one at a time, each
 // with its six neighbours for the gradient term, so only the derivatives of u stay in registers.
 // D (optional) receives the Jacobi diagonal.
+// Coefficient sources of the MajorMinor row: SoA planes, or a grain table (ngrains x 21 values,
+// staged in LDS) indexed by the node's grain id.  at(q, o) = slot q at node m + o, o one of the
+// row's seven offsets (0, -+1 per direction; `dir` names it: 0 centre, 1 + 2d plus, 2 + 2d minus).
+template <typename T>
+struct CoefPlanes {
+    const T* __restrict__ C;
+    int64_t tot, m;
+    int64_t off[7];
+    __device__ __forceinline__ T at(int q, int dir) const { return C[(int64_t)q * tot + m + off[dir]]; }
+};
+template <typename T>
+struct CoefGrains {
+    const T* tab;  // LDS
+    int row[7];    // 21 * grain id at the seven offsets
+    __device__ __forceinline__ T at(int q, int dir) const { return tab[row[dir] + q]; }
+};
+
+template <typename T, typename CS>
+__device__ __forceinline__ void apply_interior_mm_cs(const Geom& g, const CS& cs, const T* __restrict__ u, int64_t m,
+                                                     T f[3], T* D);
+
 template <typename T>
 __device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __restrict__ C, const T* __restrict__ u,
                                                   int64_t m, T f[3], T* D) {
+    CoefPlanes<T> cs{C, g.tot, m, {0, 1, -1, g.sy, -g.sy, g.sz, -g.sz}};
+    apply_interior_mm_cs<T>(g, cs, u, m, f, D);
+}
+
+template <typename T, typename CS>
+__device__ __forceinline__ void apply_interior_mm_cs(const Geom& g, const CS& cs, const T* __restrict__ u, int64_t m,
+                                                     T f[3], T* D) {
     const int64_t st[3] = {1, g.sy, g.sz};
     const T ih[3] = {T(0.5 / g.dx[0]), T(0.5 / g.dx[1]), T(0.5 / g.dx[2])};  // central first difference
     const T i2[3] = {T(1.0 / (g.dx[0] * g.dx[0])), T(1.0 / (g.dx[1] * g.dx[1])), T(1.0 / (g.dx[2] * g.dx[2]))};
@@ -308,11 +336,10 @@ __device__ __forceinline__ void apply_interior_mm(const Geom& g, const T* __rest
     auto slot = [&](auto ti, auto tj) {
         constexpr int I = decltype(ti)::value, J = decltype(tj)::value;
         constexpr int q = mm_slot(mm_pa(I), mm_pb(I), mm_pa(J), mm_pb(J));
-        const T* Cq = C + (int64_t)q * g.tot;
-        const T c = Cq[m];
+        const T c = cs.at(q, 0);
         T dc[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) dc[d] = (Cq[m + st[d]] - Cq[m - st[d]]) * ih[d];
+        for (int d = 0; d < 3; d++) dc[d] = (cs.at(q, 1 + 2 * d) - cs.at(q, 2 + 2 * d)) * ih[d];
         mm_slot_terms<I, J, T>(c, dc, gg, G, f, D, w);
     };
     for_slots<0, 0>(slot);

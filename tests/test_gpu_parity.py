@@ -273,3 +273,56 @@ def test_fixed_iter_and_update_opts(cuda, ora):
     lin.apply(ref, sol)
     assert np.abs(to_host(ref) - rhs).max() <= 1e-11 * np.abs(rhs).max()
 
+
+
+def test_grain_table_rows_match_planes(cuda, ora):
+    """A SetModelGrains model reads its interior rows from the grain table (ids + 21-slot records
+    staged in LDS) instead of the 21 coefficient planes; the arithmetic is the same, so Fapply,
+    the residual and Fsmooth agree bit for bit with the same model given as full records, and a
+    later full SetModel replaces the grain path."""
+    pr = P.polycrystal_problem(n=40, ngrains=24, seed=7)
+    N = 41 ** 3
+    C = pr["table"][pr["gid"]].ravel()
+    ops = []
+    for grains in (True, False):
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]))
+        if grains:
+            op.SetModelGrains(pr["gid"], pr["table"])
+        else:
+            op.SetModel(C, LAYOUT_AOS)
+        ops.append(op)
+    u = rng(151).uniform(-1, 1, 3 * N)
+    b = rng(152).uniform(-1, 1, 3 * N)
+    res = []
+    for op in ops:
+        out = to_dev(np.zeros(3 * N), cuda)
+        op.Fapply(0, 0, out, to_dev(u, cuda))
+        r = to_dev(np.zeros(3 * N), cuda)
+        op.correctionResidual(0, 0, r, to_dev(u, cuda), to_dev(b, cuda))
+        x = to_dev(u, cuda)
+        op.Fsmooth(0, 0, x, to_dev(b, cuda))
+        res.append([to_host(out), to_host(r), to_host(x)])
+    for a, c in zip(res[0], res[1]):
+        assert np.array_equal(a, c)
+    close(res[0][0], ora.fapply(pr["ncell"], pr["dx"], P.MAJORMINOR, C, pr["bct"], u))
+    # mixed solves: the FP32 table and the FP32 planes are the same conversions
+    sols = []
+    for grains in (True, False):
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=PREC_MIXED,
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=3)
+        if grains:
+            op.SetModelGrains(pr["gid"], pr["table"])
+        else:
+            op.SetModel(C, LAYOUT_AOS)
+        rhs = to_dev(ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"]), cuda)
+        sol = to_dev(np.zeros(3 * N), cuda)
+        st = Krylov(op, restart=10, max_iter=200).solve(sol, rhs, 1e-10, 0.0)
+        sols.append((st["iterations"], to_host(sol)))
+    assert sols[0][0] == sols[1][0] and np.array_equal(sols[0][1], sols[1][1])
+    # a full model set afterwards replaces the per-grain path
+    op = ops[0]
+    C2 = C * 1.5
+    op.SetModel(C2, LAYOUT_AOS)
+    out = to_dev(np.zeros(3 * N), cuda)
+    op.Fapply(0, 0, out, to_dev(u, cuda))
+    close(to_host(out), ora.fapply(pr["ncell"], pr["dx"], P.MAJORMINOR, C2, pr["bct"], u))
