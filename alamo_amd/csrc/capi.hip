@@ -92,6 +92,9 @@ struct alamo_mg {
     double* red_host = nullptr;     // pinned host mirror (KRY_MAXV + 4 doubles)
     double* bottom_work = nullptr;  // 8 x 3 x tot(bottom)
     int* bottom_iters = nullptr;
+    double* dense_A = nullptr;      // dense bottom: assembled coarsest operator (column-major n x n)
+    double* dense_inv = nullptr;    // unit columns during assembly, then the row-major inverse
+    int64_t dense_n = 0;
     double* sol_res = nullptr;      // level-0 FP64 residual for the mixed-precision outer loop
     bool model_set = false, prepared = false;
     uint64_t model_version = 0;  // unique per SetModel call (the network's features follow it)
@@ -311,9 +314,16 @@ static void bottom(alamo_mg* h) {
     // the bottom solve always runs in FP64
     if constexpr (std::is_same<T, float>::value)
         k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(lb.resf, lb.res, n);
-    vfill<double>(lb.cor, n, 0.0, h->stream);
-    launch_bottom<double>(h->sym, lb.g, h->bc, lb.C, lb.D, lb.cor, lb.res, h->bottom_work, h->d.bottom_tol_rel,
-                          h->d.bottom_max_iter, h->bottom_iters, h->stream);
+    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) {
+        k_dense_matvec<<<nblocks(n, 4), 256, 0, h->stream>>>(h->dense_inv, lb.res, lb.cor, n);
+    } else if (h->d.bottom_solver == ALAMO_BOTTOM_SMOOTHER) {
+        vfill<double>(lb.cor, n, 0.0, h->stream);
+        smooth<double>(h, bot, 8);
+    } else {
+        vfill<double>(lb.cor, n, 0.0, h->stream);
+        launch_bottom<double>(h->sym, lb.g, h->bc, lb.C, lb.D, lb.cor, lb.res, h->bottom_work, h->d.bottom_tol_rel,
+                              h->d.bottom_max_iter, h->bottom_iters, h->stream);
+    }
     if constexpr (std::is_same<T, float>::value)
         k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(lb.cor, lb.corf, n);
 }
@@ -372,6 +382,81 @@ static void one_cycle(alamo_mg* h) {
 // ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
+// Gauss-Jordan elimination with partial pivoting of the column-major n x n matrix A into the
+// row-major inverse; false when a pivot falls below 1e-11 max|A| (a singular coarsest operator,
+// e.g. one that no displacement condition pins)
+static bool dense_invert(const std::vector<double>& A, int64_t n, std::vector<double>& inv) {
+    std::vector<double> M(n * n);
+    double amax = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) {
+            M[i * n + j] = A[j * n + i];
+            amax = std::max(amax, std::fabs(M[i * n + j]));
+        }
+    inv.assign(n * n, 0.0);
+    for (int64_t i = 0; i < n; i++) inv[i * n + i] = 1.0;
+    for (int64_t c = 0; c < n; c++) {
+        int64_t p = c;
+        for (int64_t r = c + 1; r < n; r++)
+            if (std::fabs(M[r * n + c]) > std::fabs(M[p * n + c])) p = r;
+        if (!(std::fabs(M[p * n + c]) > 1e-11 * amax)) return false;
+        if (p != c)
+            for (int64_t j = 0; j < n; j++) {
+                std::swap(M[p * n + j], M[c * n + j]);
+                std::swap(inv[p * n + j], inv[c * n + j]);
+            }
+        const double d = 1.0 / M[c * n + c];
+        for (int64_t j = 0; j < n; j++) {
+            M[c * n + j] *= d;
+            inv[c * n + j] *= d;
+        }
+        for (int64_t r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = M[r * n + c];
+            if (f == 0.0) continue;
+            for (int64_t j = 0; j < n; j++) {
+                M[r * n + j] -= f * M[c * n + j];
+                inv[r * n + j] -= f * inv[c * n + j];
+            }
+        }
+    }
+    return true;
+}
+
+// solver.bottom_solver "dense": assemble the coarsest FP64 operator (with its psi and boundary
+// rows) column by column on the device, invert it on the host and keep the inverse on the device.
+// Pad entries of a pitched row layout get identity rows (their right-hand side is 0).
+static int dense_bottom_factor(alamo_mg* h) {
+    LevelBuf& lb = h->L.back();
+    const int64_t n = 3 * lb.g.tot;
+    if (n > ALAMO_DENSE_BOTTOM_MAX)
+        FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest level has " + std::to_string(n) + " unknowns, more than " +
+                                std::to_string(ALAMO_DENSE_BOTTOM_MAX));
+    if (h->dense_n != n) {
+        TRY(dalloc(h, &h->dense_A, n * n));
+        TRY(dalloc(h, &h->dense_inv, n * n));
+        h->dense_n = n;
+    }
+    k_unit_columns<<<gs_blocks(n * n), 256, 0, h->stream>>>(h->dense_inv, n);
+    const unsigned cols = (unsigned)n;
+    if (h->sym == ISO) k_bottom_columns<ISO><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+    else if (h->sym == MAJORMINOR)
+        k_bottom_columns<MAJORMINOR><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+    else k_bottom_columns<DIAGONAL><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+    HIPCHK(hipGetLastError());
+    std::vector<double> A(n * n), inv;
+    HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int64_t q = 0; q < n; q++)
+        if ((q % lb.g.tot) % lb.g.sy >= lb.g.N[0]) A[q * n + q] = 1.0;
+    if (!dense_invert(A, n, inv))
+        FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest-level operator is singular (no displacement "
+                            "condition pins it); use bottom_solver bicgstab or smoother");
+    HIPCHK(hipMemcpyAsync(h->dense_inv, inv.data(), n * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ALAMO_OK;
+}
+
 static int prepare(alamo_mg* h) {
     if (!h->model_set) FAIL(ALAMO_ERR_STATE, "alamo_mg_prepare: SetModel has not been called");
     const int nl = (int)h->L.size();
@@ -404,6 +489,7 @@ static int prepare(alamo_mg* h) {
         }
     }
     HIPCHK(hipGetLastError());
+    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) TRY(dense_bottom_factor(h));
     h->prepared = true;
     return ALAMO_OK;
 }
@@ -826,6 +912,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->verbose = 0;
     d->row_align = 0;
     d->fixed_iter = 0;
+    d->bottom_solver = ALAMO_BOTTOM_BICGSTAB;
     return ALAMO_OK;
 }
 
@@ -844,6 +931,8 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
         }
     if (d->sweeps_per_smooth < 1 || d->pre_smooth < 0 || d->post_smooth < 0) FAIL(ALAMO_ERR_ARG, "bad smoothing");
     if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
+    if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
+        FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -874,6 +963,7 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
         if (!mixed || l == gs.size() - 1 || l == 0) {
             rc |= dalloc(h, &lb.cor, v);
             rc |= dalloc(h, &lb.res, v);
+            if (mixed && l == gs.size() - 1) rc |= dalloc(h, &lb.tmp, v);  // the smoother bottom's sweeps
             if (!mixed) {
                 rc |= dalloc(h, &lb.rescor, v);
                 rc |= dalloc(h, &lb.tmp, v);
@@ -950,6 +1040,10 @@ int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
     if (d->sweeps_per_smooth < 1 || d->pre_smooth < 0 || d->post_smooth < 0) FAIL(ALAMO_ERR_ARG, "bad smoothing");
     if (d->cycle != ALAMO_CYCLE_V && d->cycle != ALAMO_CYCLE_F) FAIL(ALAMO_ERR_ARG, "unknown cycle");
     if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
+    if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
+        FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
+    if (d->bottom_solver != h->d.bottom_solver) h->prepared = false;  // the dense inverse is built at prepare
+    h->d.bottom_solver = d->bottom_solver;
     h->d.cycle = d->cycle;
     h->d.pre_smooth = d->pre_smooth;
     h->d.post_smooth = d->post_smooth;

@@ -532,4 +532,33 @@ __global__ void __launch_bounds__(BOTTOM_BLOCK) k_bottom_bicgstab(Geom g, BCTabl
     if (threadIdx.x == 0 && iters) *iters = it;
 }
 
+// ---------------------------------------------------------------------------
+// Dense bottom solve (solver.bottom_solver "dense"): the coarsest operator is assembled column
+// by column at prepare (column j = A e_j: block j applies the level's operator to unit vector j,
+// column j of E), inverted on the host, and each cycle's bottom solve is x = A^-1 b, one
+// wavefront per row of the row-major inverse.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_unit_columns(double* __restrict__ E, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n * n; q += (int64_t)gridDim.x * blockDim.x)
+        E[q] = (q / n == q % n) ? 1.0 : 0.0;
+}
+
+template <int SYM>
+__global__ void __launch_bounds__(256) k_bottom_columns(Geom g, BCTable bc, const double* __restrict__ C,
+                                                        const double* __restrict__ E, double* __restrict__ A) {
+    const int64_t n = 3 * g.tot;
+    bottom_apply<double, SYM>(g, bc, C, E + (int64_t)blockIdx.x * n, A + (int64_t)blockIdx.x * n);
+}
+
+__global__ void __launch_bounds__(256) k_dense_matvec(const double* __restrict__ Ainv, const double* __restrict__ b,
+                                                      double* __restrict__ x, int64_t n) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;  // whole wavefronts leave together
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s += Ainv[row * n + j] * b[j];
+    s = wave_sum(s);
+    if (lane == 0) x[row] = s;
+}
+
 }  // namespace alamo

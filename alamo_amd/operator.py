@@ -30,6 +30,21 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+BOTTOM_SOLVERS = {"bicgstab": 0, "smoother": 1, "dense": 2}
+
+
+def bottom_solver_code(name):
+    """solver.bottom_solver value (Linear.H:280-282) -> ALAMO_BOTTOM_*; the reference's "cg" is not
+    built (the operator is not symmetric; the reference itself forces bicgstab, Linear.H:258)."""
+    if isinstance(name, int):
+        if name not in BOTTOM_SOLVERS.values():
+            raise ValueError("unknown bottom_solver %d" % name)
+        return name
+    if name not in BOTTOM_SOLVERS:
+        raise ValueError("bottom_solver must be one of %s, not %r" % (sorted(BOTTOM_SOLVERS), name))
+    return BOTTOM_SOLVERS[name]
+
+
 class Elastic:
     """Operator::Elastic<SYM> on one box covering a non-periodic nodal domain.
 
@@ -37,7 +52,8 @@ class Elastic:
 
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
-                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0, fixed_iter=0):
+                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0, fixed_iter=0,
+                 bottom_solver="bicgstab"):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
         for q in range(3):
@@ -58,6 +74,7 @@ class Elastic:
         d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
         d.row_align = int(row_align)
         d.fixed_iter = int(fixed_iter)
+        d.bottom_solver = bottom_solver_code(bottom_solver)
         self.desc = d
         self.sym = sym
         self.ncell = tuple(int(n) for n in ncell)
@@ -92,12 +109,15 @@ class Elastic:
 
     def UpdateOpts(self, **kw):
         """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
-        sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose)."""
+        sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose,
+        bottom_solver)."""
         allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
-                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose"}
+                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver"}
         for k, v in kw.items():
             if k not in allowed:
                 raise ValueError("not a solver option: " + k)
+            if k == "bottom_solver":
+                v = bottom_solver_code(v)
             setattr(self.desc, k, v)
         check(lib().alamo_mg_update_opts(self.h, ctypes.byref(self.desc)))
 
@@ -285,6 +305,10 @@ class Linear:
 
     def setPostSmooth(self, n):
         self._opt(post_smooth=int(n))
+
+    def setBottomSolver(self, name):
+        """solver.bottom_solver (Linear.H:280-282): "bicgstab", "smoother" or "dense"."""
+        self._opt(bottom_solver=name)
 
     def apply(self, rhs, sol):
         """Linear::apply (Linear.H:169-174): rhs = A sol on the finest level"""

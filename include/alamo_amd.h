@@ -34,6 +34,12 @@ enum { ALAMO_LAYOUT_AOS = 0, /* BaseFab<Set::Matrix4<3,SYM>>: ncoef doubles per 
        ALAMO_LAYOUT_SOA = 1  /* ncoef planes of (n0+1)(n1+1)(n2+1) doubles */ };
 enum { ALAMO_PREC_FP64 = 0, ALAMO_PREC_MIXED = 1 /* FP32 cycles inside FP64 iterative refinement */ };
 enum { ALAMO_CYCLE_V = 0, ALAMO_CYCLE_F = 1 };
+/* solver.bottom_solver (Solver/Nonlocal/Linear.H:241,258,280-282) */
+enum { ALAMO_BOTTOM_BICGSTAB = 0, /* Jacobi-preconditioned BiCGStab on the coarsest level (the default) */
+       ALAMO_BOTTOM_SMOOTHER = 1, /* 8 Fsmooth calls from zero (amrex::MLMG's final_smooth default) */
+       ALAMO_BOTTOM_DENSE = 2     /* direct: the coarsest operator assembled and inverted at prepare
+                                     (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle */ };
+#define ALAMO_DENSE_BOTTOM_MAX 1200
 enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, ALAMO_ERR_NOCONV = 4,
        ALAMO_ERR_NAN = 5 };
 
@@ -70,6 +76,8 @@ typedef struct {
     int fixed_iter;            /* solver.fixed_iter (Linear.H:181,267): > 0 runs exactly that many
                                   multigrid iterations and returns without a convergence failure
                                   (amrex::MLMG::setFixedIter); 0 off */
+    int bottom_solver;         /* ALAMO_BOTTOM_*; the reference's "cg" choice is not built (the
+                                  operator is not symmetric) */
 } alamo_mg_desc;
 
 typedef struct {

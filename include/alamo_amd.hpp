@@ -151,6 +151,16 @@ public:
     void setPreSmooth(int n) { opt().pre_smooth = n; push(); }
     void setPostSmooth(int n) { opt().post_smooth = n; push(); }
     void setBottomTolerance(double t) { opt().bottom_tol_rel = t; push(); }
+    // solver.bottom_solver (Linear.H:280-282): "bicgstab", "smoother" or "dense" ("cg" is not built)
+    void setBottomSolver(const std::string& name) {
+        int b = name == "bicgstab" ? ALAMO_BOTTOM_BICGSTAB
+                : name == "smoother" ? ALAMO_BOTTOM_SMOOTHER
+                : name == "dense"    ? ALAMO_BOTTOM_DENSE
+                                     : -1;
+        if (b < 0) throw alamo::Abort(ALAMO_ERR_ARG, "unknown bottom_solver " + name);
+        opt().bottom_solver = b;
+        push();
+    }
     // Linear::apply (Linear.H:169-174): rhs = A sol on the finest level
     void apply(double* rhs, const double* sol) { defined().Apply(0, 0, rhs, sol); }
     // Linear::solveaffine (Linear.H:53-130) subtracts A 0 under inhomogeneous boundaries first;

@@ -1,0 +1,121 @@
+"""GPU: the bottom-solver choices of solver.bottom_solver (Solver/Nonlocal/Linear.H:241,258,280-282).
+
+* dense: the coarsest operator assembled column by column on the GPU and inverted at prepare.  On
+  a one-level hierarchy a multigrid iteration is then one exact solve, so the solution must equal
+  the direct solve of the oracle's operator (assembled from the oracle's Fapply on unit vectors,
+  solved by LAPACK through numpy) to 1e-10 of its scale, after exactly one iteration.
+* dense, smoother and bicgstab on a multilevel hierarchy (a polycrystal under compression with
+  Neumann sides, whose coarsest level keeps free nodes) converge to 1e-10 and agree with the oracle's FP64
+  multigrid solution (BiCGStab bottom) to 1e-8 relative L2 (the three stop at different points
+  inside the 1e-10 residual ball).
+* the error paths: a singular coarsest operator (traction on every boundary: the rigid motions
+  are in its null space) and a coarsest level above the dense limit fail at prepare.
+"""
+import numpy as np
+import pytest
+
+from alamo_amd import problems as P
+from alamo_amd._lib import AlamoError
+from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, PRECOND_MG
+from helpers import random_iso, random_mm, random_diag, to_dev, to_host, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def coeffs(sym, nodes, seed):
+    return {P.ISO: random_iso, P.MAJORMINOR: random_mm, P.DIAGONAL: random_diag}[sym](nodes, seed)
+
+
+def oracle_matrix(ora, ncell, dx, sym, C, bct):
+    n = 3 * int(np.prod([c + 1 for c in ncell]))
+    A = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        A[:, j] = ora.fapply(ncell, dx, sym, C, bct, e)
+        e[j] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("sym,ncell", [(P.ISO, (4, 4, 4)), (P.MAJORMINOR, (3, 4, 5)), (P.DIAGONAL, (4, 3, 3))])
+def test_dense_bottom_single_level_is_a_direct_solve(cuda, ora, sym, ncell):
+    print("dense single level", sym, ncell, flush=True)
+    nodes = int(np.prod([c + 1 for c in ncell]))
+    dx = tuple(1.0 / c for c in ncell)
+    C = coeffs(sym, nodes, 11)
+    bct, bcv = P.bc_constant()
+    bct = np.array(bct, dtype=np.int32).reshape(26, 3)
+    # mix traction faces into the displacement box so the boundary rows of every kind enter the matrix
+    bct[[0, 3, 7, 12]] = P.TRACTION
+    b = rng(3).standard_normal(3 * nodes)
+    A = oracle_matrix(ora, ncell, dx, sym, C, bct)
+    x_ref = np.linalg.solve(A, b)
+    op = Elastic(ncell, dx, sym=sym, bc=(bct, bcv), max_coarsening_level=0, bottom_solver="dense")
+    op.SetModel(C, LAYOUT_AOS)
+    assert op.NMGLevels() == 1
+    sol = to_dev(np.zeros_like(b), cuda)
+    st = Linear(op).solve(sol, to_dev(b, cuda), 1e-11, 0.0)
+    assert st["converged"] == 1 and st["iterations"] == 1, st
+    x = to_host(sol)
+    err = np.abs(x - x_ref).max() / np.abs(x_ref).max()
+    print("  max rel err vs LAPACK %.2e" % err)
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_bottom_solvers_agree_with_oracle(cuda, ora, precision):
+    print("bottom solvers, precision", precision, flush=True)
+    # polycrystal compression (bottom fixed, top displaced, Neumann sides): the coarsest level
+    # (3^3 nodes) keeps its middle plane free, so the bottom solve is not trivial
+    pr = P.polycrystal_problem(n=32, ngrains=8, seed=3)
+    bc = (pr["bct"], pr["bcv"])
+    N = 33 ** 3
+    C = pr["table"][pr["gid"]].ravel()
+    b = ora.bc_init(pr["ncell"], bc[0], bc[1])
+    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.MAJORMINOR, C, None, bc[0], np.zeros(3 * N), b)
+    x_ora, it_ora, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.MAJORMINOR, C, bc[0], rhs, tol_rel=1e-12)
+    assert it_ora > 0
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=bc, precision=precision, max_iter=100)
+    op.SetModel(C, LAYOUT_AOS)
+    lin = Linear(op)
+    its = {}
+    for name in ("bicgstab", "dense", "smoother"):
+        lin.setBottomSolver(name)  # switched on the built operator; dense is assembled at the next prepare
+        sol = to_dev(np.zeros_like(rhs), cuda)
+        st = lin.solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
+        x = to_host(sol)
+        err = np.linalg.norm(x - x_ora) / np.linalg.norm(x_ora)
+        its[name] = st["iterations"]
+        print("  %s: %d iterations, rel L2 vs oracle %.2e" % (name, st["iterations"], err))
+        assert st["converged"] == 1 and err <= 1e-8
+    # the exact bottom solve needs no more cycles than the 1e-4 BiCGStab one (within one or two:
+    # the two cycles are different preconditioners of the same iteration)
+    assert its["dense"] <= its["bicgstab"] + 2
+    # and the same inverse serves the Krylov solver's V-cycle preconditioner
+    lin.setBottomSolver("dense")
+    sol = to_dev(np.zeros_like(rhs), cuda)
+    st = Krylov(op, restart=10, precond=PRECOND_MG, max_iter=100).solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
+    err = np.linalg.norm(to_host(sol) - x_ora) / np.linalg.norm(x_ora)
+    print("  fgmres + dense bottom: %d iterations, rel L2 %.2e" % (st["iterations"], err))
+    assert st["converged"] == 1 and err <= 1e-8
+
+
+def test_dense_bottom_error_paths(cuda):
+    print("dense bottom errors", flush=True)
+    ncell = (8, 8, 8)
+    nodes = 9 ** 3
+    dx = (0.125,) * 3
+    C = coeffs(P.ISO, nodes, 5)
+    bct = np.full((26, 3), P.TRACTION, dtype=np.int32)
+    op = Elastic(ncell, dx, sym=P.ISO, bc=(bct, np.zeros((26, 3))), bottom_solver="dense")
+    op.SetModel(C, LAYOUT_AOS)
+    with pytest.raises(AlamoError, match="singular"):
+        op.prepareForSolve()
+    bct, bcv = P.bc_constant()
+    big = Elastic(ncell, dx, sym=P.ISO, bc=(bct, bcv), max_coarsening_level=0, bottom_solver="dense")
+    big.SetModel(C, LAYOUT_AOS)
+    with pytest.raises(AlamoError, match="more than 1200"):
+        big.prepareForSolve()
+    # the same operator solves with the default bottom solver
+    big.UpdateOpts(bottom_solver="bicgstab")
+    big.prepareForSolve()
