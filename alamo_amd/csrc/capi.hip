@@ -343,7 +343,7 @@ static void vcycle(alamo_mg* h, int top, bool first_done = false) {
     for (int l = bot - 1; l >= top; l--) {
         LevelBuf& lb = h->L[l];
         LevelBuf& lc = h->L[l + 1];
-        k_interp_add<T><<<nblocks(lb.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
         smooth<T>(h, l, h->d.post_smooth);
     }
 }
@@ -362,7 +362,7 @@ static void fcycle(alamo_mg* h) {
         LevelBuf& lc = h->L[l + 1];
         const int64_t n = 3 * lb.g.tot;
         vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
-        k_interp_add<T><<<nblocks(lb.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
         // res = res - L(cor)
         launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
                        T(0), h->stream);
@@ -569,6 +569,34 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
 static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, int nv, double* out);
 #include "learned_host.h"
 
+// z = V-cycle(b) from a zero guess, b given in precision Ti, with the level-0 correction in z
+// itself: the correction and its ping-pong partner on level 0 swap once per sweep after the
+// first, so z takes the place of the partner when that count (pre + post sweeps - 1) is odd and
+// of the correction when it is even, and the cycle's last sweep writes z (no copy).  A V-cycle
+// only reads its level-0 right-hand side, so when b is already in the cycle precision the level
+// reads b in place; otherwise the first sweep's pass stages it.  z must keep zeros in the pad
+// entries of a pitched layout; the level's own buffers are restored afterwards.
+template <typename Ti, typename T>
+static void vcycle_into(alamo_mg* h, const Ti* b, T* z) {
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    T* own_cor = LB<T>::cor(l0);
+    T* own_tmp = LB<T>::tmp(l0);
+    T* own_res = LB<T>::res(l0);
+    const int swaps = (h->d.pre_smooth + h->d.post_smooth) * h->d.sweeps_per_smooth - 1;
+    if (swaps % 2) LB<T>::tmp(l0) = z;
+    else LB<T>::cor(l0) = z;
+    T* stage = nullptr;
+    if constexpr (std::is_same<Ti, T>::value) LB<T>::res(l0) = const_cast<T*>(b);
+    else stage = own_res;
+    k_jacobi_from_zero<Ti, T><<<gs_blocks(n), 256, 0, h->stream>>>(b, stage, LB<T>::cor(l0), LB<T>::D(l0), n,
+                                                                   T(h->d.omega));
+    vcycle<T>(h, 0, true);
+    LB<T>::cor(l0) = own_cor;
+    LB<T>::tmp(l0) = own_tmp;
+    LB<T>::res(l0) = own_res;
+}
+
 static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
@@ -593,14 +621,12 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
         k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(l0.corf, z, n);
     } else {
         if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
-            k_jacobi_from_zero<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.res, l0.cor, l0.D, n,
-                                                                                    h->d.omega);
-            vcycle<double>(h, 0, true);
+            vcycle_into<double, double>(h, v, z);
         } else {
             HIPCHK(hipMemcpyAsync(l0.res, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
             one_cycle<double>(h);
+            HIPCHK(hipMemcpyAsync(z, l0.cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         }
-        HIPCHK(hipMemcpyAsync(z, l0.cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
     return ALAMO_OK;
 }
@@ -669,14 +695,12 @@ static int precond_f32(alamo_mg* h, const float* v, float* z) {
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
     if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
-        k_jacobi_from_zero<float, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, l0.corf, l0.Df, n,
-                                                                              (float)h->d.omega);
-        vcycle<float>(h, 0, true);
+        vcycle_into<float, float>(h, v, z);
     } else {
         HIPCHK(hipMemcpyAsync(l0.resf, v, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
         one_cycle<float>(h);
+        HIPCHK(hipMemcpyAsync(z, l0.corf, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     }
-    HIPCHK(hipMemcpyAsync(z, l0.corf, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     return ALAMO_OK;
 }
 
@@ -1305,7 +1329,7 @@ int alamo_mg_interpolate(alamo_mg* h, int amrlev, int fmglev, double* fine, cons
     if (fmglev >= (int)h->L.size() - 1) FAIL(ALAMO_ERR_ARG, "fmglev must have a coarser level");
     LevelBuf& lc = h->L[fmglev + 1];
     LevelBuf& lf = h->L[fmglev];
-    k_interp_add<double><<<nblocks(lf.g.tot), 256, 0, (hipStream_t)stream>>>(lc.g, lf.g, crse, fine);
+    k_interp_add<double><<<interp_blocks(lc.g, lf.g), 256, 0, (hipStream_t)stream>>>(lc.g, lf.g, crse, fine);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -1696,7 +1720,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
         for (int l = lc - 1; l >= 0; l--) {
             double* fine = l == 0 ? r : lv[l];
             vfill<double>(fine, 3 * h->L[l].g.tot, 0.0, h->stream);
-            k_interp_add<double><<<nblocks(h->L[l].g.tot), 256, 0, h->stream>>>(h->L[l + 1].g, h->L[l].g, cur, fine);
+            k_interp_add<double><<<interp_blocks(h->L[l + 1].g, h->L[l].g), 256, 0, h->stream>>>(h->L[l + 1].g, h->L[l].g, cur, fine);
             cur = fine;
         }
         // z = V(r), q = r - A z, e = A^-1 q

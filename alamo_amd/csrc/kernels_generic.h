@@ -8,8 +8,18 @@ namespace alamo {
 
 enum Mode { APPLY = 0, RESID = 1, JACOBI = 2, DIAG = 3 };
 
-// (i, j, k) of element m of a level's array; i >= N[0] marks a pad entry of a pitched row
+// (i, j, k) of element m of a level's array; i >= N[0] marks a pad entry of a pitched row.
+// Levels below 2^31 elements (every level up to 1290^3 nodes) divide in 32 bits: a 64-bit
+// division is a long software sequence on the GPU, a 32-bit one a few instructions.
 __device__ __forceinline__ void node_ijk(const Geom& g, int64_t m, int& i, int& j, int& k) {
+    if (g.tot <= 0x7fffffff) {
+        const uint32_t mm = (uint32_t)m, sz = (uint32_t)g.sz, sy = (uint32_t)g.sy;
+        const uint32_t kk = mm / sz, r = mm - kk * sz, jj = r / sy;
+        k = (int)kk;
+        j = (int)jj;
+        i = (int)(r - jj * sy);
+        return;
+    }
     k = (int)(m / g.sz);
     int64_t r = m - (int64_t)k * g.sz;
     j = (int)(r / g.sy);
@@ -17,6 +27,11 @@ __device__ __forceinline__ void node_ijk(const Geom& g, int64_t m, int& i, int& 
 }
 // element offset of compact node index v (i fastest over the N[0] x N[1] x N[2] nodes)
 __device__ __forceinline__ int64_t node_offset(const Geom& g, int64_t v) {
+    if (g.tot <= 0x7fffffff) {
+        const uint32_t vv = (uint32_t)v, N0 = (uint32_t)g.N[0], N01 = N0 * (uint32_t)g.N[1];
+        const uint32_t k = vv / N01, r = vv - k * N01, j = r / N0;
+        return (int64_t)(r - j * N0) + g.sy * j + g.sz * k;
+    }
     const int64_t N01 = (int64_t)g.N[0] * g.N[1];
     const int k = (int)(v / N01);
     const int64_t r = v - (int64_t)k * N01;
@@ -258,35 +273,59 @@ __global__ void __launch_bounds__(256) k_restrict_psi(Geom gc, Geom gf, const do
 }
 
 // fine += I(crse), trilinear nodal prolongation by parity (Operator/Operator.cpp:399-428).
+// One work-item per coarse x-interval of a fine row: it adds the even fine node i = 2I and the
+// odd node 2I + 1 (when I < n_c), so no wavefront diverges on the x parity, and the row parities
+// of j and k are uniform along the row.  Each node keeps its case's formula and term order.
 template <typename T>
 __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* __restrict__ crse,
                                                     T* __restrict__ fine) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= gf.tot) return;
-    int i, j, k;
-    node_ijk(gf, m, i, j, k);
-    if (i >= gf.N[0]) return;
-    const int I = i / 2, J = j / 2, K = k / 2;
-    const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nt = (int64_t)gc.N[0] * gf.N[1] * gf.N[2];
+    if (t >= nt) return;
+    int I, j, k;
+    if (nt <= 0x7fffffff) {
+        const uint32_t tt = (uint32_t)t, Nc0 = (uint32_t)gc.N[0], N1 = (uint32_t)gf.N[1];
+        const uint32_t row = tt / Nc0, kk = row / N1;
+        I = (int)(tt - row * Nc0);
+        k = (int)kk;
+        j = (int)(row - kk * N1);
+    } else {
+        const int64_t row = t / gc.N[0];
+        I = (int)(t - row * gc.N[0]);
+        k = (int)(row / gf.N[1]);
+        j = (int)(row - (int64_t)k * gf.N[1]);
+    }
+    const int J = j / 2, K = k / 2;
+    const bool ej = (j & 1) == 0, ek = (k & 1) == 0, odd = I < gc.n[0];
     const int64_t mc = I + gc.sy * J + gc.sz * K;
+    const int64_t mf = 2 * I + gf.sy * j + gf.sz * k;
     const int64_t sx = 1, sy = gc.sy, sz = gc.sz;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         const T* Cp = crse + p * gc.tot + mc;
         auto at = [&](int a, int b, int c) { return Cp[a * sx + b * sy + c * sz]; };
-        T v;
-        if (ei && ej && ek) v = at(0, 0, 0);
-        else if (ej && ek) v = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
-        else if (ek && ei) v = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
-        else if (ei && ej) v = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
-        else if (ei) v = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
-        else if (ej) v = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
-        else if (ek) v = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
-        else
-            v = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
-                            at(1, 1, 0) + at(1, 1, 1));
-        fine[p * gf.tot + m] += v;
+        T ve, vo = T(0);
+        if (ej && ek) ve = at(0, 0, 0);
+        else if (ek) ve = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
+        else if (ej) ve = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
+        else ve = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
+        if (odd) {
+            if (ej && ek) vo = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
+            else if (ej) vo = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
+            else if (ek) vo = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
+            else
+                vo = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
+                                 at(1, 1, 0) + at(1, 1, 1));
+        }
+        T* Fp = fine + p * gf.tot + mf;
+        Fp[0] += ve;
+        if (odd) Fp[1] += vo;
     }
+}
+
+// launch shape of k_interp_add: one work-item per coarse x node of every fine row
+inline unsigned interp_blocks(const Geom& gc, const Geom& gf) {
+    return (unsigned)(((int64_t)gc.N[0] * gf.N[1] * gf.N[2] + 255) / 256);
 }
 
 // ---------------------------------------------------------------------------

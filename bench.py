@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
     p.add_argument("--krylov-sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
+                   help="bottom solver of the FGMRES V-cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
     p.add_argument("--methods", default="fp64v,mixedv,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
@@ -127,7 +129,7 @@ def main():
         nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
                      pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
-                     verbose=a.verbose)
+                     verbose=a.verbose, bottom_solver=a.krylov_bottom if krylov else "bicgstab")
         op.SetModel(pr["C"])
         op.prepareForSolve()
         if rhs is None:
@@ -234,7 +236,7 @@ def main():
                         % (a.corr, a.contrast, a.seed, n, n + 1),
             "method": best,
             "fgmres": {"restart": a.restart, "vcycle_smooth_calls": a.krylov_smooth,
-                       "jacobi_sweeps_per_call": a.krylov_sweeps},
+                       "jacobi_sweeps_per_call": a.krylov_sweeps, "bottom_solver": a.krylov_bottom},
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
