@@ -21,7 +21,7 @@ class MGDesc(ctypes.Structure):
                 ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
                 ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
                 ("bottom_tol_rel", c_double), ("verbose", c_int), ("row_align", c_int), ("fixed_iter", c_int),
-                ("bottom_solver", c_int)]
+                ("bottom_solver", c_int), ("max_fmg_iter", c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -67,6 +67,9 @@ SIGNATURES = {
     "alamo_mg_strain": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_stress": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_energy": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_newton_dw": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_newton_w": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_set_uniform": (c_int, [c_void_p, c_int]),
     "alamo_mg_trac_xhi": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),
     "alamo_blend_models": (c_int, [ctypes.c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),
@@ -79,6 +82,7 @@ SIGNATURES = {
     "alamo_mg_fapply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_diagonal": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_fsmooth": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_error0x": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_normalize": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_residual": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_restrict": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),

@@ -107,6 +107,7 @@ struct alamo_mg {
     float* kZf = nullptr;
     int kry_mf = 0;
     bool psi_set = false;     // Elastic::m_psi_set
+    bool uniform = false;     // Elastic::m_uniform
     double psi_small = 1e-8;  // Elastic::m_psi_small
     double* kcoef = nullptr;  // device copy of Gram-Schmidt / solution coefficients
     double* kpart = nullptr;  // multi-dot partials
@@ -164,8 +165,8 @@ static void launch_node_sym(int mode, const Geom& g, const BCTable& bc, const T*
 template <typename T>
 static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, const T* C, const T* u, const T* b,
                         T* out, T omega, hipStream_t s) {
-    if (g.psi) {
-        // the psi terms live in the node-parallel rows
+    if (g.psi || g.uniform) {
+        // the psi terms and the uniform-model rows live in the node-parallel kernels
         if (sym == ISO) launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);
         else if (sym == MAJORMINOR) launch_node_sym<T, MAJORMINOR>(mode, g, bc, C, u, b, out, omega, s);
         else launch_node_sym<T, DIAGONAL>(mode, g, bc, C, u, b, out, omega, s);
@@ -373,9 +374,11 @@ static void fcycle(alamo_mg* h) {
     }
 }
 
+// amrex::MLMG::oneIter: an F-cycle while iter < max_fmg_iter, then V-cycles (the reference sets
+// max_fmg_iter to 1e8, Linear.H:261, so every iteration is an F-cycle)
 template <typename T>
-static void one_cycle(alamo_mg* h) {
-    if (h->d.cycle == ALAMO_CYCLE_F) fcycle<T>(h);
+static void one_cycle(alamo_mg* h, int iter = 0) {
+    if (h->d.cycle == ALAMO_CYCLE_F && iter < h->d.max_fmg_iter) fcycle<T>(h);
     else vcycle<T>(h, 0);
 }
 
@@ -532,10 +535,10 @@ static int solve_impl(alamo_mg* h, double* sol, const double* rhs, double tol_re
     while ((fixed ? it < fixed : (rn > target && it < h->d.max_iter)) && std::isfinite(rn)) {
         if (mixed) {
             k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(res0(), h->L[0].resf, n);
-            one_cycle<float>(h);
+            one_cycle<float>(h, it);
             k_axpy<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, h->L[0].corf, n, 1.0);
         } else {
-            one_cycle<double>(h);
+            one_cycle<double>(h, it);
             k_axpy<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(sol, h->L[0].cor, n, 1.0);
         }
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, res0(), 0.0, h->stream);
@@ -891,21 +894,25 @@ static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_r
 // ===========================================================================
 template <int KIND>
 static void launch_post(int sym, const Geom& g, const double* C, const double* u, double* out, int voigt,
-                        hipStream_t s) {
+                        const double* F0, hipStream_t s) {
     const unsigned nb = nblocks(g.tot);
-    if (sym == ISO) k_post<ISO, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
-    else if (sym == MAJORMINOR) k_post<MAJORMINOR, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
-    else k_post<DIAGONAL, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt);
+    if (sym == ISO) k_post<ISO, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt, F0);
+    else if (sym == MAJORMINOR) k_post<MAJORMINOR, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt, F0);
+    else k_post<DIAGONAL, KIND><<<nb, 256, 0, s>>>(g, C, u, out, voigt, F0);
 }
 
-static int post_impl(alamo_mg* h, int amrlev, const double* u, double* out, int voigt, int kind, void* stream) {
+static int post_impl(alamo_mg* h, int amrlev, const double* u, double* out, int voigt, int kind, void* stream,
+                     const double* F0 = nullptr) {
     if (!h || !u || !out) FAIL(ALAMO_ERR_ARG, "null argument");
     if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
     if (kind != POST_STRAIN && !h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
     LevelBuf& l0 = h->L[0];
-    if (kind == POST_STRAIN) launch_post<POST_STRAIN>(h->sym, l0.g, l0.C, u, out, voigt, (hipStream_t)stream);
-    else if (kind == POST_STRESS) launch_post<POST_STRESS>(h->sym, l0.g, l0.C, u, out, voigt, (hipStream_t)stream);
-    else launch_post<POST_ENERGY>(h->sym, l0.g, l0.C, u, out, 0, (hipStream_t)stream);
+    const hipStream_t s = (hipStream_t)stream;
+    if (kind == POST_STRAIN) launch_post<POST_STRAIN>(h->sym, l0.g, l0.C, u, out, voigt, nullptr, s);
+    else if (kind == POST_STRESS) launch_post<POST_STRESS>(h->sym, l0.g, l0.C, u, out, voigt, nullptr, s);
+    else if (kind == POST_ENERGY) launch_post<POST_ENERGY>(h->sym, l0.g, l0.C, u, out, 0, nullptr, s);
+    else if (kind == POST_DW) launch_post<POST_DW>(h->sym, l0.g, l0.C, u, out, 0, F0, s);
+    else launch_post<POST_W>(h->sym, l0.g, l0.C, u, out, 0, F0, s);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -937,6 +944,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->row_align = 0;
     d->fixed_iter = 0;
     d->bottom_solver = ALAMO_BOTTOM_BICGSTAB;
+    d->max_fmg_iter = 100000000;
     return ALAMO_OK;
 }
 
@@ -1068,6 +1076,7 @@ int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     if (d->bottom_solver != h->d.bottom_solver) h->prepared = false;  // the dense inverse is built at prepare
     h->d.bottom_solver = d->bottom_solver;
+    h->d.max_fmg_iter = d->max_fmg_iter;
     h->d.cycle = d->cycle;
     h->d.pre_smooth = d->pre_smooth;
     h->d.post_smooth = d->post_smooth;
@@ -1228,6 +1237,36 @@ int alamo_mg_stress(alamo_mg* h, int amrlev, const double* u, double* sigma, int
 }
 int alamo_mg_energy(alamo_mg* h, int amrlev, const double* u, double* energy, void* stream) {
     return post_impl(h, amrlev, u, energy, 0, POST_ENERGY, stream);
+}
+int alamo_newton_dw(alamo_mg* h, int amrlev, const double* u, const double* F0, double* dw, void* stream) {
+    return post_impl(h, amrlev, u, dw, 0, POST_DW, stream, F0);
+}
+int alamo_newton_w(alamo_mg* h, int amrlev, const double* u, const double* F0, double* w, void* stream) {
+    return post_impl(h, amrlev, u, w, 0, POST_W, stream, F0);
+}
+
+int alamo_mg_set_uniform(alamo_mg* h, int uniform) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    h->uniform = uniform != 0;
+    for (LevelBuf& lb : h->L) lb.g.uniform = h->uniform ? 1 : 0;
+    return ALAMO_OK;
+}
+
+int alamo_mg_error0x(alamo_mg* h, int amrlev, int mglev, double* R0x, const double* x, void* stream) {
+    TRY(check_level(h, amrlev, mglev));
+    if (!R0x || !x) FAIL(ALAMO_ERR_ARG, "null field");
+    LevelBuf& lb = h->L[mglev];
+    const int64_t n = 3 * lb.g.tot;
+    const hipStream_t s = (hipStream_t)stream;
+    double* d0x = nullptr;
+    HIPCHK(hipMallocAsync((void**)&d0x, n * sizeof(double), s));
+    // D0x = x / D, R0x = A D0x, R0x = x - R0x
+    HIPCHK(hipMemcpyAsync(d0x, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_divide<<<gs_blocks(n), 256, 0, s>>>(d0x, lb.D, n);
+    launch_node<double>(h->sym, RESID, lb.g, h->bc, lb.C, d0x, x, R0x, 0.0, s);
+    HIPCHK(hipFreeAsync(d0x, s));
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
 }
 
 int alamo_blend_models(int64_t nnodes, int nmodels, int nrec, const double* eta, const double* table, double* out,

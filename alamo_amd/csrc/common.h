@@ -50,6 +50,8 @@ struct Geom {
     const double* gtab;
     const float* gtabf;
     int ngrains;
+    // Elastic::SetUniform(true): the interior rows omit grad(C) . grad(u)
+    int uniform;
 };
 
 // Boundary types per region and component, packed for kernel arguments.

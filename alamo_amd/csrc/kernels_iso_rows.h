@@ -54,6 +54,15 @@ static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row
 #endif
 template <typename T> struct Rows64Cfg { static constexpr int RW = ROWS_RW, KCH = ROWS_KCH; };
 template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_F32, KCH = ALAMO_ROWS64_KCH_F32; };
+// Tile shape for levels of >= ROWS64_BIG cells in y and z, where taller or deeper tiles still
+// give every CU several workgroups.  Measured at 257^3 nodes (the 4 x 16 shape in brackets):
+// FP32 8 x 16: Jacobi 217 us (227), residual 210 (212), apply 161 (173); FP64 Jacobi and
+// residual 4 x 64: 413 (421) and 381 (394); FP64 apply 8 x 32: 297 (305).  Every shape writes
+// the same values bit for bit.
+constexpr int ROWS64_BIG = 192;
+template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 64; };
+template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 32; };
+template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 16; };
 
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
@@ -502,6 +511,12 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
     dim3 blk(64, ROWS_RW);
     if (g_rows_tile64) {
         // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
+        if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG) {
+            constexpr int RW = Rows64Big<T, MODE>::RW, KCH = Rows64Big<T, MODE>::KCH;
+            dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
+            k_iso_rows64<T, MODE, FOLD, RW, KCH><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
+            return;
+        }
         constexpr int RW = Rows64Cfg<T>::RW, KCH = Rows64Cfg<T>::KCH;
         dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
         k_iso_rows64<T, MODE, FOLD><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);

@@ -216,6 +216,7 @@ __device__ __forceinline__ void apply_interior(const Geom& g, const T* __restric
                 for (int ll = 0; ll < 3; ll++) s += C4<SYM>(c, ii, jj, kk, ll) * gg[kk][ll][jj];
         f[ii] = s;
     }
+    if (g.uniform) return;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         T dc[NC];

@@ -6,17 +6,22 @@
 // with the GetStencil gradients (one-sided on the domain's faces).  Tensors are written as 9
 // SoA planes in row-major order, or as 6 Voigt planes 00, 11, 22, 12, 20, 01.
 //
+//   Newton::DW       sig = C : (grad u - F0)                  (Solver/Nonlocal/Newton.H:571-611)
+//   Newton::W        w = (grad u - F0) : sig / 2              (Solver/Nonlocal/Newton.H:528-569)
+// the last two for an affine model (Model/Solid/Affine/Isotropic.H:38-45, kinematic variable
+// grad u), F0 9 values per node (may be null), DW as 9 row-major planes, no psi factor.
+//
 //   Integrator::Mechanics::UpdateModel  model = sum_n eta_n model_n   (Integrator/Mechanics.H:242-254)
 #pragma once
 #include "node_ops.h"
 
 namespace alamo {
 
-enum PostKind { POST_STRAIN = 0, POST_STRESS = 1, POST_ENERGY = 2 };
+enum PostKind { POST_STRAIN = 0, POST_STRESS = 1, POST_ENERGY = 2, POST_DW = 3, POST_W = 4 };
 
 template <int SYM, int KIND>
 __global__ void __launch_bounds__(256) k_post(Geom g, const double* __restrict__ C, const double* __restrict__ u,
-                                              double* __restrict__ out, int voigt) {
+                                              double* __restrict__ out, int voigt, const double* __restrict__ F0) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= g.tot) return;
     int i, j, k;
@@ -29,6 +34,24 @@ __global__ void __launch_bounds__(256) k_post(Geom g, const double* __restrict__
     for (int p = 0; p < 3; p++)
         for (int d = 0; d < 3; d++) G[p][d] = d1(u + p * g.tot, m, st[d], idx[d], hi[d], g.dx[d]);
     double T[3][3];
+    if (KIND == POST_DW || KIND == POST_W) {
+        if (F0)
+            for (int p = 0; p < 3; p++)
+                for (int d = 0; d < 3; d++) G[p][d] -= F0[9 * m + 3 * p + d];
+        double c[NCoef<SYM>::value];
+        load_coef<double, SYM>(C, g.tot, m, c);
+        c_dot<SYM>(c, G, T);
+        if (KIND == POST_W) {
+            double w = 0.0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) w += G[a][b] * T[a][b];
+            out[m] = 0.5 * w;
+        } else {
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) out[(3 * a + b) * g.tot + m] = T[a][b];
+        }
+        return;
+    }
     if (KIND == POST_STRAIN) {
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) T[a][b] = 0.5 * (G[a][b] + G[b][a]);

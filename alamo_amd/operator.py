@@ -110,9 +110,9 @@ class Elastic:
     def UpdateOpts(self, **kw):
         """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
         sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose,
-        bottom_solver)."""
+        bottom_solver, max_fmg_iter)."""
         allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
-                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver"}
+                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver", "max_fmg_iter"}
         for k, v in kw.items():
             if k not in allowed:
                 raise ValueError("not a solver option: " + k)
@@ -262,6 +262,47 @@ class Elastic:
         """Newton<T>::prepareForSolve residual form (Newton.H:144-231)."""
         check(lib().alamo_newton_rhs(self.h, _ptr(u), _ptr(b), _ptr(F0), _ptr(rhs)))
 
+    # --- flags and queries of the operator (Elastic.H:48,104-107; Operator.H:57-59,96-101) -----
+    def SetUniform(self, uniform):
+        """Elastic::SetUniform: True drops grad(C) . grad(u) from the interior rows (Elastic.cpp:245)."""
+        check(lib().alamo_mg_set_uniform(self.h, int(bool(uniform))))
+        self.uniform = bool(uniform)
+
+    def SetHomogeneous(self, homogeneous):
+        """Elastic::SetHomogeneous: stored; the reference never reads m_homogeneous (Elastic.H:48,172)."""
+        self.homogeneous = bool(homogeneous)
+
+    def SetTesting(self, testing):
+        """Elastic::SetTesting: stored; m_testing is not read by the operator (Elastic.H:104,170)."""
+        self.testing = bool(testing)
+
+    def SetAverageDownCoeffs(self, flag):
+        """Elastic::SetAverageDownCoeffs: the coarse-AMR-level overwrite of Elastic.cpp:611; with the
+        one AMR level of this handle it has nothing to overwrite."""
+        self.average_down_coeffs = bool(flag)
+
+    def SetNormalizeDDW(self, flag):
+        """Operator::SetNormalizeDDW: stored; m_normalize_ddw is not read (Operator.H:59,124)."""
+        self.normalize_ddw = bool(flag)
+
+    def isSingular(self, amrlev):
+        """Operator<Node>::isSingular (Operator.H:96): m_is_bottom_singular, always false."""
+        return False
+
+    def isBottomSingular(self):
+        return False
+
+    def getNGrow(self, amrlev=0, mglev=0):
+        """Operator<Node>::getNGrow (Operator.H:101)"""
+        return 2
+
+    def getNComp(self):
+        return 3
+
+    def Error0x(self, amrlev, mglev, R0x, x, stream=None):
+        """Elastic::Error0x (Elastic.cpp:382-404): R0x = x - A (x / D)."""
+        check(lib().alamo_mg_error0x(self.h, amrlev, mglev, _ptr(R0x), _ptr(x), _ptr(stream)))
+
 
 class Linear:
     """Solver::Nonlocal::Linear (Linear.H:18-389)."""
@@ -310,6 +351,42 @@ class Linear:
         """solver.bottom_solver (Linear.H:280-282): "bicgstab", "smoother" or "dense"."""
         self._opt(bottom_solver=name)
 
+    def setMaxFmgIter(self, n):
+        """amrex::MLMG::setMaxFmgIter: F-cycles for the first n iterations, then V-cycles (the
+        reference's own setting is 1e8, Linear.H:261)."""
+        self._opt(max_fmg_iter=int(n))
+
+    # Linear::Parse (Linear.H:296-389): the solver.* keys of the reference's input file, given
+    # here as a mapping (key -> value).  final_smooth, bottom_smooth, bottom_tol_abs and the
+    # cg_* keys have no counterpart in this handle and are rejected; dump_on_fail /
+    # abort_on_fail are kept as attributes (a failed solve raises unless raise_on_fail=False).
+    _PARSE_KEYS = {"max_iter": int, "bottom_max_iter": int, "max_fmg_iter": int, "fixed_iter": int,
+                   "verbose": int, "pre_smooth": int, "post_smooth": int, "bottom_solver": str,
+                   "bottom_tol_rel": float, "omega": float}
+
+    def Parse(self, pp):
+        rest = dict(pp)
+        opts = {}
+        for k in list(rest):
+            if k in self._PARSE_KEYS:
+                v = self._PARSE_KEYS[k](rest.pop(k))
+                if k == "omega":
+                    self.linop.SetOmega(v)
+                elif not (isinstance(v, int) and v < 0):  # -1: keep the default, as Linear.H:263-278
+                    opts[k] = v
+        for k in ("tol_rel", "tol_abs", "dump_on_fail", "abort_on_fail", "average_down_coeffs", "normalize_ddw"):
+            if k in rest:
+                setattr(self, k, rest.pop(k))
+        if getattr(self, "average_down_coeffs", False):
+            self.linop.SetAverageDownCoeffs(True)
+        if getattr(self, "normalize_ddw", False):
+            self.linop.SetNormalizeDDW(True)
+        if rest:
+            raise ValueError("solver keys not supported by this handle: %s" % sorted(rest))
+        if opts:
+            self._opt(**opts)
+        return self
+
     def apply(self, rhs, sol):
         """Linear::apply (Linear.H:169-174): rhs = A sol on the finest level"""
         self.linop.Fapply(0, 0, rhs, sol)
@@ -331,6 +408,79 @@ class Linear:
         out = st.as_dict()
         out["rc"] = rc
         return out
+
+
+class Newton(Linear):
+    """Solver::Nonlocal::Newton<T> (Newton.H:14-654) for an affine model: stiffness C (the model's
+    DDW, any layout SetModel takes) and eigenstrain F0 (9 AoS doubles per node, or None), both
+    device fields.  Each Newton-Raphson iteration sets the model, forms rhs = b - div DW(grad u)
+    (boundary rows b - BC(u, grad u, DW)), solves A du = rhs with the linear solver and adds du
+    to u; du starts at zero once and each later linear solve starts from the previous du, as in
+    the reference (Newton.H:279-358)."""
+
+    def __init__(self, op=None):
+        super().__init__(op)
+        self.m_nriters = 1         # Newton.H:630
+        self.m_nrtolerance = 0.0   # Newton.H:631
+        self.m_psi = None
+
+    def setNRIters(self, n):
+        self.m_nriters = int(n)
+
+    def setPsi(self, psi, psi_small=1e-8):
+        """Newton::setPsi (Newton.H:46-50): the operator's psi field too."""
+        self.m_psi = psi
+        self.linop.SetPsi(psi, psi_small)
+
+    def _prepare(self, u, b, C, F0, rhs, layout):
+        # the operator keeps psi across SetModel, so the right-hand side takes the psi form
+        # (Newton.H:236-266) once setPsi has been called
+        self.linop.SetModel(C, layout)
+        self.linop.NewtonRHS(u, b, F0, rhs)
+
+    def solve(self, u, b, C, F0, tol_rel, tol_abs, layout=LAYOUT_AOS, raise_on_fail=True):
+        """Newton::solve: returns the relative correction norm of the last iteration (0 when
+        m_nrtolerance never stopped it early), u updated in place."""
+        import torch
+        if self.linop is None:
+            raise _lib.AlamoError(3, "Solver not defined")
+        dsol = torch.zeros_like(u)
+        rhs = torch.zeros_like(b)
+        self.last_stats = []
+        for _ in range(self.m_nriters):
+            self._prepare(u, b, C, F0, rhs, layout)
+            self.last_stats.append(Linear.solve(self, dsol, rhs, tol_rel, tol_abs, raise_on_fail))
+            relnorm = float(dsol.abs().max())  # solnorm stays 0 in the reference (Newton.H:330-346)
+            u.add_(dsol)
+            if relnorm < self.m_nrtolerance:
+                return relnorm
+        return 0.0
+
+    def compResidual(self, res, u, b, C, F0, layout=LAYOUT_AOS):
+        """Newton::compResidual (Newton.H:445-500): res = b - div DW(grad u), boundary rows
+        b - BC(u, grad u, DW)."""
+        self._prepare(u, b, C, F0, res, layout)
+
+    def compLinearSolverResidual(self, res, u, b):
+        """Newton::compLinearSolverResidual (Newton.H:502-526): res = b - A u."""
+        self.linop.solutionResidual(0, res, u, b)
+
+    def W(self, w, u, F0):
+        """Newton::W (Newton.H:528-569): w = (grad u - F0) : C : (grad u - F0) / 2 per node."""
+        check(lib().alamo_newton_w(self.linop.h, 0, _ptr(u), _ptr(F0), _ptr(w), None))
+
+    def DW(self, dw, u, F0):
+        """Newton::DW (Newton.H:571-611): dw = C : (grad u - F0), 9 row-major planes."""
+        check(lib().alamo_newton_dw(self.linop.h, 0, _ptr(u), _ptr(F0), _ptr(dw), None))
+
+    def Parse(self, pp):
+        """Newton::Parse (Newton.H:643-652): Linear::Parse, then nriters and nrtolerance."""
+        rest = dict(pp)
+        if "nriters" in rest:
+            self.m_nriters = int(rest.pop("nriters"))
+        if "nrtolerance" in rest:
+            self.m_nrtolerance = float(rest.pop("nrtolerance"))
+        return Linear.Parse(self, rest)
 
 
 def blend_models(eta, table, out, layout=LAYOUT_AOS, stream=None):

@@ -78,6 +78,9 @@ typedef struct {
                                   (amrex::MLMG::setFixedIter); 0 off */
     int bottom_solver;         /* ALAMO_BOTTOM_*; the reference's "cg" choice is not built (the
                                   operator is not symmetric) */
+    int max_fmg_iter;          /* solver.max_fmg_iter (Linear.H:261,264): with cycle F, the first
+                                  max_fmg_iter iterations are F-cycles and the rest V-cycles
+                                  (amrex::MLMG::setMaxFmgIter); the reference's 1e8 is the default */
 } alamo_mg_desc;
 
 typedef struct {
@@ -129,6 +132,16 @@ int alamo_mg_set_model_grains_host(alamo_mg* h, const uint16_t* gid, const doubl
 int alamo_mg_strain(alamo_mg* h, int amrlev, const double* u, double* eps, int voigt, void* stream);
 int alamo_mg_stress(alamo_mg* h, int amrlev, const double* u, double* sigma, int voigt, void* stream);
 int alamo_mg_energy(alamo_mg* h, int amrlev, const double* u, double* energy, void* stream);
+/* Solver::Nonlocal::Newton::DW / W (src/Solver/Nonlocal/Newton.H:528-611) for the affine model of
+ * the handle's stiffness and eigenstrain F0 (Model/Solid/Affine/Isotropic.H:38-45, kinematic variable
+ * grad u, GetStencil gradients): dw = C : (grad u - F0) as 9 row-major planes; w = (grad u - F0) : dw / 2
+ * per node (written).  F0: 9 AoS doubles per node, or NULL for F0 = 0.  Device pointers. */
+int alamo_newton_dw(alamo_mg* h, int amrlev, const double* u, const double* F0, double* dw, void* stream);
+int alamo_newton_w(alamo_mg* h, int amrlev, const double* u, const double* F0, double* w, void* stream);
+/* Elastic::SetUniform (src/Operator/Elastic.H:105, used at Elastic.cpp:245): nonzero drops the
+ * grad(C) . grad(u) term of the interior rows on every level (the reference's default and the
+ * integrators' setting is 0). */
+int alamo_mg_set_uniform(alamo_mg* h, int uniform);
 
 /* Integrator::Base::Mechanics::Integrate, trac_hi[0] (src/Integrator/Base/Mechanics.H:391-448, the
  * thermo output of tension tests): the x-hi face integral of sigma e_x, each face cell taking the mean
@@ -173,6 +186,8 @@ int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* st
 int alamo_mg_normalize(alamo_mg* h, int amrlev, int mglev, double* x, void* stream);
 /* Operator<Grid::Node>::Fsmooth (Operator.cpp:88-154): sweeps_per_smooth damped-Jacobi sweeps. */
 int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream);
+/* Elastic::Error0x (src/Operator/Elastic.cpp:382-404): R0x = x - A (x / D) on level mglev. */
+int alamo_mg_error0x(alamo_mg* h, int amrlev, int mglev, double* R0x, const double* x, void* stream);
 /* solutionResidual / correctionResidual (Operator.cpp:635-657): r = b - A x. */
 int alamo_mg_residual(alamo_mg* h, int amrlev, int mglev, double* r, const double* x, const double* b,
                       void* stream);

@@ -58,6 +58,8 @@ struct Level {
     // m_psi_small; psi == nullptr is the reference's m_psi_set == false
     const double* psi = nullptr;
     double psi_small = 1e-8;
+    // Elastic::SetUniform(true) (Elastic.H:105): the interior rows drop grad(C) . grad(u)
+    bool uniform = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -308,8 +310,9 @@ static void fapply_node(const Level& L, int sym, const double* C, const int* bct
             s += C4(sym, c, ii, jj, kk, ll) * gg[kk][ll][jj];
         f[ii] = s;
     }
-    // grad(C) . grad(u), with Stencil<MATRIX4,...>::D = (C(+) - C(-)) * 0.5 / dx  (Elastic.cpp:245-254)
-    for (int d = 0; d < 3; d++) {
+    // grad(C) . grad(u), with Stencil<MATRIX4,...>::D = (C(+) - C(-)) * 0.5 / dx  (Elastic.cpp:245-254),
+    // skipped when m_uniform
+    for (int d = 0; d < 3 && !L.uniform; d++) {
         const double* cp = C + (m + st[d]) * nc;
         const double* cm = C + (m - st[d]) * nc;
         double dc[21];
@@ -850,6 +853,33 @@ static void post(const Level& L, int sym, const double* C, const double* u, int 
             }
 }
 
+// Solver::Nonlocal::Newton::DW / W (Solver/Nonlocal/Newton.H:528-611) for an affine model
+// (Model/Solid/Affine/Isotropic.H:38-45, kinematic variable grad u): sig = C:(grad u - F0),
+// written as 9 values per node row-major (kind 3), and w = (1/2) tr((grad u - F0)^T sig)
+// (kind 4), with the GetStencil gradients (one-sided on the faces).  F0 holds 9 values per node
+// (may be null: F0 = 0).
+static void model_post(const Level& L, int sym, const double* C, const double* F0, const double* u, int kind,
+                       double* out) {
+    const int nc = ncoef(sym);
+    for (int k = 0; k < L.N[2]; k++)
+        for (int j = 0; j < L.N[1]; j++)
+            for (int i = 0; i < L.N[0]; i++) {
+                const int64_t m = L.id(i, j, k);
+                double G[3][3], S[3][3];
+                grad_u(L, u, i, j, k, G);
+                if (F0)
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) G[a][b] -= F0[9 * m + 3 * a + b];
+                c_dot_mat(sym, C + m * nc, G, S);
+                if (kind == 3) {
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) out[(3 * a + b) * L.tot + m] = S[a][b];
+                } else {
+                    double w = 0.0;
+                    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) w += G[a][b] * S[a][b];
+                    out[m] = 0.5 * w;
+                }
+            }
+}
+
 // BC::Operator::Elastic::Constant::Init (BC/Operator/Elastic/Constant.H:197-286),
 // non-periodic: boundary nodes of b take the prescribed values.
 static void bc_init(const Level& L, const int* bct, const double* bcv, int homogeneous, double* b) {
@@ -1011,6 +1041,17 @@ void ora_post(const int* ncell, const double* dx, int sym, const double* C, cons
     L.psi = psi;
     L.psi_small = psi_small;
     post(L, sym, C, u, kind, voigt, out);
+}
+void ora_model_post(const int* ncell, const double* dx, int sym, const double* C, const double* F0, const double* u,
+                    int kind, double* out) {
+    Level L(ncell, dx);
+    model_post(L, sym, C, F0, u, kind, out);
+}
+void ora_fapply_uniform(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* u,
+                        double* f) {
+    Level L(ncell, dx);
+    L.uniform = true;
+    fapply(L, sym, C, bct, u, f);
 }
 // Integrator::Mechanics::UpdateModel blending (Integrator/Mechanics.H:242-254)
 void ora_blend(int64_t nnodes, int nmodels, int nrec, const double* eta, const double* table, double* out, int aos) {

@@ -194,6 +194,32 @@ def post(ncell, dx, sym, C, u, kind, voigt=False, psi=None, psi_small=1e-8, ener
     return out
 
 
+def fapply_uniform(ncell, dx, sym, C, bct, u):
+    """Fapply with Elastic::SetUniform(true): interior rows without grad(C) . grad(u)"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); u = _c(u)
+    f = np.zeros_like(u)
+    lib().ora_fapply_uniform(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(u), _d(f))
+    return f
+
+
+def model_dw(ncell, dx, sym, C, F0, u):
+    """Newton::DW of an affine model: sig = C:(grad u - F0), 9 row-major planes"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); u = _c(u)
+    out = np.zeros(9 * int(np.prod(nodes(ncell))))
+    f0p = None if F0 is None else _c(F0)
+    lib().ora_model_post(_i(ncell), _d(dx), sym, _d(C), None if f0p is None else _d(f0p), _d(u), 3, _d(out))
+    return out
+
+
+def model_w(ncell, dx, sym, C, F0, u):
+    """Newton::W of an affine model: w = (grad u - F0):C:(grad u - F0) / 2 per node"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); u = _c(u)
+    out = np.zeros(int(np.prod(nodes(ncell))))
+    f0p = None if F0 is None else _c(F0)
+    lib().ora_model_post(_i(ncell), _d(dx), sym, _d(C), None if f0p is None else _d(f0p), _d(u), 4, _d(out))
+    return out
+
+
 def blend(eta, table, aos=True):
     eta = _c(eta); table = _c(table)
     nmodels, nrec = table.shape

@@ -1,0 +1,156 @@
+"""GPU parity of the operator flags, Error0x, the Newton solver and the affine model's W / DW.
+
+* SetUniform(True) (Operator/Elastic.cpp:245): Fapply / residual / Fsmooth against the oracle's
+  uniform rows to 1e-12 of the field scale, isotropic (a level large enough for the row kernel,
+  which the flag routes to the node kernels) and MajorMinor; SetUniform(False) restores the
+  default rows bit for bit.
+* Error0x (Elastic.cpp:382-404): x - A (x / D) against the oracle's Fapply and Diagonal.
+* Newton::DW / W (Newton.H:528-611): random u, node-varying C and F0, against the oracle.
+* Newton::solve (Newton.H:279-358) on the Eshelby problem: the oracle's multigrid solution of
+  the oracle's Newton right-hand side to 1e-8 relative L2, for one and for two Newton
+  iterations (the second solve starts from the first correction); compResidual is the oracle's
+  Newton right-hand side at the same state, compLinearSolverResidual is b - A u.
+* max_fmg_iter (amrex::MLMG::setMaxFmgIter): 0 with cycle F runs the V-cycle solve bit for bit;
+  2 converges to the oracle's solution.
+"""
+import numpy as np
+import pytest
+
+from alamo_amd import problems as P
+from alamo_amd.operator import Elastic, Linear, Newton, LAYOUT_AOS, CYCLE_F, CYCLE_V
+from helpers import random_iso, random_mm, random_bc, to_dev, to_host, rng, eshelby_rhs
+
+pytestmark = pytest.mark.gpu
+
+
+def close(gpu, ref, rtol=1e-12):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(gpu - ref).max()
+    assert err <= rtol * scale, "max err %.3e vs scale %.3e" % (err, scale)
+
+
+@pytest.mark.parametrize("sym,ncell", [(P.ISO, (128, 100, 80)), (P.ISO, (12, 10, 9)), (P.MAJORMINOR, (11, 7, 9))])
+def test_uniform_rows(cuda, ora, sym, ncell):
+    print("uniform rows", sym, ncell, flush=True)
+    nodes = int(np.prod([n + 1 for n in ncell]))
+    dx = tuple(1.0 / n for n in ncell)
+    C = random_iso(nodes, 3) if sym == P.ISO else random_mm(nodes, 3)
+    bct, bcv = random_bc(21)
+    op = Elastic(ncell, dx, sym=sym, bc=(bct, bcv))
+    op.SetModel(C, LAYOUT_AOS)
+    u = rng(4).standard_normal(3 * nodes)
+    ud = to_dev(u, cuda)
+    f0 = to_dev(np.zeros_like(u), cuda)
+    op.Fapply(0, 0, f0, ud)
+    op.SetUniform(True)
+    f = to_dev(np.zeros_like(u), cuda)
+    op.Fapply(0, 0, f, ud)
+    ref = ora.fapply_uniform(ncell, dx, sym, C, bct, u)
+    close(to_host(f), ref)
+    # residual and one Fsmooth call through the same rows
+    b = rng(5).standard_normal(3 * nodes)
+    r = to_dev(np.zeros_like(u), cuda)
+    op.correctionResidual(0, 0, r, ud, to_dev(b, cuda))
+    close(to_host(r), b - ref)
+    op.SetUniform(False)
+    f1 = to_dev(np.zeros_like(u), cuda)
+    op.Fapply(0, 0, f1, ud)
+    assert np.array_equal(to_host(f1), to_host(f0))
+
+
+def test_error0x(cuda, ora):
+    ncell = (12, 10, 9)
+    nodes = 13 * 11 * 10
+    dx = tuple(1.0 / n for n in ncell)
+    C = random_iso(nodes, 8)
+    # boundary types whose rows have a nonzero diagonal (x / D is then finite everywhere)
+    bct, bcv = random_bc(9, types=(P.DISPLACEMENT, P.TRACTION, P.NEUMANN))
+    op = Elastic(ncell, dx, sym=P.ISO, bc=(bct, bcv))
+    op.SetModel(C, LAYOUT_AOS)
+    op.prepareForSolve()
+    x = rng(10).standard_normal(3 * nodes)
+    R = to_dev(np.zeros_like(x), cuda)
+    op.Error0x(0, 0, R, to_dev(x, cuda))
+    D = ora.diagonal(ncell, dx, P.ISO, C, bct)
+    ref = x - ora.fapply(ncell, dx, P.ISO, C, bct, x / D)
+    close(to_host(R), ref)
+
+
+@pytest.mark.parametrize("sym", [P.ISO, P.MAJORMINOR])
+def test_model_dw_w(cuda, ora, sym):
+    ncell = (10, 9, 12)
+    nodes = 11 * 10 * 13
+    dx = tuple(1.0 / n for n in ncell)
+    C = random_iso(nodes, 12) if sym == P.ISO else random_mm(nodes, 12)
+    u = rng(13).standard_normal(3 * nodes)
+    F0 = rng(14).standard_normal(9 * nodes) * 0.1
+    op = Elastic(ncell, dx, sym=sym)
+    op.SetModel(C, LAYOUT_AOS)
+    nw = Newton(op)
+    dw = to_dev(np.zeros(9 * nodes), cuda)
+    w = to_dev(np.zeros(nodes), cuda)
+    for f0 in (F0, None):
+        nw.DW(dw, to_dev(u, cuda), None if f0 is None else to_dev(f0, cuda))
+        nw.W(w, to_dev(u, cuda), None if f0 is None else to_dev(f0, cuda))
+        close(to_host(dw), ora.model_dw(ncell, dx, sym, C, f0, u))
+        close(to_host(w), ora.model_w(ncell, dx, sym, C, f0, u))
+
+
+def test_newton_solve(cuda, ora):
+    n = 32
+    pr = P.eshelby_problem(n)
+    bct, bcv = P.bc_constant()
+    N = (n + 1) ** 3
+    b = ora.bc_init(pr["ncell"], bct, bcv)
+    rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, np.zeros(3 * N), b)
+    x_ora, it, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bct, rhs, tol_rel=1e-12)
+    assert it > 0
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(bct, bcv))
+    nw = Newton(op).Parse({"max_iter": 100, "nriters": 1, "nrtolerance": 0.0, "bottom_solver": "bicgstab"})
+    u = to_dev(np.zeros(3 * N), cuda)
+    C, F0, bd = to_dev(pr["C"], cuda), to_dev(pr["F0"], cuda), to_dev(b, cuda)
+    assert nw.solve(u, bd, C, F0, 1e-12, 0.0) == 0.0
+    x = to_host(u)
+    err = np.linalg.norm(x - x_ora) / np.linalg.norm(x_ora)
+    print("newton vs oracle rel L2 %.2e" % err)
+    assert err <= 1e-8
+    # compResidual: the Newton right-hand side b - div DW(grad u) of the solved state.  Its
+    # divergence stencil is not Fapply's (Stencil.H:588-657), so it is not the linear residual
+    res = to_dev(np.zeros(3 * N), cuda)
+    nw.compResidual(res, u, bd, C, F0)
+    close(to_host(res), ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, x, b))
+    # two Newton iterations from u = 0 against the same steps on the oracle: the second linear
+    # solve starts from the first correction (dsol is zeroed once, Newton.H:313)
+    nw.setNRIters(2)
+    u2 = to_dev(np.zeros(3 * N), cuda)
+    nw.solve(u2, bd, C, F0, 1e-12, 0.0)
+    rhs1 = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], bct, x_ora, b)
+    d2, it2, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bct, rhs1, x0=x_ora, tol_rel=1e-12)
+    assert it2 > 0
+    ref2 = x_ora + d2
+    err2 = np.linalg.norm(to_host(u2) - ref2) / np.linalg.norm(ref2)
+    print("two Newton iterations vs oracle rel L2 %.2e (second step moved u by %.2e)"
+          % (err2, np.linalg.norm(d2) / np.linalg.norm(x_ora)))
+    assert err2 <= 1e-8
+    # compLinearSolverResidual = b - A u, on a random field
+    xr = rng(15).standard_normal(3 * N)
+    lres = to_dev(np.zeros(3 * N), cuda)
+    nw.compLinearSolverResidual(lres, to_dev(xr, cuda), to_dev(rhs, cuda))
+    close(to_host(lres), rhs - ora.fapply(pr["ncell"], pr["dx"], P.ISO, pr["C"], bct, xr))
+
+
+def test_max_fmg_iter(cuda, ora):
+    pr, bc, rhs = eshelby_rhs(ora, 32)
+    sols = {}
+    for name, cyc, fmg in (("v", CYCLE_V, 0), ("f0", CYCLE_F, 0), ("f2", CYCLE_F, 2)):
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, cycle=cyc)
+        op.SetModel(pr["C"])
+        lin = Linear(op)
+        lin.setMaxFmgIter(fmg)
+        sol = to_dev(np.zeros_like(rhs), cuda)
+        st = lin.solve(sol, to_dev(rhs, cuda), 1e-11, 0.0)
+        sols[name] = (to_host(sol), st["iterations"])
+        print("  %s: %d iterations" % (name, st["iterations"]))
+    assert sols["v"][1] == sols["f0"][1] and np.array_equal(sols["v"][0], sols["f0"][0])
+    x_ora, _, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-12)
+    assert np.linalg.norm(sols["f2"][0] - x_ora) / np.linalg.norm(x_ora) <= 1e-8

@@ -23,6 +23,7 @@ CASES = [
     (P.ISO, (70, 33, 40)),
     (P.ISO, (128, 100, 80)),
     (P.ISO, (130, 97, 83)),
+    (P.ISO, (100, 200, 196)),  # >= 192 cells in y and z: the taller / deeper row-kernel tiles
     (P.MAJORMINOR, (8, 8, 8)),
     (P.MAJORMINOR, (11, 7, 9)),
     (P.DIAGONAL, (9, 8, 10)),
@@ -99,7 +100,7 @@ def test_diagonal_residual_fsmooth(cuda, ora, sym, ncell):
     close(to_host(xn), x / Dref)
 
 
-@pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180)])
+@pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180), (100, 200, 196)])
 def test_iso_dirichlet_x_faces(cuda, ora, ncell):
     """Dirichlet x faces in all components: the row kernel writes those face rows itself and
     the boundary kernel covers the y and z faces (their edges keep traction/Neumann rows)."""
