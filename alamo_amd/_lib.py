@@ -70,6 +70,8 @@ SIGNATURES = {
     "alamo_newton_dw": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_newton_w": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_set_uniform": (c_int, [c_void_p, c_int]),
+    "alamo_newton_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
+                                   P(SolveStats), P(c_double)]),
     "alamo_mg_trac_xhi": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),
     "alamo_blend_models": (c_int, [ctypes.c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "alamo_mg_set_psi": (c_int, [c_void_p, c_int, c_void_p, c_double]),

@@ -1430,6 +1430,41 @@ int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_r
     return solve_impl(h, sol, rhs, tol_rel, tol_abs, stats);
 }
 
+int alamo_newton_solve(alamo_mg* h, double* u, const double* b, const double* F0, int nriters, double nrtolerance,
+                       double tol_rel, double tol_abs, alamo_solve_stats* stats, double* relnorm) {
+    if (!h || !u || !b) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    if (nriters < 0) FAIL(ALAMO_ERR_ARG, "nriters must be >= 0");
+    const int64_t n = 3 * h->L[0].g.tot;
+    double *dsol = nullptr, *rhs = nullptr;
+    HIPCHK(hipMallocAsync((void**)&dsol, n * sizeof(double), h->stream));
+    HIPCHK(hipMallocAsync((void**)&rhs, n * sizeof(double), h->stream));
+    vfill<double>(dsol, n, 0.0, h->stream);  // zeroed once (Newton.H:313)
+    vfill<double>(rhs, n, 0.0, h->stream);   // pads of a pitched layout stay 0
+    int rc = ALAMO_OK;
+    double rel = 0.0;
+    for (int it = 0; it < nriters && rc == ALAMO_OK; it++) {
+        rc = alamo_newton_rhs(h, u, b, F0, rhs);
+        if (rc == ALAMO_OK) rc = solve_impl(h, dsol, rhs, tol_rel, tol_abs, stats);
+        if (rc != ALAMO_OK) break;
+        double cor = 0.0;
+        rc = norms_dev<double>(h, dsol, n, &cor, nullptr);  // max |du|; solnorm stays 0 (Newton.H:330-346)
+        if (rc != ALAMO_OK) break;
+        k_axpy<double, double><<<gs_blocks(n), 256, 0, h->stream>>>(u, dsol, n, 1.0);
+        if (cor < nrtolerance) {
+            rel = cor;
+            break;
+        }
+    }
+    const std::string err = g_err;
+    (void)hipFreeAsync(dsol, h->stream);
+    (void)hipFreeAsync(rhs, h->stream);
+    if (hipStreamSynchronize(h->stream) != hipSuccess && rc == ALAMO_OK) FAIL(ALAMO_ERR_HIP, "hipStreamSynchronize failed");
+    g_err = err;
+    if (relnorm) *relnorm = rel;
+    return rc;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

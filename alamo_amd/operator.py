@@ -413,10 +413,10 @@ class Linear:
 class Newton(Linear):
     """Solver::Nonlocal::Newton<T> (Newton.H:14-654) for an affine model: stiffness C (the model's
     DDW, any layout SetModel takes) and eigenstrain F0 (9 AoS doubles per node, or None), both
-    device fields.  Each Newton-Raphson iteration sets the model, forms rhs = b - div DW(grad u)
-    (boundary rows b - BC(u, grad u, DW)), solves A du = rhs with the linear solver and adds du
-    to u; du starts at zero once and each later linear solve starts from the previous du, as in
-    the reference (Newton.H:279-358)."""
+    device fields.  solve sets the model, then each Newton-Raphson iteration (alamo_newton_solve)
+    forms rhs = b - div DW(grad u) (boundary rows b - BC(u, grad u, DW)), solves A du = rhs with
+    the linear solver and adds du to u; du starts at zero once and each later linear solve starts
+    from the previous du, as in the reference (Newton.H:279-358)."""
 
     def __init__(self, op=None):
         super().__init__(op)
@@ -441,20 +441,18 @@ class Newton(Linear):
     def solve(self, u, b, C, F0, tol_rel, tol_abs, layout=LAYOUT_AOS, raise_on_fail=True):
         """Newton::solve: returns the relative correction norm of the last iteration (0 when
         m_nrtolerance never stopped it early), u updated in place."""
-        import torch
         if self.linop is None:
             raise _lib.AlamoError(3, "Solver not defined")
-        dsol = torch.zeros_like(u)
-        rhs = torch.zeros_like(b)
-        self.last_stats = []
-        for _ in range(self.m_nriters):
-            self._prepare(u, b, C, F0, rhs, layout)
-            self.last_stats.append(Linear.solve(self, dsol, rhs, tol_rel, tol_abs, raise_on_fail))
-            relnorm = float(dsol.abs().max())  # solnorm stays 0 in the reference (Newton.H:330-346)
-            u.add_(dsol)
-            if relnorm < self.m_nrtolerance:
-                return relnorm
-        return 0.0
+        self.linop.SetModel(C, layout)
+        st = SolveStats()
+        rel = ctypes.c_double(0.0)
+        rc = lib().alamo_newton_solve(self.linop.h, _ptr(u), _ptr(b), _ptr(F0), int(self.m_nriters),
+                                      float(self.m_nrtolerance), tol_rel, tol_abs, ctypes.byref(st), ctypes.byref(rel))
+        self.last_stats = st.as_dict()
+        self.last_stats["rc"] = rc
+        if rc != 0 and raise_on_fail:
+            check(rc)
+        return rel.value
 
     def compResidual(self, res, u, b, C, F0, layout=LAYOUT_AOS):
         """Newton::compResidual (Newton.H:445-500): res = b - div DW(grad u), boundary rows

@@ -205,6 +205,15 @@ int alamo_bc_init(alamo_mg* h, double* b, int homogeneous);
  * linear/affine model DW = C:(grad u - F0): rhs = b - div(DW) inside, b - BC(u, grad u, DW)
  * on the boundary.  F0 may be NULL (9 doubles per node, row-major, device pointer). */
 int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs);
+/* Solver::Nonlocal::Newton<T>::solve (src/Solver/Nonlocal/Newton.H:279-358) for the affine model of
+ * the handle's stiffness (set before) and eigenstrain F0 (9 AoS per node, or NULL): nriters times
+ * rhs = alamo_newton_rhs(u, b, F0), du = alamo_linear_solve(du, rhs) (du zeroed once, so each later
+ * solve starts from the previous du), u += du, stopping early when max |du| < nrtolerance.
+ * *relnorm (may be NULL) receives that max |du| on an early stop, else 0, as the reference
+ * returns.  stats (may be NULL): the last linear solve.  u, b device; scratch is allocated on the
+ * handle's stream and freed before return. */
+int alamo_newton_solve(alamo_mg* h, double* u, const double* b, const double* F0, int nriters, double nrtolerance,
+                       double tol_rel, double tol_abs, alamo_solve_stats* stats, double* relnorm);
 
 /* Solver::Nonlocal::Linear::solve (Linear.H:131-147) -> amrex::MLMG::solve: sol is the
  * initial guess on entry and the solution on exit (device pointers, level 0).  Stops when

@@ -109,6 +109,27 @@ public:
     void Energy(int amrlev, double* energy, const double* u) const {
         alamo::check(alamo_mg_energy(h_, amrlev, u, energy, nullptr));
     }
+    // Elastic::Error0x (Elastic.cpp:382-404): R0x = x - A (x / D)
+    void Error0x(int amrlev, int mglev, double* R0x, const double* x) const {
+        alamo::check(alamo_mg_error0x(h_, amrlev, mglev, R0x, x, nullptr));
+    }
+    // Elastic::SetUniform (Elastic.H:105): true drops grad(C) . grad(u) from the interior rows
+    void SetUniform(bool uniform) {
+        uniform_ = uniform;
+        if (h_) alamo::check(alamo_mg_set_uniform(h_, uniform ? 1 : 0));
+    }
+    // stored flags the reference keeps without reading them here (Elastic.H:48,104,106-107;
+    // Operator.H:59): homogeneity, testing, the coarse-AMR coefficient overwrite (one AMR level
+    // here), DDW normalisation
+    void SetHomogeneous(bool a) { homogeneous_ = a; }
+    void SetTesting(bool a) { testing_ = a; }
+    void SetAverageDownCoeffs(bool a) { average_down_coeffs_ = a; }
+    void SetNormalizeDDW(bool a) { normalize_ddw_ = a; }
+    // Operator<Grid::Node>::isSingular / isBottomSingular / getNGrow / getNComp (Operator.H:96-105)
+    bool isSingular(int /*amrlev*/) const { return false; }
+    bool isBottomSingular() const { return false; }
+    int getNGrow(int = 0, int = 0) const { return 2; }
+    int getNComp() const { return 3; }
     int NMGLevels() const { int n = 0; alamo::check(alamo_mg_num_levels(h_, &n)); return n; }
     // push changed solver options of desc() to a built handle (before the build they are read at creation)
     void UpdateOpts() {
@@ -119,10 +140,14 @@ public:
 
 private:
     void ensure() {
-        if (!h_) alamo::check(alamo_mg_create(&desc_, &h_));
+        if (h_) return;
+        alamo::check(alamo_mg_create(&desc_, &h_));
+        if (uniform_) alamo::check(alamo_mg_set_uniform(h_, 1));
     }
     alamo_mg_desc desc_{};
     alamo_mg* h_ = nullptr;
+    bool uniform_ = false, homogeneous_ = false, testing_ = false, average_down_coeffs_ = false,
+         normalize_ddw_ = false;
 };
 
 }  // namespace Operator
@@ -151,6 +176,8 @@ public:
     void setPreSmooth(int n) { opt().pre_smooth = n; push(); }
     void setPostSmooth(int n) { opt().post_smooth = n; push(); }
     void setBottomTolerance(double t) { opt().bottom_tol_rel = t; push(); }
+    // amrex::MLMG::setMaxFmgIter: F-cycles for the first n iterations (the reference sets 1e8)
+    void setMaxFmgIter(int n) { opt().max_fmg_iter = n; push(); }
     // solver.bottom_solver (Linear.H:280-282): "bicgstab", "smoother" or "dense" ("cg" is not built)
     void setBottomSolver(const std::string& name) {
         int b = name == "bicgstab" ? ALAMO_BOTTOM_BICGSTAB
@@ -179,7 +206,7 @@ public:
     }
     const alamo_solve_stats& stats() const { return last_; }
 
-private:
+protected:
     ::Operator::Elastic<SYM>& defined() {
         if (!linop_) throw alamo::Abort(ALAMO_ERR_STATE, "Solver not defined");
         return *linop_;
@@ -188,6 +215,51 @@ private:
     void push() { linop_->UpdateOpts(); }
     ::Operator::Elastic<SYM>* linop_ = nullptr;
     alamo_solve_stats last_{};
+};
+
+// Solver::Nonlocal::Newton<T> (Newton.H:14-654) for an affine model: stiffness C (the model's
+// DDW, device, any SetModel layout) and eigenstrain F0 (9 AoS doubles per node, device, or null).
+template <int SYM>
+class Newton : public Linear<SYM> {
+public:
+    Newton() = default;
+    explicit Newton(::Operator::Elastic<SYM>& op) : Linear<SYM>(op) {}
+    void setNRIters(int n) { nriters_ = n; }           // newton.nriters (Newton.H:646)
+    void setNRTolerance(double t) { nrtolerance_ = t; } // newton.nrtolerance (Newton.H:649)
+    // Newton::setPsi (Newton.H:46-50)
+    void setPsi(const double* psi, double psi_small = 1e-8) { this->defined().SetPsi(0, psi, psi_small); }
+    // Newton::solve (Newton.H:279-358): u updated in place; returns max |du| on an early stop, else 0
+    double solve(double* u, const double* b, const double* C, const double* F0, double tol_rel, double tol_abs,
+                 int layout = ALAMO_LAYOUT_AOS) {
+        this->defined().SetModel(0, C, layout);
+        alamo_solve_stats st{};
+        double rel = 0.0;
+        alamo::check(alamo_newton_solve(this->defined().handle(), u, b, F0, nriters_, nrtolerance_, tol_rel, tol_abs,
+                                        &st, &rel));
+        this->last_ = st;
+        return rel;
+    }
+    // Newton::compResidual (Newton.H:445-500): res = b - div DW(grad u), boundary rows b - BC(...)
+    void compResidual(double* res, const double* u, const double* b, const double* C, const double* F0,
+                      int layout = ALAMO_LAYOUT_AOS) {
+        this->defined().SetModel(0, C, layout);
+        alamo::check(alamo_newton_rhs(this->defined().handle(), u, b, F0, res));
+    }
+    // Newton::compLinearSolverResidual (Newton.H:502-526): res = b - A u
+    void compLinearSolverResidual(double* res, const double* u, const double* b) {
+        alamo::check(alamo_mg_residual(this->defined().handle(), 0, 0, res, u, b, nullptr));
+    }
+    // Newton::W / DW (Newton.H:528-611) of the operator's model
+    void W(double* w, const double* u, const double* F0) {
+        alamo::check(alamo_newton_w(this->defined().handle(), 0, u, F0, w, nullptr));
+    }
+    void DW(double* dw, const double* u, const double* F0) {
+        alamo::check(alamo_newton_dw(this->defined().handle(), 0, u, F0, dw, nullptr));
+    }
+
+private:
+    int nriters_ = 1;
+    double nrtolerance_ = 0.0;
 };
 
 // Outer flexible GMRES around the operator's cycle (no reference counterpart; same Define /

@@ -5,6 +5,8 @@
 // creates a network to check the Net class.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -57,6 +59,21 @@ int main(int argc, char** argv) {
         printf("krylov iterations %d res_inf %.3e converged %d\n", krylov.stats().iterations, r,
                krylov.stats().converged);
         krylov.Clear();
+        // Newton from u = 0 with F0 = 0: its right-hand side is b, so it repeats the linear solve
+        double* du;
+        if (hipMalloc(&du, x.size() * 8)) return 3;
+        hipMemset(du, 0, x.size() * 8);
+        Solver::Nonlocal::Newton<Operator::Isotropic> newton(op);
+        newton.setMaxIter(300);
+        newton.solve(du, db, dC, nullptr, 1e-12, 0.0);
+        std::vector<double> xn(x.size());
+        hipMemcpy(xn.data(), du, x.size() * 8, hipMemcpyDeviceToHost);
+        double dmax = 0.0;
+        for (size_t q = 0; q < x.size(); q++) dmax = std::max(dmax, std::fabs(xn[q] - x[q]));
+        printf("newton iterations %d max diff to linear %.3e\n", newton.stats().iterations, dmax);
+        newton.Clear();
+        hipFree(du);
+        printf("ngrow %d ncomp %d singular %d\n", op.getNGrow(), op.getNComp(), (int)op.isSingular(0));
         alamo::Net net(7);
         printf("net params %lld\n", (long long)net.NumParams());
     } catch (const alamo::Abort& e) {

@@ -36,6 +36,7 @@ def test_facade_solve_matches_oracle(cuda, ora, tmp_path):
     print(res.stdout, res.stderr)
     assert res.returncode == 0 and res.stdout.count("converged 1") == 2 and "net params" in res.stdout
     assert "fixed iterations 2" in res.stdout
+    assert "max diff to linear 0.000e+00" in res.stdout and "ngrow 2 ncomp 3 singular 0" in res.stdout
     x = np.fromfile(out, dtype=np.float64)
     xk = np.fromfile(outk, dtype=np.float64)
     n = 16
