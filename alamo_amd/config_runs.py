@@ -35,6 +35,8 @@ def main():
     p.add_argument("--corr-eval", type=float, default=16.0)
     p.add_argument("--restart", type=int, default=10)
     p.add_argument("--sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
+                   help="bottom solver of the FGMRES V-cycle (plain multigrid keeps bicgstab)")
     p.add_argument("--polycrystal", action="store_true")
     p.add_argument("--poly-n", type=int, default=256)
     a = p.parse_args()
@@ -82,7 +84,8 @@ def main():
     for name, prec, kind in methods:
         krylov = kind is not None
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_V,
-                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=400)
+                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=400,
+                     bottom_solver=a.bottom if krylov else "bicgstab")
         op.SetModel(C, layout=0)
         if rhs is None:
             b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
@@ -127,7 +130,8 @@ def polycrystal(a):
     out, xs, rhs = {}, {}, None
     for name, prec in (("fp64-fgmres", PREC_FP64), ("mixed-fgmres", PREC_MIXED)):
         op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=prec,
-                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, max_iter=400)
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, max_iter=400,
+                     bottom_solver=a.bottom)
         op.SetModelGrains(pr["gid"], pr["table"])
         if rhs is None:
             b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
@@ -150,7 +154,8 @@ def polycrystal(a):
         del op
     out["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(xs["mixed-fgmres"] - xs["fp64-fgmres"]) /
                                         torch.linalg.norm(xs["fp64-fgmres"]))
-    out["workload"] = "polycrystal %d^3 cells, 64 grains (seed 64), cubic 1.68/1.21/0.75, u_z 1e-3, 1e-10" % n
+    out["workload"] = ("polycrystal %d^3 cells, 64 grains (seed 64), cubic 1.68/1.21/0.75, u_z 1e-3, 1e-10, %s bottom"
+                       % (n, a.bottom))
     print(json.dumps({"polycrystal": out}))
 
 
