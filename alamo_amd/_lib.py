@@ -21,7 +21,7 @@ class MGDesc(ctypes.Structure):
                 ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
                 ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
                 ("bottom_tol_rel", c_double), ("verbose", c_int), ("row_align", c_int), ("fixed_iter", c_int),
-                ("bottom_solver", c_int), ("max_fmg_iter", c_int)]
+                ("bottom_solver", c_int), ("max_fmg_iter", c_int), ("sweeps_finest", c_int)]
 
 
 class SolveStats(ctypes.Structure):

@@ -33,8 +33,10 @@ def main():
     p.add_argument("--contrast-train", type=float, default=100.0)
     p.add_argument("--eval-n", type=int, default=512)
     p.add_argument("--corr-eval", type=float, default=16.0)
-    p.add_argument("--restart", type=int, default=10)
+    p.add_argument("--restart", type=int, default=8)
     p.add_argument("--sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--sweeps-finest", type=int, default=1,
+                   help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --sweeps)")
     p.add_argument("--bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES V-cycle (plain multigrid keeps bicgstab)")
     p.add_argument("--polycrystal", action="store_true")
@@ -58,7 +60,7 @@ def main():
     for f in range(a.fields):
         pr = P.two_phase_problem(n=a.train_n, corr_cells=8, contrast=a.contrast_train, seed=1 + f)
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=PREC_MIXED,
-                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps)
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest)
         op.SetModel(pr["C"])
         st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f, solve_max_iter=100, solve_tol=1e-6)
         losses.append((st["first_loss"], st["last_loss"]))
@@ -85,7 +87,7 @@ def main():
         krylov = kind is not None
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_V,
                      pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=400,
-                     bottom_solver=a.bottom if krylov else "bicgstab")
+                     bottom_solver=a.bottom if krylov else "bicgstab", sweeps_finest=a.sweeps_finest if krylov else 0)
         op.SetModel(C, layout=0)
         if rhs is None:
             b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
@@ -131,7 +133,7 @@ def polycrystal(a):
     for name, prec in (("fp64-fgmres", PREC_FP64), ("mixed-fgmres", PREC_MIXED)):
         op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=prec,
                      cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, max_iter=400,
-                     bottom_solver=a.bottom)
+                     bottom_solver=a.bottom, sweeps_finest=a.sweeps_finest)
         op.SetModelGrains(pr["gid"], pr["table"])
         if rhs is None:
             b = torch.zeros(3 * N, dtype=torch.float64, device=dev)

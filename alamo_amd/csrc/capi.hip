@@ -271,9 +271,14 @@ static void sweeps(alamo_mg* h, int l, int count) {
     }
 }
 
+// Jacobi sweeps per Fsmooth call on level l of a cycle (desc.sweeps_finest on level 0 when set)
+static int sps(const alamo_mg* h, int l) {
+    return (l == 0 && h->d.sweeps_finest > 0) ? h->d.sweeps_finest : h->d.sweeps_per_smooth;
+}
+
 template <typename T>
 static void smooth(alamo_mg* h, int l, int ncalls) {
-    sweeps<T>(h, l, ncalls * h->d.sweeps_per_smooth);
+    sweeps<T>(h, l, ncalls * sps(h, l));
 }
 
 // First damped-Jacobi sweep from a zero correction: A 0 = 0, so x = omega b / D pointwise (the
@@ -295,7 +300,7 @@ __global__ void k_jacobi_from_zero(const Ti* __restrict__ b_in, T* __restrict__ 
 template <typename T>
 static void presmooth_from_zero(alamo_mg* h, int l, bool first_done) {
     LevelBuf& lb = h->L[l];
-    const int total = h->d.pre_smooth * h->d.sweeps_per_smooth;
+    const int total = h->d.pre_smooth * sps(h, l);
     const int64_t n = 3 * lb.g.tot;
     if (total == 0) {
         vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
@@ -586,7 +591,7 @@ static void vcycle_into(alamo_mg* h, const Ti* b, T* z) {
     T* own_cor = LB<T>::cor(l0);
     T* own_tmp = LB<T>::tmp(l0);
     T* own_res = LB<T>::res(l0);
-    const int swaps = (h->d.pre_smooth + h->d.post_smooth) * h->d.sweeps_per_smooth - 1;
+    const int swaps = (h->d.pre_smooth + h->d.post_smooth) * sps(h, 0) - 1;
     if (swaps % 2) LB<T>::tmp(l0) = z;
     else LB<T>::cor(l0) = z;
     T* stage = nullptr;
@@ -945,6 +950,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->fixed_iter = 0;
     d->bottom_solver = ALAMO_BOTTOM_BICGSTAB;
     d->max_fmg_iter = 100000000;
+    d->sweeps_finest = 0;
     return ALAMO_OK;
 }
 
@@ -965,6 +971,7 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
     if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
+    if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -1074,9 +1081,11 @@ int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
     if (d->fixed_iter < 0) FAIL(ALAMO_ERR_ARG, "fixed_iter must be >= 0");
     if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
+    if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
     if (d->bottom_solver != h->d.bottom_solver) h->prepared = false;  // the dense inverse is built at prepare
     h->d.bottom_solver = d->bottom_solver;
     h->d.max_fmg_iter = d->max_fmg_iter;
+    h->d.sweeps_finest = d->sweeps_finest;
     h->d.cycle = d->cycle;
     h->d.pre_smooth = d->pre_smooth;
     h->d.post_smooth = d->post_smooth;

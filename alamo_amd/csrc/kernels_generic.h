@@ -273,59 +273,63 @@ __global__ void __launch_bounds__(256) k_restrict_psi(Geom gc, Geom gf, const do
 }
 
 // fine += I(crse), trilinear nodal prolongation by parity (Operator/Operator.cpp:399-428).
-// One work-item per coarse x-interval of a fine row: it adds the even fine node i = 2I and the
-// odd node 2I + 1 (when I < n_c), so no wavefront diverges on the x parity, and the row parities
-// of j and k are uniform along the row.  Each node keeps its case's formula and term order.
+// One work-item per fine node, i fastest (coalesced loads and stores; a wavefront spans at most
+// two rows, so the j, k parities diverge only where it crosses a row end).  The x parity is per
+// lane, so every lane evaluates the even and the odd formula of its row case and keeps its own.  The odd formula's
+// x + 1 coarse neighbour is read at offset `odd` (0 on even lanes), so no read leaves the level.
+// Each node keeps its case's formula and term order.
 template <typename T>
 __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* __restrict__ crse,
                                                     T* __restrict__ fine) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nt = (int64_t)gc.N[0] * gf.N[1] * gf.N[2];
+    const int64_t nt = (int64_t)gf.N[0] * gf.N[1] * gf.N[2];
     if (t >= nt) return;
-    int I, j, k;
+    int i, j, k;
     if (nt <= 0x7fffffff) {
-        const uint32_t tt = (uint32_t)t, Nc0 = (uint32_t)gc.N[0], N1 = (uint32_t)gf.N[1];
-        const uint32_t row = tt / Nc0, kk = row / N1;
-        I = (int)(tt - row * Nc0);
+        const uint32_t tt = (uint32_t)t, N0 = (uint32_t)gf.N[0], N1 = (uint32_t)gf.N[1];
+        const uint32_t row = tt / N0, kk = row / N1;
+        i = (int)(tt - row * N0);
         k = (int)kk;
         j = (int)(row - kk * N1);
     } else {
-        const int64_t row = t / gc.N[0];
-        I = (int)(t - row * gc.N[0]);
+        const int64_t row = t / gf.N[0];
+        i = (int)(t - row * gf.N[0]);
         k = (int)(row / gf.N[1]);
         j = (int)(row - (int64_t)k * gf.N[1]);
     }
-    const int J = j / 2, K = k / 2;
-    const bool ej = (j & 1) == 0, ek = (k & 1) == 0, odd = I < gc.n[0];
+    const int I = i >> 1, o = i & 1, J = j / 2, K = k / 2;
+    const bool ej = (j & 1) == 0, ek = (k & 1) == 0;
     const int64_t mc = I + gc.sy * J + gc.sz * K;
-    const int64_t mf = 2 * I + gf.sy * j + gf.sz * k;
-    const int64_t sx = 1, sy = gc.sy, sz = gc.sz;
+    const int64_t mf = i + gf.sy * j + gf.sz * k;
+    const int64_t sy = gc.sy, sz = gc.sz;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         const T* Cp = crse + p * gc.tot + mc;
-        auto at = [&](int a, int b, int c) { return Cp[a * sx + b * sy + c * sz]; };
-        T ve, vo = T(0);
-        if (ej && ek) ve = at(0, 0, 0);
-        else if (ek) ve = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
-        else if (ej) ve = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
-        else ve = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
-        if (odd) {
-            if (ej && ek) vo = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
-            else if (ej) vo = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
-            else if (ek) vo = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
-            else
-                vo = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
-                                 at(1, 1, 0) + at(1, 1, 1));
+        // at(a, b, c) with a = 1 read at x offset o
+        auto at = [&](int a, int b, int c) { return Cp[a * o + b * sy + c * sz]; };
+        T ve, vo;
+        if (ej && ek) {
+            ve = at(0, 0, 0);
+            vo = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
+        } else if (ek) {
+            ve = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
+            vo = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
+        } else if (ej) {
+            ve = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
+            vo = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
+        } else {
+            ve = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
+            vo = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
+                             at(1, 1, 0) + at(1, 1, 1));
         }
-        T* Fp = fine + p * gf.tot + mf;
-        Fp[0] += ve;
-        if (odd) Fp[1] += vo;
+        fine[p * gf.tot + mf] += o ? vo : ve;
     }
 }
 
-// launch shape of k_interp_add: one work-item per coarse x node of every fine row
+// launch shape of k_interp_add: one work-item per fine node
 inline unsigned interp_blocks(const Geom& gc, const Geom& gf) {
-    return (unsigned)(((int64_t)gc.N[0] * gf.N[1] * gf.N[2] + 255) / 256);
+    (void)gc;
+    return (unsigned)(((int64_t)gf.N[0] * gf.N[1] * gf.N[2] + 255) / 256);
 }
 
 // ---------------------------------------------------------------------------

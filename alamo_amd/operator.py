@@ -53,7 +53,7 @@ class Elastic:
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
                  bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0, fixed_iter=0,
-                 bottom_solver="bicgstab"):
+                 bottom_solver="bicgstab", sweeps_finest=0):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
         for q in range(3):
@@ -75,6 +75,7 @@ class Elastic:
         d.row_align = int(row_align)
         d.fixed_iter = int(fixed_iter)
         d.bottom_solver = bottom_solver_code(bottom_solver)
+        d.sweeps_finest = int(sweeps_finest)
         self.desc = d
         self.sym = sym
         self.ncell = tuple(int(n) for n in ncell)
@@ -110,9 +111,10 @@ class Elastic:
     def UpdateOpts(self, **kw):
         """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
         sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose,
-        bottom_solver, max_fmg_iter)."""
+        bottom_solver, max_fmg_iter, sweeps_finest)."""
         allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
-                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver", "max_fmg_iter"}
+                   "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver", "max_fmg_iter",
+                   "sweeps_finest"}
         for k, v in kw.items():
             if k not in allowed:
                 raise ValueError("not a solver option: " + k)

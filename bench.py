@@ -32,10 +32,12 @@ def parse():
     p.add_argument("--contrast", type=float, default=1000.0)
     p.add_argument("--seed", type=int, default=256)
     p.add_argument("--fapply-reps", type=int, default=50)
-    p.add_argument("--restart", type=int, default=10, help="FGMRES restart length")
+    p.add_argument("--restart", type=int, default=8, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
     p.add_argument("--krylov-sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--krylov-sweeps-finest", type=int, default=1,
+                   help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --krylov-sweeps)")
     p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES V-cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
@@ -129,6 +131,7 @@ def main():
         nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
                      pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
+                     sweeps_finest=a.krylov_sweeps_finest if krylov else 0,
                      verbose=a.verbose, bottom_solver=a.krylov_bottom if krylov else "bicgstab")
         op.SetModel(pr["C"])
         op.prepareForSolve()
@@ -236,7 +239,9 @@ def main():
                         % (a.corr, a.contrast, a.seed, n, n + 1),
             "method": best,
             "fgmres": {"restart": a.restart, "vcycle_smooth_calls": a.krylov_smooth,
-                       "jacobi_sweeps_per_call": a.krylov_sweeps, "bottom_solver": a.krylov_bottom},
+                       "jacobi_sweeps_per_call": a.krylov_sweeps,
+                       "jacobi_sweeps_per_call_finest": a.krylov_sweeps_finest or a.krylov_sweeps,
+                       "bottom_solver": a.krylov_bottom},
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
@@ -288,7 +293,8 @@ def train_network(a, P, Elastic, Net, precision, cycle):
         vc = cycle == 0
         nsm = a.krylov_smooth if vc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
-                     pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps)
+                     pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps,
+                     sweeps_finest=a.krylov_sweeps_finest)
         op.SetModel(pr["C"])
         stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
                                solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))

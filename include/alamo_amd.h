@@ -81,6 +81,11 @@ typedef struct {
     int max_fmg_iter;          /* solver.max_fmg_iter (Linear.H:261,264): with cycle F, the first
                                   max_fmg_iter iterations are F-cycles and the rest V-cycles
                                   (amrex::MLMG::setMaxFmgIter); the reference's 1e8 is the default */
+    int sweeps_finest;         /* > 0: Jacobi sweeps per Fsmooth call of the cycles on level 0 (the
+                                  other levels keep sweeps_per_smooth); 0: sweeps_per_smooth
+                                  everywhere.  No reference counterpart (Fsmooth runs the same 2
+                                  sweeps on every level, Operator.cpp:107); alamo_mg_fsmooth keeps
+                                  sweeps_per_smooth. */
 } alamo_mg_desc;
 
 typedef struct {

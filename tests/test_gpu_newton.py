@@ -154,3 +154,25 @@ def test_max_fmg_iter(cuda, ora):
     assert sols["v"][1] == sols["f0"][1] and np.array_equal(sols["v"][0], sols["f0"][0])
     x_ora, _, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-12)
     assert np.linalg.norm(sols["f2"][0] - x_ora) / np.linalg.norm(x_ora) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sweeps_finest(cuda, ora, precision):
+    """desc.sweeps_finest: equal to sweeps_per_smooth it changes nothing (bit for bit); 1 sweep on the
+    finest level of the FGMRES V-cycle (the copy-free level-0 ping-pong with an odd sweep count)
+    still converges to the oracle's solution."""
+    from alamo_amd.operator import Krylov, PRECOND_MG
+    pr, bc, rhs = eshelby_rhs(ora, 32)
+    x_ora, _, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-12)
+    sols = {}
+    for fin in (0, 3, 1, 2):
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=precision, cycle=CYCLE_V, pre_smooth=1,
+                     post_smooth=1, sweeps_per_smooth=3, sweeps_finest=fin, bottom_solver="dense")
+        op.SetModel(pr["C"])
+        sol = to_dev(np.zeros_like(rhs), cuda)
+        st = Krylov(op, restart=8, precond=PRECOND_MG).solve(sol, to_dev(rhs, cuda), 1e-11, 0.0)
+        sols[fin] = to_host(sol)
+        err = np.linalg.norm(sols[fin] - x_ora) / np.linalg.norm(x_ora)
+        print("  sweeps_finest %d: %d iterations, rel L2 %.2e" % (fin, st["iterations"], err))
+        assert st["converged"] == 1 and err <= 1e-8
+    assert np.array_equal(sols[0], sols[3])
