@@ -55,14 +55,15 @@ static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row
 template <typename T> struct Rows64Cfg { static constexpr int RW = ROWS_RW, KCH = ROWS_KCH; };
 template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_F32, KCH = ALAMO_ROWS64_KCH_F32; };
 // Tile shape for levels of >= ROWS64_BIG cells in y and z, where taller or deeper tiles still
-// give every CU several workgroups.  Measured at 257^3 nodes (the 4 x 16 shape in brackets):
-// FP32 8 x 16: Jacobi 217 us (227), residual 210 (212), apply 161 (173); FP64 Jacobi and
-// residual 4 x 64: 413 (421) and 381 (394); FP64 apply 8 x 32: 297 (305).  Every shape writes
-// the same values bit for bit.
+// give every CU several workgroups, launched in the XCD-contiguous tile order (neighbouring
+// tiles share their halo rows in one XCD's L2).  Measured at 257^3 nodes in one process, the
+// 4 x 16 shape in the hardware order in brackets: FP32 8 x 32: Jacobi 211 us (233), residual
+// 195 (214), apply 150 (175); FP64 Jacobi and residual 4 x 64: 403 (414) and 378 (388); FP64
+// apply 8 x 32: 290 (305).  Every shape and order writes the same values bit for bit.
 constexpr int ROWS64_BIG = 192;
 template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 64; };
 template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 32; };
-template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 16; };
+template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 32; };
 
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
@@ -273,7 +274,8 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, cons
 // by that tile's lane 63 from its halo value.
 // ---------------------------------------------------------------------------
 constexpr int R64_W = 66;  // LDS row: halo, 64 nodes, halo
-template <typename T, int MODE, int FOLD, int RW = Rows64Cfg<T>::RW, int KCH = Rows64Cfg<T>::KCH>
+template <typename T, int MODE, int FOLD, int RW = Rows64Cfg<T>::RW, int KCH = Rows64Cfg<T>::KCH,
+          bool XCD = (ALAMO_ROWS_XCD != 0)>
 __global__ void __launch_bounds__(64 * RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
 k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* __restrict__ b, T* __restrict__ out,
              T omega) {
@@ -284,7 +286,7 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 66 + 1 + lane]
     const int lane = threadIdx.x, w = threadIdx.y;
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-    if (ALAMO_ROWS_XCD) {
+    if (XCD) {
         const int gx = gridDim.x, gy = gridDim.y;
         const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
         bx = t % gx;
@@ -514,7 +516,7 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
         if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG) {
             constexpr int RW = Rows64Big<T, MODE>::RW, KCH = Rows64Big<T, MODE>::KCH;
             dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
-            k_iso_rows64<T, MODE, FOLD, RW, KCH><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
+            k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
             return;
         }
         constexpr int RW = Rows64Cfg<T>::RW, KCH = Rows64Cfg<T>::KCH;
