@@ -41,10 +41,11 @@ def main():
                    help="bottom solver of the FGMRES V-cycle (plain multigrid keeps bicgstab)")
     p.add_argument("--polycrystal", action="store_true")
     p.add_argument("--poly-n", type=int, default=256)
-    # the polycrystal (MajorMinor, Neumann sides) needs the full smoothing on the finest level: one
-    # sweep there at restart 8 does not reach 1e-10 in 400 iterations
-    p.add_argument("--poly-restart", type=int, default=10)
-    p.add_argument("--poly-sweeps-finest", type=int, default=0)
+    # the polycrystal (MajorMinor, Neumann sides) wants longer restarts than the two-phase field:
+    # restart 16 with 2 finest-level sweeps (one sweep at restart 8 does not reach 1e-10 in 400
+    # iterations)
+    p.add_argument("--poly-restart", type=int, default=16)
+    p.add_argument("--poly-sweeps-finest", type=int, default=2)
     a = p.parse_args()
     if a.polycrystal:
         return polycrystal(a)
