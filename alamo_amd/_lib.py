@@ -70,6 +70,10 @@ SIGNATURES = {
     "alamo_newton_dw": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_newton_w": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_set_uniform": (c_int, [c_void_p, c_int]),
+    "alamo_mg_norm_inf": (c_int, [c_void_p, c_int, c_void_p, P(c_double), c_void_p]),
+    "alamo_mg_norm_l2": (c_int, [c_void_p, c_int, c_void_p, P(c_double), c_void_p]),
+    "alamo_mg_reflux": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "alamo_learned_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(KrylovOpts), P(SolveStats)]),
     "alamo_newton_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
                                    P(SolveStats), P(c_double)]),
     "alamo_mg_trac_xhi": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),

@@ -1391,6 +1391,12 @@ int alamo_mg_norms(alamo_mg* h, int mglev, const double* x, double* norm_inf, do
     h->stream = keep;
     return rc;
 }
+int alamo_mg_norm_inf(alamo_mg* h, int mglev, const double* x, double* norm, void* stream) {
+    return alamo_mg_norms(h, mglev, x, norm, nullptr, stream);
+}
+int alamo_mg_norm_l2(alamo_mg* h, int mglev, const double* x, double* norm, void* stream) {
+    return alamo_mg_norms(h, mglev, x, nullptr, norm, stream);
+}
 
 int alamo_bc_init(alamo_mg* h, double* b, int homogeneous);
 int alamo_newton_rhs(alamo_mg* h, const double* u, const double* b, const double* F0, double* rhs);
@@ -1416,6 +1422,15 @@ int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_r
         FAIL(ALAMO_ERR_ARG, "unknown preconditioner");
     if (o.precond == ALAMO_PRECOND_LEARNED && !h->net) FAIL(ALAMO_ERR_STATE, "no network attached (alamo_mg_set_net)");
     return fgmres_impl(h, sol, rhs, tol_rel, tol_abs, &o, stats);
+}
+
+int alamo_learned_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                        const alamo_krylov_opts* opts, alamo_solve_stats* stats) {
+    alamo_krylov_opts o;
+    if (opts) o = *opts;
+    else alamo_krylov_opts_default(&o);
+    o.precond = ALAMO_PRECOND_LEARNED;
+    return alamo_krylov_solve(h, sol, rhs, tol_rel, tol_abs, &o, stats);
 }
 
 int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z) {
@@ -1994,6 +2009,12 @@ int alamo_amr_reflux(const int* ncell_crse, const int* fine_lo, const int* fine_
     amr::k_reflux<<<nblocks(nv), 256, 0, (hipStream_t)stream>>>(d, fb, mask, crse_res, fine_res);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
+}
+int alamo_mg_reflux(alamo_mg* h, const int* fine_lo, const int* fine_hi, const int* mask, double* crse_res,
+                    const double* fine_res, void* stream) {
+    if (!h) FAIL(ALAMO_ERR_ARG, "null handle");
+    if (h->L[0].g.sy != h->L[0].g.N[0]) FAIL(ALAMO_ERR_ARG, "alamo_mg_reflux: the AMR helpers take compact fields");
+    return alamo_amr_reflux(h->d.n_cell, fine_lo, fine_hi, mask, crse_res, fine_res, stream);
 }
 
 int alamo_amr_average_down(const int* ncell_crse, const int* fine_lo, const int* fine_hi, double* crse_sol,

@@ -301,6 +301,24 @@ class Elastic:
     def getNComp(self):
         return 3
 
+    def Reflux(self, fine_lo, fine_hi, mask, crse_res, fine_res, stream=None):
+        """Operator<Grid::Node>::reflux (Operator.cpp:526-633) of one fine box onto this handle's
+        level-0 residual (compact layout); mask from alamo_amd.amr.fine_mask."""
+        lo = np.ascontiguousarray(np.asarray(fine_lo, dtype=np.int32).reshape(3))
+        hi = np.ascontiguousarray(np.asarray(fine_hi, dtype=np.int32).reshape(3))
+        check(lib().alamo_mg_reflux(self.h, lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p),
+                                    _ptr(mask), _ptr(crse_res), _ptr(fine_res), _ptr(stream)))
+
+    def NormInf(self, mglev, x, stream=None):
+        v = ctypes.c_double()
+        check(lib().alamo_mg_norm_inf(self.h, mglev, _ptr(x), ctypes.byref(v), _ptr(stream)))
+        return v.value
+
+    def NormL2(self, mglev, x, stream=None):
+        v = ctypes.c_double()
+        check(lib().alamo_mg_norm_l2(self.h, mglev, _ptr(x), ctypes.byref(v), _ptr(stream)))
+        return v.value
+
     def Error0x(self, amrlev, mglev, R0x, x, stream=None):
         """Elastic::Error0x (Elastic.cpp:382-404): R0x = x - A (x / D)."""
         check(lib().alamo_mg_error0x(self.h, amrlev, mglev, _ptr(R0x), _ptr(x), _ptr(stream)))
@@ -519,6 +537,21 @@ class Krylov:
         out = st.as_dict()
         out["rc"] = rc
         return out
+
+
+def learned_solve(op, sol, rhs, tol_rel, tol_abs, restart=16, max_iter=400, stagnation_ratio=0.9,
+                  raise_on_fail=True):
+    """alamo_learned_solve: FGMRES with the learned preconditioner of the network attached to op."""
+    o = KrylovOpts()
+    check(lib().alamo_krylov_opts_default(ctypes.byref(o)))
+    o.restart, o.max_iter, o.stagnation_ratio = restart, max_iter, stagnation_ratio
+    st = SolveStats()
+    rc = lib().alamo_learned_solve(op.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(o), ctypes.byref(st))
+    if rc != 0 and raise_on_fail:
+        check(rc)
+    out = st.as_dict()
+    out["rc"] = rc
+    return out
 
 
 class Net:

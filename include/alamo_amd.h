@@ -202,6 +202,10 @@ int alamo_mg_restrict(alamo_mg* h, int amrlev, int cmglev, double* crse, const d
 int alamo_mg_interpolate(alamo_mg* h, int amrlev, int fmglev, double* fine, const double* crse, void* stream);
 /* max-norm and L2 norm of a level vector (MLMG's ResNormInf; 3 components). Synchronises. */
 int alamo_mg_norms(alamo_mg* h, int mglev, const double* x, double* norm_inf, double* norm_l2, void* stream);
+/* the two halves of alamo_mg_norms, under the names a caller binding MLMG's norm calls uses
+ * (MultiFab::norm0 / norm2 over the 3 components of a level vector). */
+int alamo_mg_norm_inf(alamo_mg* h, int mglev, const double* x, double* norm, void* stream);
+int alamo_mg_norm_l2(alamo_mg* h, int mglev, const double* x, double* norm, void* stream);
 
 /* BC::Operator::Elastic::Constant::Init (BC/Operator/Elastic/Constant.H:197-286):
  * write the prescribed boundary values into b (device pointer, level 0). */
@@ -245,6 +249,10 @@ typedef struct {
 int alamo_krylov_opts_default(alamo_krylov_opts* o);
 int alamo_krylov_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        const alamo_krylov_opts* opts, alamo_solve_stats* stats);
+/* The learned solve: alamo_krylov_solve with the learned preconditioner (the network attached by
+ * alamo_mg_set_net), opts may be NULL (the defaults); opts->precond is ignored. */
+int alamo_learned_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
+                        const alamo_krylov_opts* opts, alamo_solve_stats* stats);
 
 /* Two-level AMR helpers of Operator<Grid::Node> (src/Operator/Operator.cpp).  The coarse level
  * is one box covering the non-periodic domain of ncell_crse[3] cells (nodes (n+1)^3, i fastest).
@@ -261,6 +269,10 @@ int alamo_amr_fine_mask(const int* ncell_crse, int nbox, const int* fine_lo, con
  * box's crse_fine / fine nodes becomes the restricted fine residual. */
 int alamo_amr_reflux(const int* ncell_crse, const int* fine_lo, const int* fine_hi, const int* mask, double* crse_res,
                      const double* fine_res, void* stream);
+/* Operator<Grid::Node>::reflux under its operator name: alamo_amr_reflux for the coarse level of
+ * the handle (ncell_crse = the handle's n_cell at mglev 0). */
+int alamo_mg_reflux(alamo_mg* h, const int* fine_lo, const int* fine_hi, const int* mask, double* crse_res,
+                    const double* fine_res, void* stream);
 /* Operator<Grid::Node>::averageDownSolutionRHS (Operator.cpp:435-447): nodal injection of the fine
  * box's solution onto the coarse nodes it covers. */
 int alamo_amr_average_down(const int* ncell_crse, const int* fine_lo, const int* fine_hi, double* crse_sol,

@@ -176,3 +176,40 @@ def test_sweeps_finest(cuda, ora, precision):
         print("  sweeps_finest %d: %d iterations, rel L2 %.2e" % (fin, st["iterations"], err))
         assert st["converged"] == 1 and err <= 1e-8
     assert np.array_equal(sols[0], sols[3])
+
+
+def test_abi_names_of_the_reference_calls(cuda, ora):
+    """alamo_mg_norm_inf / _l2, alamo_learned_solve and alamo_mg_reflux give exactly what the
+    entry points they wrap give."""
+    from alamo_amd import amr
+    from alamo_amd.operator import Krylov, Net, learned_solve, PRECOND_LEARNED
+    from alamo_amd._lib import AlamoError
+    import torch
+    pr, bc, rhs = eshelby_rhs(ora, 16)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc)
+    op.SetModel(pr["C"])
+    x = rng(16).standard_normal(rhs.size)
+    xd = to_dev(x, cuda)
+    ninf, nl2 = op.Norms(0, xd)
+    assert op.NormInf(0, xd) == ninf and op.NormL2(0, xd) == nl2
+    assert abs(ninf - np.abs(x).max()) == 0.0 and abs(nl2 - np.linalg.norm(x)) <= 1e-13 * nl2
+    with pytest.raises(AlamoError, match="no network"):
+        learned_solve(op, to_dev(np.zeros_like(rhs), cuda), to_dev(rhs, cuda), 1e-10, 0.0)
+    net = Net(seed=4)
+    net.attach(op)
+    a, b = to_dev(np.zeros_like(rhs), cuda), to_dev(np.zeros_like(rhs), cuda)
+    sa = learned_solve(op, a, to_dev(rhs, cuda), 1e-10, 0.0, restart=8)
+    sb = Krylov(op, restart=8, precond=PRECOND_LEARNED).solve(b, to_dev(rhs, cuda), 1e-10, 0.0)
+    assert sa["converged"] == 1 and sa["iterations"] == sb["iterations"]
+    assert np.array_equal(to_host(a), to_host(b))
+    # reflux onto the handle's level 0 equals the AMR helper on the same coarse grid
+    NC = np.array(pr["ncell"])
+    lo, hi = (4, 4, 4), (15, 11, 19)
+    m = torch.zeros(int(np.prod(NC + 1)), dtype=torch.int32, device=cuda)
+    amr.fine_mask(NC, [lo], [hi], m)
+    g = amr.grown_fine_nodes(lo, hi)
+    fine = to_dev(rng(17).uniform(-1, 1, 3 * int(np.prod(g))), cuda)
+    c1, c2 = to_dev(x, cuda), to_dev(x, cuda)
+    op.Reflux(lo, hi, m, c1, fine)
+    amr.reflux(NC, lo, hi, m, c2, fine)
+    assert np.array_equal(to_host(c1), to_host(c2))
