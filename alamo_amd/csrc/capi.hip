@@ -687,6 +687,20 @@ static const double g_reorth_eta = [] {
     const char* e = getenv("ALAMO_KRY_REORTH");
     return e ? atof(e) : 0.1;
 }();
+// Basis vectors kept unnormalised: vector j + 1 of the basis is stored as the Gram-Schmidt
+// result w'' with its exact scale sc[j+1] = 1 / |w''| (the norm the axpy pass returns), so no
+// pass over the vector normalises it.  The scales enter only host-side coefficients: the
+// multi-dot values (h_i = sc_j sc_i w'.V_i), the axpy coefficients (sc_i^2 w'.V_i), the
+// Hessenberg subdiagonal (sc_j |w''|) and the solution update (y_j sc_j), which needs the
+// preconditioner to be homogeneous, M(s v) = s M(v): true of every multigrid cycle here
+// (damped Jacobi, restriction, prolongation, the dense and smoother bottoms are linear;
+// BiCGStab's iterates and relative stopping test scale with the right-hand side) and of
+// none, not of the learned stage, whose steps normalise explicitly.  ALAMO_KRY_LAZY=0 always
+// normalises.
+static const bool g_kry_lazy = [] {
+    const char* e = getenv("ALAMO_KRY_LAZY");
+    return !(e && strcmp(e, "0") == 0);
+}();
 
 static int upload_coef(alamo_mg* h, const double* c, int nv) {
     TRY(krylov_scratch(h));
@@ -761,11 +775,13 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
     int kind = o->precond;
     double beta_prev = beta;
     double H[KRY_MAXV + 1][KRY_MAXV], cs[KRY_MAXV], sn[KRY_MAXV], gv[KRY_MAXV + 1], y[KRY_MAXV];
-    double d1[KRY_MAXV], d2[KRY_MAXV];
+    double d1[KRY_MAXV], d2[KRY_MAXV], cf[KRY_MAXV];
+    double sc[KRY_MAXV + 1];  // basis vector i = sc[i] x stored vector i
     if (o->verbose) fprintf(stderr, "alamo fgmres: iter 0 |r|inf %.6e |r|2 %.6e target %.3e\n", r0, beta, target);
     while (rinf > target && it < o->max_iter && std::isfinite(rinf) && beta > 0.0) {
         const double ratio = rinf / beta;
         k_scale<double, TB><<<gs_blocks(n), 256, 0, h->stream>>>(V, r, n, 1.0 / beta);
+        sc[0] = 1.0;
         for (int i = 0; i <= m; i++) gv[i] = 0.0;
         gv[0] = beta;
         int jend = 0;
@@ -789,24 +805,39 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
                 }
                 launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             }
-            // classical Gram-Schmidt; w is vector j + 1 of the basis block, so the same multi-dot
-            // pass returns |w|^2 in d1[j + 1]
+            // classical Gram-Schmidt on the stored vectors (w' = A z'_j, z'_j = M(stored v_j)); w
+            // is vector j + 1 of the basis block, so the same multi-dot pass returns |w'|^2 in
+            // d1[j + 1]; the true Arnoldi quantities follow from the scales sc
+            const bool lazy = g_kry_lazy && kind != ALAMO_PRECOND_LEARNED;
+            const double sj = sc[j];
             TRY(multidot_t<TB>(h, w, V, n, j + 2, d1));
             const double wn = std::sqrt(std::max(d1[j + 1], 0.0));
-            TRY(upload_coef(h, d1, j + 1));
-            double hn = 0;
-            TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &hn));
-            for (int i = 0; i <= j; i++) H[i][j] = d1[i];
-            if (hn < g_reorth_eta * wn) {
+            for (int i = 0; i <= j; i++) {
+                H[i][j] = sj * sc[i] * d1[i];
+                cf[i] = sc[i] * sc[i] * d1[i];
+            }
+            TRY(upload_coef(h, cf, j + 1));
+            double g = 0;  // |w''| of the stored vector
+            TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g));
+            if (g < g_reorth_eta * wn) {
                 // the first pass cancelled most of w: re-orthogonalise (CGS2)
                 TRY(multidot_t<TB>(h, w, V, n, j + 1, d2));
-                TRY(upload_coef(h, d2, j + 1));
-                TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &hn));
-                for (int i = 0; i <= j; i++) H[i][j] += d2[i];
+                for (int i = 0; i <= j; i++) {
+                    H[i][j] += sj * sc[i] * d2[i];
+                    cf[i] = sc[i] * sc[i] * d2[i];
+                }
+                TRY(upload_coef(h, cf, j + 1));
+                TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g));
                 reorths++;
             }
+            const double hn = sj * g;  // the true |w''|
             H[j + 1][j] = hn;
-            if (hn > 0.0) k_scale<TB, TB><<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / hn);
+            if (lazy) {
+                sc[j + 1] = g > 0.0 ? 1.0 / g : 1.0;
+            } else {
+                if (g > 0.0) k_scale<TB, TB><<<gs_blocks(n), 256, 0, h->stream>>>(w, w, n, 1.0 / g);
+                sc[j + 1] = 1.0;
+            }
             for (int i = 0; i < j; i++) {
                 const double a = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
                 H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
@@ -844,7 +875,8 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
             beta_prev = beta;
             continue;
         }
-        TRY(upload_coef(h, y, jend));
+        for (int i = 0; i < jend; i++) cf[i] = y[i] * sc[i];  // z_i = sc_i x stored z_i
+        TRY(upload_coef(h, cf, jend));
         k_multiaxpy_add<TB><<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, r, 0.0, h->stream);
         TRY(norms_dev<double>(h, r, n, &rinf, &beta));
