@@ -44,13 +44,15 @@ static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row
 #endif
 #define ALAMO_ROWS_ATTR __launch_bounds__(64 * ROWS_RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
 
-// rows per workgroup and planes per chunk of the aligned-tile kernel, per precision (the FP32
-// kernel needs about half the registers, which leaves room for taller workgroups)
+// rows per workgroup and planes per chunk of the aligned-tile kernel below the large-level
+// class, per precision.  Measured at 129^3 nodes in one process (4 x 16 in the hardware order in
+// brackets), both in the XCD-contiguous order: FP32 4 x 8 Jacobi 21.6 us (29.1), residual
+// 19.7 (28.5); FP64 4 x 16 Jacobi 39.7 (47.1), residual 36.4 (46.0)
 #ifndef ALAMO_ROWS64_RW_F32
 #define ALAMO_ROWS64_RW_F32 4
 #endif
 #ifndef ALAMO_ROWS64_KCH_F32
-#define ALAMO_ROWS64_KCH_F32 16
+#define ALAMO_ROWS64_KCH_F32 8
 #endif
 template <typename T> struct Rows64Cfg { static constexpr int RW = ROWS_RW, KCH = ROWS_KCH; };
 template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_F32, KCH = ALAMO_ROWS64_KCH_F32; };
@@ -521,7 +523,7 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
         }
         constexpr int RW = Rows64Cfg<T>::RW, KCH = Rows64Cfg<T>::KCH;
         dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
-        k_iso_rows64<T, MODE, FOLD><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
+        k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
         return;
     }
     dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,
