@@ -66,6 +66,11 @@ constexpr int ROWS64_BIG = 192;
 template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 64; };
 template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 32; };
 template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 32; };
+// levels of < ROWS64_MID cells in y or z (65^3 nodes and the like): 2 x 2 tiles keep about
+// 1000 workgroups in flight.  At 65^3 in FP32: Jacobi 6.3 us, residual 6.0 us (4 x 8: 13.1 and
+// 12.1; the node-parallel rows took 18 and 15)
+constexpr int ROWS64_MID = 96;
+struct Rows64Small { static constexpr int RW = 2, KCH = 2; };
 
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
@@ -517,6 +522,12 @@ static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b
         // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
         if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG) {
             constexpr int RW = Rows64Big<T, MODE>::RW, KCH = Rows64Big<T, MODE>::KCH;
+            dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
+            k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
+            return;
+        }
+        if (g.n[1] < ROWS64_MID || g.n[2] < ROWS64_MID) {
+            constexpr int RW = Rows64Small::RW, KCH = Rows64Small::KCH;
             dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
             k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
             return;

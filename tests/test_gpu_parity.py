@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-12
 
 CASES = [
-    # (sym, ncell)  -- levels of >= 2^20 nodes run the LDS-tiled isotropic kernel (incl. the
+    # (sym, ncell)  -- levels of >= 2^18 nodes run the LDS-tiled isotropic kernel (incl. the
     # folded x-face rows), smaller ones the node-parallel kernels
     (P.ISO, (8, 8, 8)),
     (P.ISO, (12, 10, 9)),
@@ -24,6 +24,7 @@ CASES = [
     (P.ISO, (128, 100, 80)),
     (P.ISO, (130, 97, 83)),
     (P.ISO, (100, 200, 196)),  # >= 192 cells in y and z: the taller / deeper row-kernel tiles
+    (P.ISO, (64, 64, 64)),     # >= 2^18 nodes, < 96 cells in y / z: the 2 x 2 row-kernel tiles
     (P.MAJORMINOR, (8, 8, 8)),
     (P.MAJORMINOR, (11, 7, 9)),
     (P.DIAGONAL, (9, 8, 10)),
@@ -100,7 +101,7 @@ def test_diagonal_residual_fsmooth(cuda, ora, sym, ncell):
     close(to_host(xn), x / Dref)
 
 
-@pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180), (100, 200, 196)])
+@pytest.mark.parametrize("ncell", [(128, 100, 80), (300, 20, 180), (100, 200, 196), (70, 40, 100)])
 def test_iso_dirichlet_x_faces(cuda, ora, ncell):
     """Dirichlet x faces in all components: the row kernel writes those face rows itself and
     the boundary kernel covers the y and z faces (their edges keep traction/Neumann rows)."""
