@@ -59,13 +59,13 @@ template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_
 // Tile shape for levels of >= ROWS64_BIG cells in y and z, where taller or deeper tiles still
 // give every CU several workgroups, launched in the XCD-contiguous tile order (neighbouring
 // tiles share their halo rows in one XCD's L2).  Measured at 257^3 nodes in one process, the
-// 4 x 16 shape in the hardware order in brackets: FP32 8 x 32: Jacobi 211 us (233), residual
-// 195 (214), apply 150 (175); FP64 Jacobi and residual 4 x 64: 403 (414) and 378 (388); FP64
-// apply 8 x 32: 290 (305).  Every shape and order writes the same values bit for bit.
+// 4 x 16 shape in the hardware order in brackets: FP32 8 x 64: Jacobi 211 us (233), residual
+// 192 (214), apply 151 (175); FP64 Jacobi and residual 4 x 128: 392 (414) and 365 (388); FP64
+// apply 8 x 128: 281 (305).  Every shape and order writes the same values bit for bit.
 constexpr int ROWS64_BIG = 192;
-template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 64; };
-template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 32; };
-template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 32; };
+template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 128; };
+template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 128; };
+template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 64; };
 // levels of < ROWS64_MID cells in y or z (65^3 nodes and the like): 2 x 2 tiles keep about
 // 1000 workgroups in flight.  At 65^3 in FP32: Jacobi 6.3 us, residual 6.0 us (4 x 8: 13.1 and
 // 12.1; the node-parallel rows took 18 and 15)
