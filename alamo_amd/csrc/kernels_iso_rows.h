@@ -281,21 +281,50 @@ __global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, cons
 // by that tile's lane 63 from its halo value.
 // ---------------------------------------------------------------------------
 constexpr int R64_W = 66;  // LDS row: halo, 64 nodes, halo
+
+// a y- or z-face node q (boundary_ijk order, q < boundary_count_yz) when every region is
+// Dirichlet: f = u (diagonal 1)
+template <typename T, int MODE>
+__device__ __forceinline__ void dirichlet_yz_node(const Geom& g, const T* __restrict__ u, const T* __restrict__ b,
+                                                  T* __restrict__ out, T omega, int64_t q) {
+    int i, j, k;
+    boundary_ijk(g, q, i, j, k);
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const int64_t qq = v * g.tot + m;
+        const T x = u[qq];
+        if (MODE == APPLY) out[qq] = x;
+        else if (MODE == RESID) out[qq] = b[qq] - x;
+        else out[qq] = (T(1) - omega) * x + omega * b[qq];
+    }
+}
+
+// The tile grid is gridDim.x x gridDim.y x gz_tiles; with FACES, the workgroups beyond it in z
+// write the nfaces y- and z-face nodes of an all-Dirichlet level instead (one node per
+// work-item) and leave before any LDS or barrier, so the faces need no launch of their own.
 template <typename T, int MODE, int FOLD, int RW = Rows64Cfg<T>::RW, int KCH = Rows64Cfg<T>::KCH,
-          bool XCD = (ALAMO_ROWS_XCD != 0)>
+          bool XCD = (ALAMO_ROWS_XCD != 0), bool FACES = false>
 __global__ void __launch_bounds__(64 * RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
 k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* __restrict__ b, T* __restrict__ out,
-             T omega) {
+             T omega, int gz_tiles, int64_t nfaces) {
     static_assert(RW >= 2, "the first and the last wavefront stage one halo row each");
     constexpr int NR = RW + 2;  // staged rows: halo, RW rows, halo
     static_assert(NR * 10 <= 64 * RW, "the wavefronts' gathers cover the edge values of every staged row");
+    if (FACES && (int)blockIdx.z >= gz_tiles) {
+        const int64_t q = ((int64_t)(blockIdx.z - gz_tiles) * gridDim.y * gridDim.x +
+                           (int64_t)blockIdx.y * gridDim.x + blockIdx.x) * (64 * RW) +
+                          threadIdx.y * 64 + threadIdx.x;
+        if (q < nfaces) dirichlet_yz_node<T, MODE>(g, u, b, out, omega, q);
+        return;
+    }
     constexpr int A = NR * R64_W;
     __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 66 + 1 + lane]
     const int lane = threadIdx.x, w = threadIdx.y;
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (XCD) {
         const int gx = gridDim.x, gy = gridDim.y;
-        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gz_tiles);
         bx = t % gx;
         by = (t / gx) % gy;
         bz = t / (gx * gy);
@@ -497,17 +526,7 @@ __global__ void __launch_bounds__(256) k_dirichlet_yz(Geom g, const T* __restric
                                                       T* __restrict__ out, T omega, int64_t nb) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nb) return;
-    int i, j, k;
-    boundary_ijk(g, q, i, j, k);
-    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-        const int64_t qq = v * g.tot + m;
-        const T x = u[qq];
-        if (MODE == APPLY) out[qq] = x;
-        else if (MODE == RESID) out[qq] = b[qq] - x;
-        else out[qq] = (T(1) - omega) * x + omega * b[qq];
-    }
+    dirichlet_yz_node<T, MODE>(g, u, b, out, omega, q);
 }
 
 // 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons)
@@ -515,31 +534,43 @@ static const bool g_rows_tile64 = [] {
     const char* e = getenv("ALAMO_ROWS_TILE");
     return !(e && strcmp(e, "62") == 0);
 }();
+// row-kernel launch of one tile shape; nfaces > 0 appends the workgroups that write the y- and
+// z-face nodes of an all-Dirichlet level
+template <typename T, int MODE, int FOLD, int RW, int KCH>
+static void launch_rows64(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, int64_t nfaces,
+                          hipStream_t s) {
+    const int gx = (g.n[0] + 63) / 64, gy = (g.n[1] - 1 + RW - 1) / RW, gz = (g.n[2] - 1 + KCH - 1) / KCH;
+    if (nfaces > 0) {
+        const int64_t slice = (int64_t)gx * gy * 64 * RW;
+        const int fz = (int)((nfaces + slice - 1) / slice);
+        k_iso_rows64<T, MODE, FOLD, RW, KCH, true, true><<<dim3(gx, gy, gz + fz), dim3(64, RW), 0, s>>>(
+            g, C, u, b, out, omega, gz, nfaces);
+    } else {
+        k_iso_rows64<T, MODE, FOLD, RW, KCH, true, false><<<dim3(gx, gy, gz), dim3(64, RW), 0, s>>>(
+            g, C, u, b, out, omega, gz, 0);
+    }
+}
+
+// 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons);
+// returns whether the launch also wrote the y- and z-face nodes (nfaces > 0, 64-output tiles)
 template <typename T, int MODE, int FOLD>
-static void launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s) {
+static bool launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s,
+                               int64_t nfaces = 0) {
     dim3 blk(64, ROWS_RW);
     if (g_rows_tile64) {
         // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
-        if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG) {
-            constexpr int RW = Rows64Big<T, MODE>::RW, KCH = Rows64Big<T, MODE>::KCH;
-            dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
-            k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
-            return;
-        }
-        if (g.n[1] < ROWS64_MID || g.n[2] < ROWS64_MID) {
-            constexpr int RW = Rows64Small::RW, KCH = Rows64Small::KCH;
-            dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
-            k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
-            return;
-        }
-        constexpr int RW = Rows64Cfg<T>::RW, KCH = Rows64Cfg<T>::KCH;
-        dim3 grid((g.n[0] + 63) / 64, (g.n[1] - 1 + RW - 1) / RW, (g.n[2] - 1 + KCH - 1) / KCH);
-        k_iso_rows64<T, MODE, FOLD, RW, KCH, true><<<grid, dim3(64, RW), 0, s>>>(g, C, u, b, out, omega);
-        return;
+        if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG)
+            launch_rows64<T, MODE, FOLD, Rows64Big<T, MODE>::RW, Rows64Big<T, MODE>::KCH>(g, C, u, b, out, omega,
+                                                                                       nfaces, s);
+        else if (g.n[1] < ROWS64_MID || g.n[2] < ROWS64_MID)
+            launch_rows64<T, MODE, FOLD, Rows64Small::RW, Rows64Small::KCH>(g, C, u, b, out, omega, nfaces, s);
+        else launch_rows64<T, MODE, FOLD, Rows64Cfg<T>::RW, Rows64Cfg<T>::KCH>(g, C, u, b, out, omega, nfaces, s);
+        return nfaces > 0;
     }
     dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,
               (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
     k_iso_rows<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
+    return false;
 }
 
 template <typename T, int MODE>
@@ -548,8 +579,8 @@ static void launch_iso_rows_mode(const Geom& g, const BCTable& bc, const T* C, c
     const bool xdir = x_faces_dirichlet(bc);
     const int64_t nb = xdir ? boundary_count_yz(g) : boundary_count(g);
     if (all_dirichlet(bc)) {
-        launch_rows_kernel<T, MODE, 1>(g, C, u, b, out, omega, s);
-        k_dirichlet_yz<T, MODE><<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(g, u, b, out, omega, nb);
+        if (!launch_rows_kernel<T, MODE, 1>(g, C, u, b, out, omega, s, nb))
+            k_dirichlet_yz<T, MODE><<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(g, u, b, out, omega, nb);
         return;
     }
     if (xdir) launch_rows_kernel<T, MODE, 1>(g, C, u, b, out, omega, s);
