@@ -45,7 +45,7 @@ struct IsoPlane {
 // The tiled kernel marches z serially inside a workgroup, so it only pays on levels large
 // enough to give every CU several workgroups; smaller levels use the node-parallel kernel.
 inline bool iso_tiled_ok(const Geom& g) {
-    return g.n[0] >= 16 && g.n[1] >= 16 && g.n[2] >= 16 && g.tot >= (int64_t)1 << 18;
+    return g.n[0] >= 16 && g.n[1] >= 16 && g.n[2] >= 16 && g.tot >= (int64_t)1 << 12;
 }
 
 template <typename T>
