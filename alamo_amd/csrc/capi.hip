@@ -173,13 +173,19 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
         return;
     }
     if (sym == ISO) {
-        // large isotropic levels take the LDS-tiled interior kernel plus the boundary kernel
-        if (mode != DIAG && iso_tiled_ok(g)) {
-            // the row kernel addresses each array with 32-bit byte offsets
-            if (g_iso_kernel == 1 || (int64_t)sizeof(T) * g.tot >= ((int64_t)1 << 32))
+        // isotropic levels of >= 4 cells per direction take the row kernel (every index it forms
+        // is clamped into the level); the row kernel addresses each array with 32-bit byte
+        // offsets, so larger arrays, and ALAMO_ISO_KERNEL=tiles, take the LDS-tiled kernel
+        if (mode != DIAG) {
+            const bool huge = (int64_t)sizeof(T) * g.tot >= ((int64_t)1 << 32);
+            if ((g_iso_kernel == 1 || huge) && iso_tiled_ok(g)) {
                 launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
-            else launch_iso_rows<T>(mode, g, bc, C, u, b, out, omega, s);
-            return;
+                return;
+            }
+            if (!huge && g_iso_kernel != 1 && iso_rows_ok(g)) {
+                launch_iso_rows<T>(mode, g, bc, C, u, b, out, omega, s);
+                return;
+            }
         }
         launch_node_sym<T, ISO>(mode, g, bc, C, u, b, out, omega, s);
     } else if (sym == MAJORMINOR) {

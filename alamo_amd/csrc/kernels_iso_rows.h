@@ -98,7 +98,9 @@ struct RowPlane {
     T dxux, dxuz, dyuy, dyuz;  // D_x u_x, D_x u_z, D_y u_y, D_y u_z
 };
 
-inline bool iso_rows_ok(const Geom& g) { return g.n[0] >= 2 && g.n[1] >= 2 && g.n[2] >= 2; }
+// levels the row kernel serves: every row, plane, halo and gather index it forms is clamped into
+// [0, n], which holds for any n >= 2; the launch routing takes it from 4 cells per direction
+inline bool iso_rows_ok(const Geom& g) { return g.n[0] >= 4 && g.n[1] >= 4 && g.n[2] >= 4; }
 
 // 32-bit byte offset from a uniform base: lets the loads and stores use the scalar-base
 // addressing form (the host checks that every offset fits)

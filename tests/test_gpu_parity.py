@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-12
 
 CASES = [
-    # (sym, ncell)  -- levels of >= 2^12 nodes and >= 16 cells per direction run the row kernel (incl. the
+    # (sym, ncell)  -- isotropic levels of >= 4 cells per direction run the row kernel (incl. the
     # folded x-face rows), smaller ones the node-parallel kernels
     (P.ISO, (8, 8, 8)),
     (P.ISO, (12, 10, 9)),
@@ -25,7 +25,8 @@ CASES = [
     (P.ISO, (130, 97, 83)),
     (P.ISO, (100, 200, 196)),  # >= 192 cells in y and z: the taller / deeper row-kernel tiles
     (P.ISO, (64, 64, 64)),     # < 96 cells in y / z: the 2 x 2 row-kernel tiles
-    (P.ISO, (16, 16, 16)),     # the smallest level the row kernel takes (16 cells, >= 2^12 nodes)
+    (P.ISO, (16, 16, 16)),
+    (P.ISO, (4, 5, 6)),        # the smallest levels the row kernel takes (>= 4 cells per direction)
     (P.MAJORMINOR, (8, 8, 8)),
     (P.MAJORMINOR, (11, 7, 9)),
     (P.DIAGONAL, (9, 8, 10)),
