@@ -343,13 +343,18 @@ static void bottom(alamo_mg* h) {
 template <typename T>
 static void vcycle(alamo_mg* h, int top, bool first_done = false) {
     const int bot = (int)h->L.size() - 1;
+    bool done = first_done;
     for (int l = top; l < bot; l++) {
         LevelBuf& lb = h->L[l];
-        presmooth_from_zero<T>(h, l, first_done && l == top);
+        presmooth_from_zero<T>(h, l, done);
         launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
                        T(0), h->stream);
         LevelBuf& lc = h->L[l + 1];
-        k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, 3, LB<T>::rescor(lb), LB<T>::res(lc));
+        // the restriction also applies the next level's first pre-smoothing sweep (from zero)
+        done = l + 1 < bot && h->d.pre_smooth * sps(h, l + 1) > 0;
+        k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, 3, LB<T>::rescor(lb), LB<T>::res(lc),
+                                                                 done ? LB<T>::cor(lc) : nullptr, LB<T>::D(lc),
+                                                                 T(h->d.omega));
     }
     bottom<T>(h);
     for (int l = bot - 1; l >= top; l--) {

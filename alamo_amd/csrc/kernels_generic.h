@@ -185,8 +185,12 @@ __global__ void __launch_bounds__(RED_BLOCK) k_sum_final(const double* __restric
 // Full-weighting restriction, one work-item per coarse node, `np` SoA planes.
 // Operator/Operator.cpp:294-352 (residual) and Operator/Elastic.cpp:786-829 (stiffness).
 template <typename T>
+// xo (may be null; 3 planes): also the first damped-Jacobi sweep of the coarse level from a zero
+// guess, xo = omega crse / Dc (0 where Dc is 0), the value the V-cycle's pre-smoothing would
+// compute from crse next
 __global__ void __launch_bounds__(256) k_restrict(Geom gc, Geom gf, int np, const T* __restrict__ fine,
-                                                  T* __restrict__ crse) {
+                                                  T* __restrict__ crse, T* __restrict__ xo = nullptr,
+                                                  const T* __restrict__ Dc = nullptr, T omega = T(0)) {
     const int64_t mc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (mc >= gc.tot) return;
     int I, J, K;
@@ -228,6 +232,10 @@ __global__ void __launch_bounds__(256) k_restrict(Geom gc, Geom gf, int np, cons
             v = c8 / T(64) + c12 / T(32) + c6 / T(16) + at(0, 0, 0) / T(8);
         }
         crse[p * gc.tot + mc] = v;
+        if (xo) {
+            const T d = Dc[p * gc.tot + mc];
+            xo[p * gc.tot + mc] = d != T(0) ? omega * v / d : T(0);
+        }
     }
 }
 
