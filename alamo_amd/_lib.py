@@ -20,7 +20,7 @@ class MGDesc(ctypes.Structure):
                 ("max_coarsening_level", c_int), ("precision", c_int), ("cycle", c_int),
                 ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
                 ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
-                ("bottom_tol_rel", c_double), ("verbose", c_int), ("row_align", c_int), ("fixed_iter", c_int),
+                ("bottom_tol_rel", c_double), ("verbose", c_int), ("smoother", c_int), ("fixed_iter", c_int),
                 ("bottom_solver", c_int), ("max_fmg_iter", c_int), ("sweeps_finest", c_int)]
 
 
@@ -144,6 +144,25 @@ class AlamoError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("alamo error %d: %s" % (code, msg))
         self.code = code
+
+
+def dev_ptr(t, n, what, dtype="float64"):
+    """Pointer of a device field, checked before any launch: a contiguous torch tensor on the GPU
+    of the given dtype holding at least n elements.  The kernels index n elements from the
+    pointer, so a host tensor, another dtype or a short buffer is an error here rather than an
+    out-of-bounds access on the device."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise AlamoError(1, "%s: expected a torch tensor on the GPU, got %s" % (what, type(t).__name__))
+    if not t.is_cuda:
+        raise AlamoError(1, "%s: tensor is on %s, not on the GPU" % (what, t.device))
+    if t.dtype != getattr(torch, dtype):
+        raise AlamoError(1, "%s: dtype %s, expected %s" % (what, t.dtype, dtype))
+    if not t.is_contiguous():
+        raise AlamoError(1, "%s: tensor is not contiguous" % what)
+    if t.numel() < n:
+        raise AlamoError(1, "%s: %d elements, need at least %d" % (what, t.numel(), n))
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def check(rc):

@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import check, dev_ptr, lib
 
 CRSE_NODE, CRSE_FINE_NODE, FINE_NODE = 0, 1, 2  # amrex::nodelap_detail
 
@@ -18,8 +18,8 @@ def _iv(a):
     return a, a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr())
+def _nodes(n):
+    return int(np.prod([int(v) + 1 for v in np.asarray(n).reshape(3)]))
 
 
 def grown_fine_nodes(lo, hi):
@@ -32,7 +32,8 @@ def fine_mask(ncell_crse, boxes_lo, boxes_hi, mask, stream=None):
     nc, pnc = _iv(ncell_crse)
     lo, plo = _iv(boxes_lo)
     hi, phi = _iv(boxes_hi)
-    check(lib().alamo_amr_fine_mask(pnc, len(lo) // 3, plo, phi, _ptr(mask), stream))
+    check(lib().alamo_amr_fine_mask(pnc, len(lo) // 3, plo, phi, dev_ptr(mask, _nodes(nc), "fine_mask mask", "int32"),
+                                    stream))
 
 
 def reflux(ncell_crse, lo, hi, mask, crse_res, fine_res, stream=None):
@@ -40,7 +41,10 @@ def reflux(ncell_crse, lo, hi, mask, crse_res, fine_res, stream=None):
     nc, pnc = _iv(ncell_crse)
     l, pl = _iv(lo)
     h, ph = _iv(hi)
-    check(lib().alamo_amr_reflux(pnc, pl, ph, _ptr(mask), _ptr(crse_res), _ptr(fine_res), stream))
+    nf = int(np.prod(grown_fine_nodes(l, h)))
+    check(lib().alamo_amr_reflux(pnc, pl, ph, dev_ptr(mask, _nodes(nc), "reflux mask", "int32"),
+                                 dev_ptr(crse_res, 3 * _nodes(nc), "reflux crse"), dev_ptr(fine_res, 3 * nf, "reflux fine"),
+                                 stream))
 
 
 def average_down(ncell_crse, lo, hi, crse_sol, fine_sol, stream=None):
@@ -48,10 +52,13 @@ def average_down(ncell_crse, lo, hi, crse_sol, fine_sol, stream=None):
     nc, pnc = _iv(ncell_crse)
     l, pl = _iv(lo)
     h, ph = _iv(hi)
-    check(lib().alamo_amr_average_down(pnc, pl, ph, _ptr(crse_sol), _ptr(fine_sol), stream))
+    nf = int(np.prod(grown_fine_nodes(l, h)))
+    check(lib().alamo_amr_average_down(pnc, pl, ph, dev_ptr(crse_sol, 3 * _nodes(nc), "average_down crse"),
+                                       dev_ptr(fine_sol, 3 * nf, "average_down fine"), stream))
 
 
 def fixup_residual_mask(ncell_crse, mask, resmsk, stream=None):
     """Operator<Grid::Node>::fixUpResidualMask (Operator.cpp:230-251)"""
     nc, pnc = _iv(ncell_crse)
-    check(lib().alamo_amr_fixup_residual_mask(pnc, _ptr(mask), _ptr(resmsk), stream))
+    check(lib().alamo_amr_fixup_residual_mask(pnc, dev_ptr(mask, _nodes(nc), "fixup mask", "int32"),
+                                              dev_ptr(resmsk, _nodes(nc), "fixup resmsk", "int32"), stream))

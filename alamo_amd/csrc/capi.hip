@@ -32,11 +32,12 @@ static std::atomic<uint64_t> g_model_version{0};
 // largest grain table (21 values per grain) a MajorMinor workgroup stages in LDS: 48 KB (292 grains
 // in FP64, 585 in FP32); larger tables use the coefficient planes
 constexpr size_t MM_GRAIN_LDS_MAX = 48 * 1024;
-// interior kernel of large isotropic levels: 0 = row-wavefront kernel (default), 1 = 64x4
-// LDS-tile kernel; ALAMO_ISO_KERNEL=tiles selects the latter (for comparisons)
-static const int g_iso_kernel = [] {
+// The LDS-tile kernel serves isotropic levels whose arrays reach 4 GiB (the row kernel's 32-bit
+// byte offsets end there, ~812^3 FP64 nodes).  ALAMO_ISO_KERNEL=tiles routes every level to it, so
+// that fallback stays parity-tested at sizes the oracle checks quickly.
+static const bool g_iso_tiles = [] {
     const char* e = getenv("ALAMO_ISO_KERNEL");
-    return e && strcmp(e, "tiles") == 0 ? 1 : 0;
+    return e && strcmp(e, "tiles") == 0;
 }();
 
 #define FAIL(code, msg)            \
@@ -71,6 +72,8 @@ struct LevelBuf {
     float* Cf = nullptr;
     float* Df = nullptr;
     float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
+    double* dsm = nullptr;  // l1-Jacobi smoother diagonal (3 planes), allocated when that smoother is used
+    float* dsmf = nullptr;
     double* psi = nullptr;  // Elastic::m_psi_mf of the level (cells), when set
     float* psif = nullptr;
     uint16_t* gid = nullptr;  // per-grain model (finest level): grain ids and the grain table
@@ -117,8 +120,7 @@ struct alamo_mg {
     std::vector<void*> allocs;
 };
 
-// align >= 2: rows of >= 4 align nodes get a pitch rounded up to a multiple of align nodes
-static Geom make_geom(const int* n, const double* dx, int align = 0) {
+static Geom make_geom(const int* n, const double* dx) {
     Geom g{};
     for (int d = 0; d < 3; d++) {
         g.n[d] = n[d];
@@ -126,7 +128,6 @@ static Geom make_geom(const int* n, const double* dx, int align = 0) {
         g.dx[d] = dx[d];
     }
     g.sy = g.N[0];
-    if (align >= 2 && g.N[0] >= 4 * align) g.sy = (g.N[0] + align - 1) / align * align;
     g.sz = g.sy * g.N[1];
     g.tot = g.sz * g.N[2];
     return g;
@@ -140,7 +141,6 @@ static int dalloc(alamo_mg* h, T** p, int64_t n) {
     void* q = nullptr;
     HIPCHK(hipMalloc(&q, std::max<int64_t>(1, n) * sizeof(T)));
     h->allocs.push_back(q);
-    // zero, so the pad entries of a pitched layout stay zero in every flat reduction
     HIPCHK(hipMemset(q, 0, std::max<int64_t>(1, n) * sizeof(T)));
     HIPCHK(hipDeviceSynchronize());  // complete before work on any stream uses the buffer
     *p = (T*)q;
@@ -175,14 +175,14 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
     if (sym == ISO) {
         // isotropic levels of >= 4 cells per direction take the row kernel (every index it forms
         // is clamped into the level); the row kernel addresses each array with 32-bit byte
-        // offsets, so larger arrays, and ALAMO_ISO_KERNEL=tiles, take the LDS-tiled kernel
+        // offsets, so arrays of 4 GiB and more (beyond ~812^3 FP64 nodes) take the LDS-tiled kernel
         if (mode != DIAG) {
             const bool huge = (int64_t)sizeof(T) * g.tot >= ((int64_t)1 << 32);
-            if ((g_iso_kernel == 1 || huge) && iso_tiled_ok(g)) {
+            if ((huge || g_iso_tiles) && iso_tiled_ok(g)) {
                 launch_iso_tiled<T>(mode, g, bc, C, u, b, out, omega, s);
                 return;
             }
-            if (!huge && g_iso_kernel != 1 && iso_rows_ok(g)) {
+            if (!huge && !g_iso_tiles && iso_rows_ok(g)) {
                 launch_iso_rows<T>(mode, g, bc, C, u, b, out, omega, s);
                 return;
             }
@@ -256,6 +256,7 @@ template <> struct LB<double> {
     static double*& rescor(LevelBuf& l) { return l.rescor; }
     static double*& tmp(LevelBuf& l) { return l.tmp; }
     static double*& hold(LevelBuf& l) { return l.hold; }
+    static double*& dsm(LevelBuf& l) { return l.dsm; }
 };
 template <> struct LB<float> {
     static float*& C(LevelBuf& l) { return l.Cf; }
@@ -265,7 +266,16 @@ template <> struct LB<float> {
     static float*& rescor(LevelBuf& l) { return l.rescorf; }
     static float*& tmp(LevelBuf& l) { return l.tmpf; }
     static float*& hold(LevelBuf& l) { return l.holdf; }
+    static float*& dsm(LevelBuf& l) { return l.dsmf; }
 };
+
+// Diagonal and damping of a first sweep from a zero guess, x = w b / d: the prepared point
+// diagonal with omega (the reference smoother), or the l1 smoother diagonal with weight 1.
+template <typename T>
+static const T* sweep_diag(alamo_mg* h, LevelBuf& lb) {
+    return h->d.smoother == ALAMO_SMOOTHER_L1 ? LB<T>::dsm(lb) : LB<T>::D(lb);
+}
+static double sweep_omega(const alamo_mg* h) { return h->d.smoother == ALAMO_SMOOTHER_L1 ? 1.0 : h->d.omega; }
 
 template <typename T>
 static void sweeps(alamo_mg* h, int l, int count) {
@@ -287,8 +297,8 @@ static void smooth(alamo_mg* h, int l, int ncalls) {
     sweeps<T>(h, l, ncalls * sps(h, l));
 }
 
-// First damped-Jacobi sweep from a zero correction: A 0 = 0, so x = omega b / D pointwise (the
-// prepared diagonal; pads of a pitched layout hold D = 0 and stay 0).  b_out, when given,
+// First smoothing sweep from a zero correction: A 0 = 0, so x = omega b / D pointwise (the
+// prepared point diagonal with omega, or the l1 smoother diagonal with 1; D = 0 leaves 0).  b_out, when given,
 // receives b in the cycle precision, so the same pass stages the cycle's right-hand side.
 template <typename Ti, typename T>
 __global__ void k_jacobi_from_zero(const Ti* __restrict__ b_in, T* __restrict__ b_out, T* __restrict__ x,
@@ -314,7 +324,7 @@ static void presmooth_from_zero(alamo_mg* h, int l, bool first_done) {
     }
     if (!first_done)
         k_jacobi_from_zero<T, T><<<gs_blocks(n), 256, 0, h->stream>>>(LB<T>::res(lb), nullptr, LB<T>::cor(lb),
-                                                                      LB<T>::D(lb), n, T(h->d.omega));
+                                                                      sweep_diag<T>(h, lb), n, T(sweep_omega(h)));
     sweeps<T>(h, l, total - 1);
 }
 
@@ -353,8 +363,8 @@ static void vcycle(alamo_mg* h, int top, bool first_done = false) {
         // the restriction also applies the next level's first pre-smoothing sweep (from zero)
         done = l + 1 < bot && h->d.pre_smooth * sps(h, l + 1) > 0;
         k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lb.g, 3, LB<T>::rescor(lb), LB<T>::res(lc),
-                                                                 done ? LB<T>::cor(lc) : nullptr, LB<T>::D(lc),
-                                                                 T(h->d.omega));
+                                                                 done ? LB<T>::cor(lc) : nullptr, sweep_diag<T>(h, lc),
+                                                                 T(sweep_omega(h)));
     }
     bottom<T>(h);
     for (int l = bot - 1; l >= top; l--) {
@@ -444,7 +454,6 @@ static bool dense_invert(const std::vector<double>& A, int64_t n, std::vector<do
 
 // solver.bottom_solver "dense": assemble the coarsest FP64 operator (with its psi and boundary
 // rows) column by column on the device, invert it on the host and keep the inverse on the device.
-// Pad entries of a pitched row layout get identity rows (their right-hand side is 0).
 static int dense_bottom_factor(alamo_mg* h) {
     LevelBuf& lb = h->L.back();
     const int64_t n = 3 * lb.g.tot;
@@ -466,8 +475,6 @@ static int dense_bottom_factor(alamo_mg* h) {
     std::vector<double> A(n * n), inv;
     HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int64_t q = 0; q < n; q++)
-        if ((q % lb.g.tot) % lb.g.sy >= lb.g.N[0]) A[q * n + q] = 1.0;
     if (!dense_invert(A, n, inv))
         FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest-level operator is singular (no displacement "
                             "condition pins it); use bottom_solver bicgstab or smoother");
@@ -498,9 +505,29 @@ static int prepare(alamo_mg* h) {
             lb.g.psi_small = h->psi_small;
         }
     }
+    const bool l1 = h->d.smoother == ALAMO_SMOOTHER_L1;
+    if (l1 && h->psi_set) FAIL(ALAMO_ERR_ARG, "the l1 smoother has no psi terms; use smoother JACOBI with SetPsi");
     for (int l = 0; l < nl; l++) {
         LevelBuf& lb = h->L[l];
         launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, lb.D, 0.0, h->stream);
+        lb.g.dsm = nullptr;
+        lb.g.dsmf = nullptr;
+        if (l1) {
+            // the smoother diagonal, from the level's coefficients and boundary table
+            if (!lb.dsm) TRY(dalloc(h, &lb.dsm, 3 * lb.g.tot));
+            const unsigned nb = nblocks(lb.g.tot);
+            if (h->sym == ISO) k_l1_diag<double, ISO><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
+            else if (h->sym == MAJORMINOR)
+                k_l1_diag<double, MAJORMINOR><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
+            else k_l1_diag<double, DIAGONAL><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
+            lb.g.dsm = lb.dsm;
+            if (h->d.precision == ALAMO_PREC_MIXED) {
+                if (!lb.dsmf) TRY(dalloc(h, &lb.dsmf, 3 * lb.g.tot));
+                k_convert<double, float><<<gs_blocks(3 * lb.g.tot), 256, 0, h->stream>>>(lb.dsm, lb.dsmf,
+                                                                                         3 * lb.g.tot);
+                lb.g.dsmf = lb.dsmf;
+            }
+        }
         if (h->d.precision == ALAMO_PREC_MIXED) {
             k_convert<double, float><<<gs_blocks(h->nc * lb.g.tot), 256, 0, h->stream>>>(lb.C, lb.Cf,
                                                                                          h->nc * lb.g.tot);
@@ -593,8 +620,8 @@ static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, in
 // first, so z takes the place of the partner when that count (pre + post sweeps - 1) is odd and
 // of the correction when it is even, and the cycle's last sweep writes z (no copy).  A V-cycle
 // only reads its level-0 right-hand side, so when b is already in the cycle precision the level
-// reads b in place; otherwise the first sweep's pass stages it.  z must keep zeros in the pad
-// entries of a pitched layout; the level's own buffers are restored afterwards.
+// reads b in place; otherwise the first sweep's pass stages it.  The level's own buffers are
+// restored afterwards.
 template <typename Ti, typename T>
 static void vcycle_into(alamo_mg* h, const Ti* b, T* z) {
     LevelBuf& l0 = h->L[0];
@@ -608,8 +635,8 @@ static void vcycle_into(alamo_mg* h, const Ti* b, T* z) {
     T* stage = nullptr;
     if constexpr (std::is_same<Ti, T>::value) LB<T>::res(l0) = const_cast<T*>(b);
     else stage = own_res;
-    k_jacobi_from_zero<Ti, T><<<gs_blocks(n), 256, 0, h->stream>>>(b, stage, LB<T>::cor(l0), LB<T>::D(l0), n,
-                                                                   T(h->d.omega));
+    k_jacobi_from_zero<Ti, T><<<gs_blocks(n), 256, 0, h->stream>>>(b, stage, LB<T>::cor(l0), sweep_diag<T>(h, l0),
+                                                                   n, T(sweep_omega(h)));
     vcycle<T>(h, 0, true);
     LB<T>::cor(l0) = own_cor;
     LB<T>::tmp(l0) = own_tmp;
@@ -630,8 +657,8 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     } else if (h->d.precision == ALAMO_PREC_MIXED) {
         // a V-cycle with pre-smoothing stages its FP32 right-hand side in its first sweep's pass
         if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
-            k_jacobi_from_zero<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, l0.corf, l0.Df, n,
-                                                                                  (float)h->d.omega);
+            k_jacobi_from_zero<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(
+                v, l0.resf, l0.corf, sweep_diag<float>(h, l0), n, (float)sweep_omega(h));
             vcycle<float>(h, 0, true);
         } else {
             k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(v, l0.resf, n);
@@ -989,7 +1016,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->bottom_max_iter = 200;
     d->bottom_tol_rel = 1e-4;
     d->verbose = 0;
-    d->row_align = 0;
+    d->smoother = ALAMO_SMOOTHER_JACOBI;
     d->fixed_iter = 0;
     d->bottom_solver = ALAMO_BOTTOM_BICGSTAB;
     d->max_fmg_iter = 100000000;
@@ -1015,6 +1042,7 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
+    if (d->smoother != ALAMO_SMOOTHER_JACOBI && d->smoother != ALAMO_SMOOTHER_L1) FAIL(ALAMO_ERR_ARG, "unknown smoother");
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -1025,12 +1053,11 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     int n[3] = {d->n_cell[0], d->n_cell[1], d->n_cell[2]};
     double dx[3] = {d->dx[0], d->dx[1], d->dx[2]};
     std::vector<Geom> gs;
-    if (d->row_align < 0 || d->row_align > 1024) FAIL(ALAMO_ERR_ARG, "row_align must be in 0..1024");
-    gs.push_back(make_geom(n, dx, d->row_align));
+    gs.push_back(make_geom(n, dx));
     while ((d->max_coarsening_level < 0 || (int)gs.size() <= d->max_coarsening_level) && n[0] % 2 == 0 &&
            n[1] % 2 == 0 && n[2] % 2 == 0 && n[0] / 2 >= 2 && n[1] / 2 >= 2 && n[2] / 2 >= 2) {
         for (int q = 0; q < 3; q++) { n[q] /= 2; dx[q] *= 2; }
-        gs.push_back(make_geom(n, dx, d->row_align));
+        gs.push_back(make_geom(n, dx));
     }
     const bool mixed = d->precision == ALAMO_PREC_MIXED;
     h->L.resize(gs.size());
@@ -1112,8 +1139,7 @@ int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3) {
 int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
     if (!h || !d) FAIL(ALAMO_ERR_ARG, "null argument");
     const alamo_mg_desc& c = h->d;
-    bool same = d->sym == c.sym && d->precision == c.precision && d->max_coarsening_level == c.max_coarsening_level &&
-                d->row_align == c.row_align;
+    bool same = d->sym == c.sym && d->precision == c.precision && d->max_coarsening_level == c.max_coarsening_level;
     for (int q = 0; q < 3; q++) same = same && d->n_cell[q] == c.n_cell[q] && d->dx[q] == c.dx[q];
     for (int f = 0; f < 26; f++)
         for (int q = 0; q < 3; q++)
@@ -1125,7 +1151,10 @@ int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
     if (d->bottom_solver < ALAMO_BOTTOM_BICGSTAB || d->bottom_solver > ALAMO_BOTTOM_DENSE)
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
-    if (d->bottom_solver != h->d.bottom_solver) h->prepared = false;  // the dense inverse is built at prepare
+    if (d->smoother != ALAMO_SMOOTHER_JACOBI && d->smoother != ALAMO_SMOOTHER_L1) FAIL(ALAMO_ERR_ARG, "unknown smoother");
+    // the dense inverse and the smoother diagonals are built at prepare
+    if (d->bottom_solver != h->d.bottom_solver || d->smoother != h->d.smoother) h->prepared = false;
+    h->d.smoother = d->smoother;
     h->d.bottom_solver = d->bottom_solver;
     h->d.max_fmg_iter = d->max_fmg_iter;
     h->d.sweeps_finest = d->sweeps_finest;
@@ -1382,7 +1411,6 @@ int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double
     LevelBuf& lb = h->L[mglev];
     double* tmp = nullptr;
     HIPCHK(hipMallocAsync((void**)&tmp, 3 * lb.g.tot * sizeof(double), (hipStream_t)stream));
-    if (lb.g.sy != lb.g.N[0]) vfill<double>(tmp, 3 * lb.g.tot, 0.0, (hipStream_t)stream);  // zero pads
     double* a = x;
     double* c = tmp;
     for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
@@ -1507,7 +1535,7 @@ int alamo_newton_solve(alamo_mg* h, double* u, const double* b, const double* F0
     HIPCHK(hipMallocAsync((void**)&dsol, n * sizeof(double), h->stream));
     HIPCHK(hipMallocAsync((void**)&rhs, n * sizeof(double), h->stream));
     vfill<double>(dsol, n, 0.0, h->stream);  // zeroed once (Newton.H:313)
-    vfill<double>(rhs, n, 0.0, h->stream);   // pads of a pitched layout stay 0
+    vfill<double>(rhs, n, 0.0, h->stream);
     int rc = ALAMO_OK;
     double rel = 0.0;
     for (int it = 0; it < nriters && rc == ALAMO_OK; it++) {
@@ -1827,7 +1855,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     TRY(net_level0(n, h));
     n->train_seed = o->seed;
     LevelBuf& l0 = h->L[0];
-    // V network voxels (compact); n3 elements of a level-0 vector (pitched rows, zero pads)
+    // V network voxels; n3 elements of a level-0 vector
     const int64_t V = (int64_t)l0.g.N[0] * l0.g.N[1] * l0.g.N[2], n3 = 3 * l0.g.tot;
     const int nlev = (int)h->L.size();
     std::vector<void*> tmp;

@@ -52,6 +52,11 @@ struct Geom {
     int ngrains;
     // Elastic::SetUniform(true): the interior rows omit grad(C) . grad(u)
     int uniform;
+    // Smoother diagonal of the l1-Jacobi smoother (3 planes, FP64 and FP32 copies; null for the
+    // reference's damped point Jacobi): the row's absolute sum sum_j |a_ij| with the sign of a_ii,
+    // so a sweep is x += (b - A x) / dsm
+    const double* dsm;
+    const float* dsmf;
 };
 
 // Boundary types per region and component, packed for kernel arguments.

@@ -8,7 +8,25 @@ namespace alamo {
 
 enum Mode { APPLY = 0, RESID = 1, JACOBI = 2, DIAG = 3 };
 
-// (i, j, k) of element m of a level's array; i >= N[0] marks a pad entry of a pitched row.
+template <typename T> __device__ __forceinline__ const T* smoother_diag(const Geom& g);
+template <> __device__ __forceinline__ const double* smoother_diag<double>(const Geom& g) { return g.dsm; }
+template <> __device__ __forceinline__ const float* smoother_diag<float>(const Geom& g) { return g.dsmf; }
+
+// One smoothing update of element q from its row result f = (A x)_q: the reference's damped point
+// Jacobi x + w (b - A x) / a_qq written as (1-w) x + w (b - (f - D x)) / D (Operator/Operator.cpp:
+// 109-146), or, when the level has an l1 smoother diagonal, x + (b - f) / dsm_q (a zero row is
+// left as it is).
+template <typename T>
+__device__ __forceinline__ T jacobi_update(const Geom& g, int64_t q, T x, T b, T f, T D, T omega) {
+    const T* ds = smoother_diag<T>(g);
+    if (ds) {
+        const T d = ds[q];
+        return d != T(0) ? x + (b - f) / d : x;
+    }
+    return (T(1) - omega) * x + omega * (b - (f - D * x)) / D;
+}
+
+// (i, j, k) of element m of a level's array.
 // Levels below 2^31 elements (every level up to 1290^3 nodes) divide in 32 bits: a 64-bit
 // division is a long software sequence on the GPU, a 32-bit one a few instructions.
 __device__ __forceinline__ void node_ijk(const Geom& g, int64_t m, int& i, int& j, int& k) {
@@ -76,12 +94,22 @@ __global__ void __launch_bounds__(256) k_node(Geom g, BCTable bc, const T* __res
         if (MODE == APPLY) out[q] = f[p];
         else if (MODE == RESID) out[q] = b[q] - f[p];
         else if (MODE == DIAG) out[q] = D[p];
-        else {
-            const T x = u[q];
-            const T Rx = f[p] - D[p] * x;
-            out[q] = (T(1) - omega) * x + omega * (b[q] - Rx) / D[p];
-        }
+        else out[q] = jacobi_update(g, q, u[q], b[q], f[p], D[p], omega);
     }
+}
+
+// l1-Jacobi smoother diagonal of every node of the level (3 planes)
+template <typename T, int SYM>
+__global__ void __launch_bounds__(256) k_l1_diag(Geom g, BCTable bc, const T* __restrict__ C, T* __restrict__ out) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
+    T D[3];
+    l1_row<T, SYM>(g, bc, C, i, j, k, D);
+#pragma unroll
+    for (int p = 0; p < 3; p++) out[p * g.tot + m] = D[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -375,10 +403,7 @@ __global__ void __launch_bounds__(256) k_mm_interior(Geom g, const T* __restrict
         const int64_t qq = p * g.tot + m;
         if (MODE == APPLY) out[qq] = f[p];
         else if (MODE == RESID) out[qq] = b[qq] - f[p];
-        else {
-            const T x = u[qq];
-            out[qq] = (T(1) - omega) * x + omega * (b[qq] - (f[p] - D[p] * x)) / D[p];
-        }
+        else out[qq] = jacobi_update(g, qq, u[qq], b[qq], f[p], D[p], omega);
     }
 }
 
@@ -415,10 +440,7 @@ __global__ void __launch_bounds__(256) k_mm_interior_gid(Geom g, const T* __rest
         const int64_t qq = p * g.tot + m;
         if (MODE == APPLY) out[qq] = f[p];
         else if (MODE == RESID) out[qq] = b[qq] - f[p];
-        else {
-            const T x = u[qq];
-            out[qq] = (T(1) - omega) * x + omega * (b[qq] - (f[p] - D[p] * x)) / D[p];
-        }
+        else out[qq] = jacobi_update(g, qq, u[qq], b[qq], f[p], D[p], omega);
     }
 }
 
@@ -433,7 +455,7 @@ __global__ void k_scatter_u16(Geom g, const uint16_t* __restrict__ src, uint16_t
 template <typename T>
 __global__ void k_divide(T* __restrict__ x, const T* __restrict__ D, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        if (D[q] != T(0)) x[q] /= D[q];  // pad entries of a pitched layout hold D = 0
+        if (D[q] != T(0)) x[q] /= D[q];
 }
 // y = a + b
 template <typename T>

@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(ISO_TX* ISO_TY) k_iso_interior(Geom g, const T
                     const T w2[3] = {idx * idx, idy * idy, idz * idz};
                     const T D = T(-2) * ((lam + T(2) * mu) * w2[p] + mu * (w2[0] + w2[1] + w2[2] - w2[p]));
                     const T x = cur.u[p];
-                    out[q] = (T(1) - omega) * x + omega * (b[q] - (f - D * x)) / D;
+                    out[q] = jacobi_update(g, q, x, b[q], f, D, omega);
                 }
             }
         }
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_boundary(Geom g, BCTable bc, const T* _
         else if (MODE == RESID) out[qq] = b[qq] - f[p];
         else {
             const T x = u[qq];
-            out[qq] = (T(1) - omega) * x + omega * (b[qq] - (f[p] - D[p] * x)) / D[p];
+            out[qq] = jacobi_update(g, qq, x, b[qq], f[p], D[p], omega);
         }
     }
 }

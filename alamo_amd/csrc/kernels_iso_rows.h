@@ -4,21 +4,18 @@
 // Set/Matrix4_Isotropic.H:137-173):
 //   f_i = mu (Lap u_i + d_i div u) + lambda d_i div u + d_i(lambda) div u + sum_d d_d(mu) (u_i,d + u_d,i)
 // Work split:
-//   * one wavefront = one x row segment of 64 nodes (x0-1 .. x0+62); lanes 1..62 produce
-//     output, lanes 0 and 63 are the x halo.  x neighbours come from the neighbouring lane
-//     through DPP wavefront shifts, never from memory;
-//   * a workgroup = ROWS_RW such rows stacked in y; each plane's (u, lambda, mu) records of
-//     the rows (plus one halo row each side) are staged in LDS, where only the y neighbours
-//     read them.  Three plane slots, one barrier per plane;
-//   * z is marched serially over a chunk of planes; each lane keeps a three-plane register
-//     queue of u, lambda, mu and the in-plane differences that the x-z and y-z cross
-//     derivatives need (d_xz u = D_z(D_x u), exactly the four-corner formula).
-// HBM sees each value once plus the tile halos: (RW + 2) / RW in y, 64 / 62 in x.
-// When both x faces are Dirichlet in every component, their face rows (f = u) are written
-// here too, so the boundary kernel only visits the y and z faces.
+//   * one wavefront = one x row segment of 64 output nodes x0 .. x0 + 63 (x0 = 64 bx); x
+//     neighbours come from the neighbouring lane through DPP wavefront shifts, the two edge
+//     lanes' outer neighbours from LDS;
+//   * a workgroup = RW such rows stacked in y; each plane's (u, lambda, mu) records of the rows
+//     (plus one halo row each side) are staged in LDS, where the y neighbours read them.  Three
+//     plane slots, one barrier per plane;
+//   * z is marched serially over a chunk of planes; each lane keeps a plane register queue of u,
+//     lambda, mu and the in-plane differences that the x-z and y-z cross derivatives need
+//     (d_xz u = D_z(D_x u), exactly the four-corner formula).
+// When both x faces are Dirichlet in every component, their face rows (f = u) are written here
+// too, so the boundary kernel only visits the y and z faces.
 #pragma once
-#include <cstdlib>
-#include <cstring>
 #include <type_traits>
 
 #include "kernels_iso.h"
@@ -31,10 +28,8 @@ namespace alamo {
 #ifndef ALAMO_ROWS_KCH
 #define ALAMO_ROWS_KCH 16
 #endif
-constexpr int ROWS_RW = ALAMO_ROWS_RW;    // wavefronts (rows) per workgroup
-constexpr int ROWS_KCH = ALAMO_ROWS_KCH;  // z planes per workgroup
-constexpr int ROWS_OUT = 62;              // output nodes per row segment
-static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row each");
+constexpr int ROWS_RW = ALAMO_ROWS_RW;    // default wavefronts (rows) per workgroup
+constexpr int ROWS_KCH = ALAMO_ROWS_KCH;  // default z planes per workgroup
 
 #ifndef ALAMO_ROWS_XCD
 #define ALAMO_ROWS_XCD 0
@@ -42,7 +37,6 @@ static_assert(ROWS_RW >= 2, "the first and the last wavefront stage one halo row
 #ifndef ALAMO_ROWS_WPE
 #define ALAMO_ROWS_WPE 1
 #endif
-#define ALAMO_ROWS_ATTR __launch_bounds__(64 * ROWS_RW) __attribute__((amdgpu_waves_per_eu(ALAMO_ROWS_WPE)))
 
 // rows per workgroup and planes per chunk of the aligned-tile kernel below the large-level
 // class, per precision.  Measured at 129^3 nodes in one process (4 x 16 in the hardware order in
@@ -113,174 +107,13 @@ __device__ __forceinline__ void stb(T* __restrict__ base, uint32_t boff, T v) {
     *(T*)((char*)base + boff) = v;
 }
 
-// FOLD: 0 = interior rows only; 1 = also the x-face rows (both x faces Dirichlet).
-// The z loop is unrolled by four over four register records, so the queue rotates by renaming
-// and each plane is fetched straight into the record that holds it.
-template <typename T, int MODE, int FOLD>
-__global__ void ALAMO_ROWS_ATTR k_iso_rows(Geom g, const T* __restrict__ C, const T* __restrict__ u,
-                                           const T* __restrict__ b, T* __restrict__ out, T omega) {
-    constexpr int NR = ROWS_RW + 2;  // staged rows: halo, RW rows, halo
-    constexpr int A = NR * 64;
-    __shared__ T lds[3][5][A];  // [plane slot][u_x, u_y, u_z, lambda, mu][row * 64 + lane]
-    const int lane = threadIdx.x, w = threadIdx.y;
-    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-    if (ALAMO_ROWS_XCD) {
-        // neighbouring tiles (x fastest, then y, then the z chunk) share an XCD and its L2
-        const int gx = gridDim.x, gy = gridDim.y;
-        const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
-        bx = t % gx;
-        by = (t / gx) % gy;
-        bz = t / (gx * gy);
-    }
-    const int x0 = 1 + ROWS_OUT * bx;  // lane l <-> node x0 - 1 + l
-    const int y0 = 1 + ROWS_RW * by;
-    const int k0 = 1 + ROWS_KCH * bz;
-    const int k1 = min(k0 + ROWS_KCH, g.n[2]);  // output planes k0 .. k1 - 1; loads k0 - 1 .. k1
-    const int x = x0 - 1 + lane;
-    const int xl = min(x, g.n[0]);
-    const int y = y0 + w;
-    const bool row_out = y <= g.n[1] - 1;
-    const bool out_i = row_out && lane >= 1 && lane <= ROWS_OUT && x <= g.n[0] - 1;
-    const bool out_f = FOLD >= 1 && row_out && (x == 0 || x == g.n[0]);
-    constexpr uint32_t E = sizeof(T);
-    const uint32_t szb = (uint32_t)g.sz * E;
-    const uint32_t colb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)min(y, g.n[1])) * E;
-    // the first and last wavefronts also stage the halo rows
-    const int hrow = w == 0 ? 0 : (w == ROWS_RW - 1 ? NR - 1 : -1);
-    const uint32_t hcolb = ((uint32_t)xl + (uint32_t)g.sy * (uint32_t)(w == 0 ? y0 - 1 : min(y0 + ROWS_RW, g.n[1]))) * E;
-    const T* __restrict__ U[3] = {u, u + g.tot, u + 2 * g.tot};
-    const T* __restrict__ la = C;
-    const T* __restrict__ ma = C + g.tot;
-    const T* __restrict__ B[3] = {b, b + g.tot, b + 2 * g.tot};
-    T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
-    const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
-    const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
-    const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
-    const int ln = lane < 63 ? lane + 1 : 63, lw = lane > 0 ? lane - 1 : 0;
-    const int rs = w * 64 + lane, rc = (w + 1) * 64 + lane, rn = (w + 2) * 64 + lane;
-
-    T ph[5];  // prefetched halo row of the next plane
-    auto fetch = [&](int p, RowPlane<T>& d) {
-        const uint32_t o = colb + szb * (uint32_t)p;
-#pragma unroll
-        for (int v = 0; v < 3; v++) d.u[v] = ldb(U[v], o);
-        d.lam = ldb(la, o);
-        d.mu = ldb(ma, o);
-        if (hrow >= 0) {
-            const uint32_t oh = hcolb + szb * (uint32_t)p;
-#pragma unroll
-            for (int v = 0; v < 3; v++) ph[v] = ldb(U[v], oh);
-            ph[3] = ldb(la, oh);
-            ph[4] = ldb(ma, oh);
-        }
-    };
-    // one plane step: stage plane p (already in qn) into its LDS slot, prefetch plane p + 1
-    // into qx, finish qn's differences and (from the third step on) write plane c = p - 1 from
-    // qm (c - 1), qc (c) and qn (c + 1)
-    auto step = [&](int p, const RowPlane<T>& qm, const RowPlane<T>& qc, RowPlane<T>& qn, RowPlane<T>& qx) {
-        const int sl = (p - k0 + 1) % 3;
-        T(*sp)[A] = lds[sl];
-#pragma unroll
-        for (int v = 0; v < 3; v++) sp[v][rc] = qn.u[v];
-        sp[3][rc] = qn.lam;
-        sp[4][rc] = qn.mu;
-        if (hrow >= 0) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) sp[v][hrow * 64 + lane] = ph[v];
-        }
-        if (p < k1) fetch(p + 1, qx);
-        qn.dxux = (from_right(qn.u[0]) - from_left(qn.u[0])) * hx;
-        qn.dxuz = (from_right(qn.u[2]) - from_left(qn.u[2])) * hx;
-        const uint32_t mcb = colb + szb * (uint32_t)(p - 1);
-        T bq[3] = {T(0), T(0), T(0)};
-        if (MODE != APPLY && p >= k0 + 1 && (out_i || out_f)) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
-        }
-        __syncthreads();
-        qn.dyuy = (sp[1][rn] - sp[1][rs]) * hy;
-        qn.dyuz = (sp[2][rn] - sp[2][rs]) * hy;
-        if (p < k0 + 1) return;
-        // output plane c = p - 1
-        const T(*sc)[A] = lds[sl == 0 ? 2 : sl - 1];
-        // x neighbours of the centre plane; DPP reads of a lane whose exec bit is off
-        // return 0, so every shift runs with the whole wavefront active
-        T ue[3], uw[3];
-#pragma unroll
-        for (int v = 0; v < 3; v++) {
-            ue[v] = from_right(qc.u[v]);
-            uw[v] = from_left(qc.u[v]);
-        }
-        const T lx = (from_right(qc.lam) - from_left(qc.lam)) * hx;
-        const T mx = (from_right(qc.mu) - from_left(qc.mu)) * hx;
-        if (out_i) {
-            T ux[3], uy[3], uz[3], lap[3];
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                const T c0 = qc.u[v], e = ue[v], wv = uw[v];
-                const T nn = sc[v][rn], ss = sc[v][rs];
-                ux[v] = (e - wv) * hx;
-                uy[v] = (nn - ss) * hy;
-                uz[v] = (qn.u[v] - qm.u[v]) * hz;
-                lap[v] = (e - T(2) * c0 + wv) * ix2 + (nn - T(2) * c0 + ss) * iy2 + (qn.u[v] - T(2) * c0 + qm.u[v]) * iz2;
-            }
-            const T uxx0 = (ue[0] - T(2) * qc.u[0] + uw[0]) * ix2;
-            const T uyy1 = (sc[1][rn] - T(2) * qc.u[1] + sc[1][rs]) * iy2;
-            const T uzz2 = (qn.u[2] - T(2) * qc.u[2] + qm.u[2]) * iz2;
-            const T uxy0 = (sc[0][rn - lane + ln] - sc[0][rn - lane + lw] - sc[0][rs - lane + ln] + sc[0][rs - lane + lw]) * hxy;
-            const T uxy1 = (sc[1][rn - lane + ln] - sc[1][rn - lane + lw] - sc[1][rs - lane + ln] + sc[1][rs - lane + lw]) * hxy;
-            const T uxz0 = (qn.dxux - qm.dxux) * hz, uxz2 = (qn.dxuz - qm.dxuz) * hz;
-            const T uyz1 = (qn.dyuy - qm.dyuy) * hz, uyz2 = (qn.dyuz - qm.dyuz) * hz;
-            const T gd[3] = {uxx0 + uxy1 + uxz2, uxy0 + uyy1 + uyz2, uxz0 + uyz1 + uzz2};
-            const T lam = qc.lam, mu = qc.mu;
-            const T dl[3] = {lx, (sc[3][rn] - sc[3][rs]) * hy, (qn.lam - qm.lam) * hz};
-            const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
-            const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
-            const T tr = G[0][0] + G[1][1] + G[2][2];
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                T f = mu * (lap[v] + gd[v]) + lam * gd[v];
-                f += dl[v] * tr + dm[0] * (G[v][0] + G[0][v]) + dm[1] * (G[v][1] + G[1][v]) +
-                     dm[2] * (G[v][2] + G[2][v]);
-                if (MODE == APPLY) stb(O[v], mcb, f);
-                else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
-                else {
-                    const T w2[3] = {ix2, iy2, iz2};
-                    const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
-                    stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
-                }
-            }
-        } else if (out_f) {
-            // Dirichlet face row f = u (diagonal 1)
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
-                else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
-                else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
-            }
-        }
-    };
-    RowPlane<T> qa, qb, qc, qd;  // plane p lives in record (p - k0 + 1) % 4
-    fetch(k0 - 1, qa);
-    for (int p = k0 - 1; p <= k1; p += 4) {
-        step(p, qc, qd, qa, qb);
-        if (p + 1 <= k1) step(p + 1, qd, qa, qb, qc);
-        if (p + 2 <= k1) step(p + 2, qa, qb, qc, qd);
-        if (p + 3 <= k1) step(p + 3, qb, qc, qd, qa);
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Aligned-tile variant: a wavefront row is the 64 output nodes x0 .. x0 + 63 with x0 = 64 bx,
-// so when the row pitch is a multiple of 128 bytes every row load is whole 128-byte lines
-// (62-output tiles start at arbitrary offsets and touch one line more per row, which costs
-// about a fifth of the time of the access pattern).  The x neighbours of lanes 0 and 63
-// (x0 - 1, x0 + 64) are loaded by those two lanes alone and staged at entries 0 and 65 of a
-// 66-wide LDS row; the other lanes take their x neighbours by DPP as in k_iso_rows.  The edge
-// values of all staged rows (RW + 2 rows x 2 edges x 5 arrays) are fetched by one gather of
-// the first wavefront per plane, so the halo costs one load instruction per workgroup and
-// plane.  With both x faces Dirichlet, a face node x = n0 just past the last tile is written
-// by that tile's lane 63 from its halo value.
+// x tiles: a wavefront row is the 64 output nodes x0 .. x0 + 63 with x0 = 64 bx.  The x
+// neighbours of lanes 0 and 63 (x0 - 1, x0 + 64) sit at entries 0 and 65 of a 66-wide LDS row;
+// the other lanes take their x neighbours by DPP.  The edge values of all staged rows (RW + 2
+// rows x 2 edges x 5 arrays) are fetched by one gather of the first wavefront per plane, so the
+// halo costs one load instruction per workgroup and plane.  With both x faces Dirichlet, a face
+// node x = n0 just past the last tile is written by that tile's lane 63 from its halo value.
 // ---------------------------------------------------------------------------
 constexpr int R64_W = 66;  // LDS row: halo, 64 nodes, halo
 
@@ -298,7 +131,7 @@ __device__ __forceinline__ void dirichlet_yz_node(const Geom& g, const T* __rest
         const T x = u[qq];
         if (MODE == APPLY) out[qq] = x;
         else if (MODE == RESID) out[qq] = b[qq] - x;
-        else out[qq] = (T(1) - omega) * x + omega * b[qq];
+        else out[qq] = smoother_diag<T>(g) ? b[qq] : (T(1) - omega) * x + omega * b[qq];
     }
 }
 
@@ -364,6 +197,7 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     const T* __restrict__ B[3] = {b, b + g.tot, b + 2 * g.tot};
     T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
     const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
+    const bool l1 = MODE == JACOBI && smoother_diag<T>(g) != nullptr;  // l1-Jacobi smoother level
     const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
     const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
     const int rs = w * R64_W + 1 + lane, rc = (w + 1) * R64_W + 1 + lane, rn = (w + 2) * R64_W + 1 + lane;
@@ -471,16 +305,33 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
                 else {
                     const T w2[3] = {ix2, iy2, iz2};
                     const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
-                    stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
+                    if (l1) {
+                        // l1 smoother: sign(D) (|D| + sum of the row's off-centre |a|), from the row's
+                        // coefficients: faces of u_v mu / h^2 -+ d(mu) / 2h ((lambda + 2 mu) / h^2 -+
+                        // (d lambda + 2 d mu) / 2h along v), faces of u_a -+ d(lambda)_v / 2h_a (along
+                        // a) and -+ d(mu)_a / 2h_v (along v), corners (lambda + mu) / 4 h_v h_a
+                        const T ih[3] = {idx, idy, idz}, hh[3] = {hx, hy, hz};
+                        T S = T(2) * fmax((lam + T(2) * mu) * w2[v], fabs(dl[v] + T(2) * dm[v]) * hh[v]);
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            if (a == v) continue;
+                            S += T(2) * fmax(mu * w2[a], fabs(dm[a]) * hh[a]) + fabs(dl[v]) * ih[a] +
+                                 fabs(dm[a]) * ih[v] + fabs(lam + mu) * ih[v] * ih[a];
+                        }
+                        const T Dl = D < T(0) ? D - S : D + S;
+                        stb(O[v], mcb, qc.u[v] + (bq[v] - f) / Dl);
+                    } else {
+                        stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * (bq[v] - (f - D * qc.u[v])) / D);
+                    }
                 }
             }
         } else if (out_f) {
-            // Dirichlet face row f = u (diagonal 1)
+            // Dirichlet face row f = u (diagonal 1; the l1 smoother's sweep sets x = b)
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 if (MODE == APPLY) stb(O[v], mcb, qc.u[v]);
                 else if (MODE == RESID) stb(O[v], mcb, bq[v] - qc.u[v]);
-                else stb(O[v], mcb, (T(1) - omega) * qc.u[v] + omega * bq[v]);
+                else stb(O[v], mcb, l1 ? bq[v] : (T(1) - omega) * qc.u[v] + omega * bq[v]);
             }
         }
         if (out_fh) {
@@ -489,7 +340,7 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
             for (int v = 0; v < 3; v++) {
                 if (MODE == APPLY) stb(O[v], fcb, ce[v]);
                 else if (MODE == RESID) stb(O[v], fcb, bh[v] - ce[v]);
-                else stb(O[v], fcb, (T(1) - omega) * ce[v] + omega * bh[v]);
+                else stb(O[v], fcb, l1 ? bh[v] : (T(1) - omega) * ce[v] + omega * bh[v]);
             }
         }
     };
@@ -531,11 +382,6 @@ __global__ void __launch_bounds__(256) k_dirichlet_yz(Geom g, const T* __restric
     dirichlet_yz_node<T, MODE>(g, u, b, out, omega, q);
 }
 
-// 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons)
-static const bool g_rows_tile64 = [] {
-    const char* e = getenv("ALAMO_ROWS_TILE");
-    return !(e && strcmp(e, "62") == 0);
-}();
 // row-kernel launch of one tile shape; nfaces > 0 appends the workgroups that write the y- and
 // z-face nodes of an all-Dirichlet level
 template <typename T, int MODE, int FOLD, int RW, int KCH>
@@ -553,26 +399,19 @@ static void launch_rows64(const Geom& g, const T* C, const T* u, const T* b, T* 
     }
 }
 
-// 64-output aligned tiles (default) or the 62-output tiles (ALAMO_ROWS_TILE=62, for comparisons);
-// returns whether the launch also wrote the y- and z-face nodes (nfaces > 0, 64-output tiles)
+// the row kernel with the tile shape of the level's size class; returns whether the launch also
+// wrote the y- and z-face nodes (nfaces > 0)
 template <typename T, int MODE, int FOLD>
 static bool launch_rows_kernel(const Geom& g, const T* C, const T* u, const T* b, T* out, T omega, hipStream_t s,
                                int64_t nfaces = 0) {
-    dim3 blk(64, ROWS_RW);
-    if (g_rows_tile64) {
-        // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
-        if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG)
-            launch_rows64<T, MODE, FOLD, Rows64Big<T, MODE>::RW, Rows64Big<T, MODE>::KCH>(g, C, u, b, out, omega,
-                                                                                       nfaces, s);
-        else if (g.n[1] < ROWS64_MID || g.n[2] < ROWS64_MID)
-            launch_rows64<T, MODE, FOLD, Rows64Small::RW, Rows64Small::KCH>(g, C, u, b, out, omega, nfaces, s);
-        else launch_rows64<T, MODE, FOLD, Rows64Cfg<T>::RW, Rows64Cfg<T>::KCH>(g, C, u, b, out, omega, nfaces, s);
-        return nfaces > 0;
-    }
-    dim3 grid((g.n[0] - 1 + ROWS_OUT - 1) / ROWS_OUT, (g.n[1] - 1 + ROWS_RW - 1) / ROWS_RW,
-              (g.n[2] - 1 + ROWS_KCH - 1) / ROWS_KCH);
-    k_iso_rows<T, MODE, FOLD><<<grid, blk, 0, s>>>(g, C, u, b, out, omega);
-    return false;
+    // tiles cover x = 0 .. 64 ceil(n0 / 64) - 1, every interior node and the x = 0 face
+    if (g.n[1] >= ROWS64_BIG && g.n[2] >= ROWS64_BIG)
+        launch_rows64<T, MODE, FOLD, Rows64Big<T, MODE>::RW, Rows64Big<T, MODE>::KCH>(g, C, u, b, out, omega, nfaces,
+                                                                                   s);
+    else if (g.n[1] < ROWS64_MID || g.n[2] < ROWS64_MID)
+        launch_rows64<T, MODE, FOLD, Rows64Small::RW, Rows64Small::KCH>(g, C, u, b, out, omega, nfaces, s);
+    else launch_rows64<T, MODE, FOLD, Rows64Cfg<T>::RW, Rows64Cfg<T>::KCH>(g, C, u, b, out, omega, nfaces, s);
+    return nfaces > 0;
 }
 
 template <typename T, int MODE>

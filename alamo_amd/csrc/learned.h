@@ -397,8 +397,7 @@ __global__ void k_prolong_T(Grid3 gf, Grid3 gc, int C, const float* __restrict__
     gcout[q] = s;
 }
 
-// Voxel v of the network (compact, V = N0 N1 N2 voxels) is node node_offset(g, v) of the
-// operator level g, whose rows may be pitched.
+// Voxel v of the network (V = N0 N1 N2 voxels) is node node_offset(g, v) of the operator level g.
 // Network input: channels 0-2 = r / s (r FP64, 3 SoA planes), 3-4 = coefficient features,
 // 5-15 = 0.  feat holds 2 f32 per voxel.
 __global__ void k_pack_input(Geom g, int64_t V, const double* __restrict__ r, double inv_s,

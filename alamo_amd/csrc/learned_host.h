@@ -64,7 +64,7 @@ static const bool g_conv_lds = [] {
 static int net_alloc(std::vector<void*>& list, void** p, size_t bytes) {
     HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 16)));
     list.push_back(*p);
-    HIPCHK(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));  // pad entries of pitched vectors stay zero
+    HIPCHK(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));
     HIPCHK(hipDeviceSynchronize());
     return ALAMO_OK;
 }
@@ -375,7 +375,7 @@ static int net_level0(alamo_net* n, alamo_mg* h) {
 // FP64 vectors: q, A y, y.
 static int net_correct(alamo_net* n, alamo_mg* h, const double* r, double* z, double* w) {
     LevelBuf& l0 = h->L[0];
-    // V network voxels (compact); n3 elements of a level-0 vector (pitched rows, zero pads)
+    // V network voxels; n3 elements of a level-0 vector
     const int64_t V = (int64_t)l0.g.N[0] * l0.g.N[1] * l0.g.N[2], n3 = 3 * l0.g.tot;
     double *q = w, *Ay = w + n3, *y = w + 2 * n3;
     TRY(net_level0(n, h));

@@ -171,6 +171,110 @@ __device__ __forceinline__ void diag_interior(const Geom& g, const T* c, T D[3])
     }
 }
 
+// ---------------------------------------------------------------------------
+// l1-Jacobi smoother diagonal: for each row component q, sign(a_qq) sum_j |a_qj| over the row's
+// coefficients, assembled per (offset, component) from the same stencils as the rows above
+// (interior: second differences, the four-corner mixed difference and central first
+// differences of u and C; boundary: the BC row with GetStencil first derivatives).  Offsets are
+// (dx, dy, dz) in {-1, 0, 1}^3, slot 13 + dx + 3 dy + 9 dz.  No psi terms (levels with psi keep
+// the point smoother).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T l1_finish(const T acc[27][3], int q) {
+    T s = T(0);
+#pragma unroll
+    for (int o = 0; o < 27; o++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) s += fabs(acc[o][k]);
+    return acc[13][q] < T(0) ? -s : s;
+}
+
+template <typename T, int SYM>
+__device__ void l1_row(const Geom& g, const BCTable& bc, const T* __restrict__ C, int i, int j, int k, T Dl1[3]) {
+    constexpr int NC = NCoef<SYM>::value;
+    const int64_t m = i + g.sy * j + g.sz * k;
+    const int hi[3] = {g.n[0], g.n[1], g.n[2]};
+    const int idx[3] = {i, j, k};
+    const int sl[3] = {1, 3, 9};
+    const T dx[3] = {T(g.dx[0]), T(g.dx[1]), T(g.dx[2])};
+    T c[NC];
+    load_coef<T, SYM>(C, g.tot, m, c);
+    const bool bnd = i == 0 || j == 0 || k == 0 || i == hi[0] || j == hi[1] || k == hi[2];
+    if (bnd) {
+        const int face = classify(i, j, k, hi);
+        double nd[3];
+        region_normal(face, nd);
+        for (int q = 0; q < 3; q++) {
+            T acc[27][3] = {};
+            const int t = bc.t[face][q];
+            // first-derivative stencil along l at this node (Numeric::GetStencil): weight w at offset
+            // sign s of slot 13 + s sl[l]
+            auto d1w = [&](int l, T coef, int comp) {
+                if (idx[l] == 0) {
+                    acc[13 + sl[l]][comp] += coef / dx[l];
+                    acc[13][comp] -= coef / dx[l];
+                } else if (idx[l] == hi[l]) {
+                    acc[13][comp] += coef / dx[l];
+                    acc[13 - sl[l]][comp] -= coef / dx[l];
+                } else {
+                    acc[13 + sl[l]][comp] += coef * T(0.5) / dx[l];
+                    acc[13 - sl[l]][comp] -= coef * T(0.5) / dx[l];
+                }
+            };
+            if (t == DISPLACEMENT) {
+                acc[13][q] = T(1);
+            } else if (t == TRACTION) {
+                // (C : grad u) n: coefficient of u_kk,l is sum_e n_e C_q e kk l
+                for (int kk = 0; kk < 3; kk++)
+                    for (int l = 0; l < 3; l++) {
+                        T coef = T(0);
+                        for (int e = 0; e < 3; e++) coef += T(nd[e]) * C4<SYM>(c, q, e, kk, l);
+                        d1w(l, coef, kk);
+                    }
+            } else if (t == NEUMANN) {
+                for (int l = 0; l < 3; l++) d1w(l, T(nd[l]), q);
+            }
+            Dl1[q] = l1_finish(acc, q);
+        }
+        return;
+    }
+    T dc[3][NC];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int64_t st = d == 0 ? 1 : (d == 1 ? g.sy : g.sz);
+#pragma unroll
+        for (int p = 0; p < NC; p++) dc[d][p] = (C[p * g.tot + m + st] - C[p * g.tot + m - st]) * T(0.5) / dx[d];
+    }
+    for (int q = 0; q < 3; q++) {
+        T acc[27][3] = {};
+        for (int kk = 0; kk < 3; kk++)
+            for (int jj = 0; jj < 3; jj++)
+                for (int ll = 0; ll < 3; ll++) {
+                    // C_q jj kk ll u_kk,ll jj
+                    const T cf = C4<SYM>(c, q, jj, kk, ll);
+                    if (jj == ll) {
+                        const T w = cf / (dx[jj] * dx[jj]);
+                        acc[13 + sl[jj]][kk] += w;
+                        acc[13 - sl[jj]][kk] += w;
+                        acc[13][kk] -= T(2) * w;
+                    } else {
+                        const T w = cf * T(0.25) / (dx[jj] * dx[ll]);
+                        acc[13 + sl[jj] + sl[ll]][kk] += w;
+                        acc[13 - sl[jj] - sl[ll]][kk] += w;
+                        acc[13 + sl[jj] - sl[ll]][kk] -= w;
+                        acc[13 - sl[jj] + sl[ll]][kk] -= w;
+                    }
+                    // (d_jj C_q jj kk ll) u_kk,ll
+                    if (!g.uniform) {
+                        const T w = C4<SYM>(dc[jj], q, jj, kk, ll) * T(0.5) / dx[ll];
+                        acc[13 + sl[ll]][kk] += w;
+                        acc[13 - sl[ll]][kk] -= w;
+                    }
+                }
+        Dl1[q] = l1_finish(acc, q);
+    }
+}
+
 // (A u) at an interior node, generic form (any SYM), direct global loads.
 template <typename T, int SYM>
 __device__ __forceinline__ void apply_interior(const Geom& g, const T* __restrict__ C, const T* __restrict__ u,

@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import KrylovOpts, MGDesc, SolveStats, check, lib
+from ._lib import KrylovOpts, MGDesc, SolveStats, check, dev_ptr as _dev, lib
 from .problems import ISO, MAJORMINOR, DIAGONAL, NCOEF
 
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
@@ -31,6 +31,19 @@ def _ptr(t):
 
 
 BOTTOM_SOLVERS = {"bicgstab": 0, "smoother": 1, "dense": 2}
+SMOOTHERS = {"jacobi": 0, "l1": 1}
+
+
+def smoother_code(name):
+    """smoother of the cycles: "jacobi" (the reference's damped point Jacobi, Operator.cpp:109-146) or
+    "l1" (l1-Jacobi, x += (b - A x) / d with d the row's absolute sum; no reference counterpart)."""
+    if isinstance(name, int):
+        if name not in SMOOTHERS.values():
+            raise ValueError("unknown smoother %d" % name)
+        return name
+    if name not in SMOOTHERS:
+        raise ValueError("smoother must be one of %s, not %r" % (sorted(SMOOTHERS), name))
+    return SMOOTHERS[name]
 
 
 def bottom_solver_code(name):
@@ -52,7 +65,7 @@ class Elastic:
 
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
-                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, row_align=0, fixed_iter=0,
+                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, smoother="jacobi", fixed_iter=0,
                  bottom_solver="bicgstab", sweeps_finest=0):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
@@ -72,7 +85,7 @@ class Elastic:
         d.pre_smooth, d.post_smooth, d.sweeps_per_smooth = pre_smooth, post_smooth, sweeps_per_smooth
         d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
         d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
-        d.row_align = int(row_align)
+        d.smoother = smoother_code(smoother)
         d.fixed_iter = int(fixed_iter)
         d.bottom_solver = bottom_solver_code(bottom_solver)
         d.sweeps_finest = int(sweeps_finest)
@@ -83,6 +96,13 @@ class Elastic:
         h = ctypes.c_void_p()
         check(lib().alamo_mg_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
+        self._size = [self.LevelLayout(l)[2] for l in range(self.NMGLevels())]
+
+    def _vec(self, t, mglev, what, ncomp=3):
+        """checked pointer of a device field of ncomp planes on level mglev"""
+        if not 0 <= mglev < len(self._size):
+            raise _lib.AlamoError(1, "mglev %d out of range" % mglev)
+        return _dev(t, ncomp * self._size[mglev], what)
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -111,15 +131,17 @@ class Elastic:
     def UpdateOpts(self, **kw):
         """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
         sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose,
-        bottom_solver, max_fmg_iter, sweeps_finest)."""
+        bottom_solver, max_fmg_iter, sweeps_finest, smoother)."""
         allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
                    "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver", "max_fmg_iter",
-                   "sweeps_finest"}
+                   "sweeps_finest", "smoother"}
         for k, v in kw.items():
             if k not in allowed:
                 raise ValueError("not a solver option: " + k)
             if k == "bottom_solver":
                 v = bottom_solver_code(v)
+            if k == "smoother":
+                v = smoother_code(v)
             setattr(self.desc, k, v)
         check(lib().alamo_mg_update_opts(self.h, ctypes.byref(self.desc)))
 
@@ -134,37 +156,8 @@ class Elastic:
         return tuple(int(v) for v in a)
 
     def LevelSize(self, mglev=0):
-        """elements per component of a device field on level mglev (nodes plus row pads)"""
+        """elements per component of a device field on level mglev (its node count)"""
         return self.LevelLayout(mglev)[2]
-
-    def to_layout(self, x, mglev=0, ncomp=3, aos=False):
-        """A compact field (``ncomp`` SoA planes, or AoS records of ``ncomp`` values, of the level's
-        nodes with i fastest; numpy or torch) in the level's device layout, pads zero.  Returns
-        ``x`` itself when the level is compact."""
-        N = [v + 1 for v in self.LevelNCell(mglev)]
-        sy = self.LevelLayout(mglev)[0]
-        if sy == N[0]:
-            return x
-        shape = (N[2], N[1], sy, ncomp) if aos else (ncomp, N[2], N[1], sy)
-        if isinstance(x, np.ndarray):
-            out = np.zeros(shape, dtype=x.dtype)
-        else:
-            import torch
-            out = torch.zeros(shape, dtype=x.dtype, device=x.device)
-        if aos:
-            out[:, :, :N[0], :] = x.reshape(N[2], N[1], N[0], ncomp)
-        else:
-            out[..., :N[0]] = x.reshape(ncomp, N[2], N[1], N[0])
-        return out.reshape(-1)
-
-    def from_layout(self, x, mglev=0, ncomp=3):
-        """The compact SoA field (``ncomp`` planes) of a field in the level's device layout."""
-        N = [v + 1 for v in self.LevelNCell(mglev)]
-        sy = self.LevelLayout(mglev)[0]
-        if sy == N[0]:
-            return x
-        y = x.reshape(ncomp, N[2], N[1], sy)[..., :N[0]]
-        return (np.ascontiguousarray(y) if isinstance(x, np.ndarray) else y.contiguous()).reshape(-1)
 
     # --- coefficients ----------------------------------------------------------
     def SetModel(self, C, layout=LAYOUT_AOS, amrlev=0):
@@ -176,11 +169,16 @@ class Elastic:
                 raise _lib.AlamoError(1, "Inconsistent # of components: got %d values, need %d" % (C.size, expect))
             check(lib().alamo_mg_set_model_host(self.h, amrlev, C.ctypes.data_as(ctypes.c_void_p), layout))
         else:
-            check(lib().alamo_mg_set_model(self.h, amrlev, _ptr(C), layout))
+            check(lib().alamo_mg_set_model(self.h, amrlev, self._vec(C, 0, "SetModel C", NCOEF[self.sym]), layout))
 
     def SetModelGrains(self, gid, table):
+        """Per-grain MajorMinor model: gid one grain id per level-0 node (i fastest), table (ngrains, 21)."""
         gid = np.ascontiguousarray(gid, dtype=np.uint16).ravel()
         table = np.ascontiguousarray(table, dtype=np.float64)
+        if gid.size != self.LevelNodes(0):
+            raise _lib.AlamoError(1, "SetModelGrains: %d grain ids for %d nodes" % (gid.size, self.LevelNodes(0)))
+        if table.ndim != 2 or table.shape[1] != 21 or table.shape[0] < 1:
+            raise _lib.AlamoError(1, "SetModelGrains: table must be (ngrains, 21), got %s" % (table.shape,))
         check(lib().alamo_mg_set_model_grains_host(self.h, gid.ctypes.data_as(ctypes.c_void_p),
                                                    table.ctypes.data_as(ctypes.c_void_p), table.shape[0]))
 
@@ -193,76 +191,88 @@ class Elastic:
                 raise _lib.AlamoError(1, "psi needs one value per cell")
             check(lib().alamo_mg_set_psi_host(self.h, amrlev, psi.ctypes.data_as(ctypes.c_void_p), float(psi_small)))
         else:
-            check(lib().alamo_mg_set_psi(self.h, amrlev, _ptr(psi), float(psi_small)))
+            check(lib().alamo_mg_set_psi(self.h, amrlev, _dev(psi, int(np.prod(self.ncell)), "SetPsi psi"),
+                                         float(psi_small)))
 
     def prepareForSolve(self):
         check(lib().alamo_mg_prepare(self.h))
 
     def GetCoeff(self, mglev, out):
-        check(lib().alamo_mg_get_coeff(self.h, mglev, _ptr(out)))
+        check(lib().alamo_mg_get_coeff(self.h, mglev, self._vec(out, mglev, "GetCoeff out", NCOEF[self.sym])))
 
     # --- operator virtuals -----------------------------------------------------
     def Fapply(self, amrlev, mglev, out, inp, stream=None):
-        check(lib().alamo_mg_fapply(self.h, amrlev, mglev, _ptr(out), _ptr(inp), stream))
+        check(lib().alamo_mg_fapply(self.h, amrlev, mglev, self._vec(out, mglev, "Fapply out"),
+                                     self._vec(inp, mglev, "Fapply in"), stream))
 
     def Apply(self, amrlev, mglev, out, inp):
         self.Fapply(amrlev, mglev, out, inp)
 
     def Diagonal(self, amrlev, mglev, diag, stream=None):
-        check(lib().alamo_mg_diagonal(self.h, amrlev, mglev, _ptr(diag), stream))
+        check(lib().alamo_mg_diagonal(self.h, amrlev, mglev, self._vec(diag, mglev, "Diagonal"), stream))
 
     def normalize(self, amrlev, mglev, x, stream=None):
         """Operator<Grid::Node>::normalize (Operator.cpp:156-165): x /= D"""
-        check(lib().alamo_mg_normalize(self.h, amrlev, mglev, _ptr(x), stream))
+        check(lib().alamo_mg_normalize(self.h, amrlev, mglev, self._vec(x, mglev, "normalize x"), stream))
 
     def Fsmooth(self, amrlev, mglev, x, b, stream=None):
-        check(lib().alamo_mg_fsmooth(self.h, amrlev, mglev, _ptr(x), _ptr(b), stream))
+        check(lib().alamo_mg_fsmooth(self.h, amrlev, mglev, self._vec(x, mglev, "Fsmooth x"),
+                                      self._vec(b, mglev, "Fsmooth b"), stream))
 
     def correctionResidual(self, amrlev, mglev, resid, x, b, stream=None):
-        check(lib().alamo_mg_residual(self.h, amrlev, mglev, _ptr(resid), _ptr(x), _ptr(b), stream))
+        check(lib().alamo_mg_residual(self.h, amrlev, mglev, self._vec(resid, mglev, "residual r"),
+                                       self._vec(x, mglev, "residual x"), self._vec(b, mglev, "residual b"), stream))
 
     def solutionResidual(self, amrlev, resid, x, b, stream=None):
         self.correctionResidual(amrlev, 0, resid, x, b, stream)
 
     def restriction(self, amrlev, cmglev, crse, fine, stream=None):
-        check(lib().alamo_mg_restrict(self.h, amrlev, cmglev, _ptr(crse), _ptr(fine), stream))
+        check(lib().alamo_mg_restrict(self.h, amrlev, cmglev, self._vec(crse, cmglev, "restriction crse"),
+                                       self._vec(fine, cmglev - 1, "restriction fine"), stream))
 
     def interpolation(self, amrlev, fmglev, fine, crse, stream=None):
-        check(lib().alamo_mg_interpolate(self.h, amrlev, fmglev, _ptr(fine), _ptr(crse), stream))
+        check(lib().alamo_mg_interpolate(self.h, amrlev, fmglev, self._vec(fine, fmglev, "interpolation fine"),
+                                          self._vec(crse, fmglev + 1, "interpolation crse"), stream))
 
     def Norms(self, mglev, x, stream=None):
         a, b = ctypes.c_double(), ctypes.c_double()
-        check(lib().alamo_mg_norms(self.h, mglev, _ptr(x), ctypes.byref(a), ctypes.byref(b), stream))
+        check(lib().alamo_mg_norms(self.h, mglev, self._vec(x, mglev, "Norms x"), ctypes.byref(a), ctypes.byref(b), stream))
         return a.value, b.value
 
     # --- post-processing (Elastic.cpp:430-603) -----------------------------------
     def Strain(self, amrlev, eps, u, voigt=False, stream=None):
         """eps = sym(grad u): 9 planes row-major, or 6 Voigt planes 00, 11, 22, 12, 20, 01"""
-        check(lib().alamo_mg_strain(self.h, amrlev, _ptr(u), _ptr(eps), int(voigt), stream))
+        check(lib().alamo_mg_strain(self.h, amrlev, self._vec(u, 0, "Strain u"),
+                                     self._vec(eps, 0, "Strain eps", 6 if voigt else 9), int(voigt), stream))
 
     def Stress(self, amrlev, sigma, u, voigt=False, stream=None):
         """sigma = (C : grad u) psi_avg"""
-        check(lib().alamo_mg_stress(self.h, amrlev, _ptr(u), _ptr(sigma), int(voigt), stream))
+        check(lib().alamo_mg_stress(self.h, amrlev, self._vec(u, 0, "Stress u"),
+                                     self._vec(sigma, 0, "Stress sigma", 6 if voigt else 9), int(voigt), stream))
 
     def Energy(self, amrlev, energy, u, stream=None):
         """energy += (C : grad u) : sym(grad u) / 2"""
-        check(lib().alamo_mg_energy(self.h, amrlev, _ptr(u), _ptr(energy), stream))
+        check(lib().alamo_mg_energy(self.h, amrlev, self._vec(u, 0, "Energy u"), self._vec(energy, 0, "Energy", 1),
+                                     stream))
 
     def TractionXhi(self, u, F0=None):
         """Base::Mechanics::Integrate trac_hi[0] (Mechanics.H:391-448): the x-hi face integral of
         sigma e_x with sigma = C : (grad u - F0); returns a 3-vector"""
         out = (ctypes.c_double * 3)()
-        check(lib().alamo_mg_trac_xhi(self.h, _ptr(u), _ptr(F0), out))
+        check(lib().alamo_mg_trac_xhi(self.h, self._vec(u, 0, "TractionXhi u"),
+                                       None if F0 is None else self._vec(F0, 0, "TractionXhi F0", 9), out))
         return np.array(list(out))
 
     # --- right-hand sides ------------------------------------------------------
     def BCInit(self, rhs, homogeneous=False):
         """BC::Operator::Elastic::Constant::Init (Constant.H:197-286)."""
-        check(lib().alamo_bc_init(self.h, _ptr(rhs), int(homogeneous)))
+        check(lib().alamo_bc_init(self.h, self._vec(rhs, 0, "BCInit rhs"), int(homogeneous)))
 
     def NewtonRHS(self, u, b, F0, rhs):
         """Newton<T>::prepareForSolve residual form (Newton.H:144-231)."""
-        check(lib().alamo_newton_rhs(self.h, _ptr(u), _ptr(b), _ptr(F0), _ptr(rhs)))
+        check(lib().alamo_newton_rhs(self.h, self._vec(u, 0, "NewtonRHS u"), self._vec(b, 0, "NewtonRHS b"),
+                                      None if F0 is None else self._vec(F0, 0, "NewtonRHS F0", 9),
+                                      self._vec(rhs, 0, "NewtonRHS rhs")))
 
     # --- flags and queries of the operator (Elastic.H:48,104-107; Operator.H:57-59,96-101) -----
     def SetUniform(self, uniform):
@@ -306,22 +316,26 @@ class Elastic:
         level-0 residual (compact layout); mask from alamo_amd.amr.fine_mask."""
         lo = np.ascontiguousarray(np.asarray(fine_lo, dtype=np.int32).reshape(3))
         hi = np.ascontiguousarray(np.asarray(fine_hi, dtype=np.int32).reshape(3))
+        nf = int(np.prod(hi - lo + 4))  # nodes of the fine box's grown nodal box [lo - 1, hi + 2]
+        nm = int(np.prod([n + 1 for n in self.ncell]))
         check(lib().alamo_mg_reflux(self.h, lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p),
-                                    _ptr(mask), _ptr(crse_res), _ptr(fine_res), _ptr(stream)))
+                                    _dev(mask, nm, "Reflux mask", "int32"), self._vec(crse_res, 0, "Reflux crse"),
+                                    _dev(fine_res, 3 * nf, "Reflux fine"), _ptr(stream)))
 
     def NormInf(self, mglev, x, stream=None):
         v = ctypes.c_double()
-        check(lib().alamo_mg_norm_inf(self.h, mglev, _ptr(x), ctypes.byref(v), _ptr(stream)))
+        check(lib().alamo_mg_norm_inf(self.h, mglev, self._vec(x, mglev, "NormInf x"), ctypes.byref(v), _ptr(stream)))
         return v.value
 
     def NormL2(self, mglev, x, stream=None):
         v = ctypes.c_double()
-        check(lib().alamo_mg_norm_l2(self.h, mglev, _ptr(x), ctypes.byref(v), _ptr(stream)))
+        check(lib().alamo_mg_norm_l2(self.h, mglev, self._vec(x, mglev, "NormL2 x"), ctypes.byref(v), _ptr(stream)))
         return v.value
 
     def Error0x(self, amrlev, mglev, R0x, x, stream=None):
         """Elastic::Error0x (Elastic.cpp:382-404): R0x = x - A (x / D)."""
-        check(lib().alamo_mg_error0x(self.h, amrlev, mglev, _ptr(R0x), _ptr(x), _ptr(stream)))
+        check(lib().alamo_mg_error0x(self.h, amrlev, mglev, self._vec(R0x, mglev, "Error0x R0x"),
+                                      self._vec(x, mglev, "Error0x x"), _ptr(stream)))
 
 
 class Linear:
@@ -422,7 +436,8 @@ class Linear:
         if self.linop is None:
             raise _lib.AlamoError(3, "Solver not defined")
         st = SolveStats()
-        rc = lib().alamo_linear_solve(self.linop.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(st))
+        rc = lib().alamo_linear_solve(self.linop.h, self.linop._vec(sol, 0, "solve sol"),
+                                      self.linop._vec(rhs, 0, "solve rhs"), tol_rel, tol_abs, ctypes.byref(st))
         if rc != 0 and raise_on_fail:
             check(rc)
         out = st.as_dict()
@@ -466,7 +481,9 @@ class Newton(Linear):
         self.linop.SetModel(C, layout)
         st = SolveStats()
         rel = ctypes.c_double(0.0)
-        rc = lib().alamo_newton_solve(self.linop.h, _ptr(u), _ptr(b), _ptr(F0), int(self.m_nriters),
+        op = self.linop
+        rc = lib().alamo_newton_solve(op.h, op._vec(u, 0, "Newton u"), op._vec(b, 0, "Newton b"),
+                                      None if F0 is None else op._vec(F0, 0, "Newton F0", 9), int(self.m_nriters),
                                       float(self.m_nrtolerance), tol_rel, tol_abs, ctypes.byref(st), ctypes.byref(rel))
         self.last_stats = st.as_dict()
         self.last_stats["rc"] = rc
@@ -485,11 +502,15 @@ class Newton(Linear):
 
     def W(self, w, u, F0):
         """Newton::W (Newton.H:528-569): w = (grad u - F0) : C : (grad u - F0) / 2 per node."""
-        check(lib().alamo_newton_w(self.linop.h, 0, _ptr(u), _ptr(F0), _ptr(w), None))
+        op = self.linop
+        check(lib().alamo_newton_w(op.h, 0, op._vec(u, 0, "W u"), None if F0 is None else op._vec(F0, 0, "W F0", 9),
+                                   op._vec(w, 0, "W w", 1), None))
 
     def DW(self, dw, u, F0):
         """Newton::DW (Newton.H:571-611): dw = C : (grad u - F0), 9 row-major planes."""
-        check(lib().alamo_newton_dw(self.linop.h, 0, _ptr(u), _ptr(F0), _ptr(dw), None))
+        op = self.linop
+        check(lib().alamo_newton_dw(op.h, 0, op._vec(u, 0, "DW u"), None if F0 is None else op._vec(F0, 0, "DW F0", 9),
+                                    op._vec(dw, 0, "DW dw", 9), None))
 
     def Parse(self, pp):
         """Newton::Parse (Newton.H:643-652): Linear::Parse, then nriters and nrtolerance."""
@@ -507,13 +528,15 @@ def blend_models(eta, table, out, layout=LAYOUT_AOS, stream=None):
     table = np.ascontiguousarray(table, dtype=np.float64)
     nmodels, nrec = table.shape
     nnodes = eta.numel() // nmodels
-    check(lib().alamo_blend_models(ctypes.c_int64(nnodes), nmodels, nrec, _ptr(eta),
-                                   table.ctypes.data_as(ctypes.c_void_p), _ptr(out), layout, stream))
+    eta_p = _dev(eta, nmodels, "blend eta")
+    check(lib().alamo_blend_models(ctypes.c_int64(nnodes), nmodels, nrec, eta_p,
+                                   table.ctypes.data_as(ctypes.c_void_p), _dev(out, nrec * nnodes, "blend out"), layout,
+                                   stream))
 
 
 def precond_apply(op, kind, r, z):
     """z = P(r) for the operator's preconditioner kind (one cycle, optionally with the network)"""
-    check(lib().alamo_mg_precond(op.h, int(kind), _ptr(r), _ptr(z)))
+    check(lib().alamo_mg_precond(op.h, int(kind), op._vec(r, 0, "precond r"), op._vec(z, 0, "precond z")))
 
 
 class Krylov:
@@ -530,7 +553,8 @@ class Krylov:
 
     def solve(self, sol, rhs, tol_rel, tol_abs, raise_on_fail=True):
         st = SolveStats()
-        rc = lib().alamo_krylov_solve(self.op.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(self.opts),
+        rc = lib().alamo_krylov_solve(self.op.h, self.op._vec(sol, 0, "Krylov sol"), self.op._vec(rhs, 0, "Krylov rhs"),
+                                      tol_rel, tol_abs, ctypes.byref(self.opts),
                                       ctypes.byref(st))
         if rc != 0 and raise_on_fail:
             check(rc)
@@ -546,7 +570,8 @@ def learned_solve(op, sol, rhs, tol_rel, tol_abs, restart=16, max_iter=400, stag
     check(lib().alamo_krylov_opts_default(ctypes.byref(o)))
     o.restart, o.max_iter, o.stagnation_ratio = restart, max_iter, stagnation_ratio
     st = SolveStats()
-    rc = lib().alamo_learned_solve(op.h, _ptr(sol), _ptr(rhs), tol_rel, tol_abs, ctypes.byref(o), ctypes.byref(st))
+    rc = lib().alamo_learned_solve(op.h, op._vec(sol, 0, "learned sol"), op._vec(rhs, 0, "learned rhs"), tol_rel, tol_abs,
+                                   ctypes.byref(o), ctypes.byref(st))
     if rc != 0 and raise_on_fail:
         check(rc)
     out = st.as_dict()

@@ -8,7 +8,7 @@ it mirrors:
 * ``sphere_indicator`` IC::Ellipse::Add node-position indicator (IC/Ellipse.H:26-86), sharp or erf form
 * ``cubic_mm``         Set::Matrix4<3,MajorMinor>::Cubic (Set/Matrix4_MajorMinor.H:518-556)
 * ``voronoi_grains``   IC::Voronoi grain partition (IC/Voronoi.H:37-118) with this package's own seeded RNG
-* ``two_phase_indicator`` Gaussian random field thresholded at 0, diffuse (no reference counterpart)
+* ``two_phase_indicator`` Gaussian random field thresholded at 0 with an erf interface (no reference counterpart)
 
 Random numbers come from SplitMix64 (published by Steele, Lea and Flood, 2014),
 so every field is reproducible from its integer seed on any machine.
@@ -312,7 +312,7 @@ def grf(ncell, corr_cells, seed):
 
 
 def two_phase_indicator(ncell, corr_cells, seed, width=1.0):
-    """Phase fraction eta in [0, 1]: the GRF thresholded at 0 with a diffuse interface,
+    """Phase fraction eta in [0, 1]: the GRF thresholded at 0 with an erf interface,
     eta = (1 + erf(g / width)) / 2.  width -> 0 gives the sharp threshold."""
     from scipy.special import erf
     g = grf(ncell, corr_cells, seed)
@@ -321,19 +321,43 @@ def two_phase_indicator(ncell, corr_cells, seed, width=1.0):
     return 0.5 * (1.0 + erf(g / width))
 
 
-def two_phase_problem(n=256, corr_cells=8, contrast=1000.0, seed=256, load=1e-3, width=2.0):
+def interface_stats(g, eta, width):
+    """Shape of a two-phase field: the 10-90 % transition width of eta across the interface in
+    cells (1.8124 width / mean |grad g| over the nodes with |g| < 0.05, per cell), the fraction of
+    nodes within 5 % of a pure phase, and the fraction with 0.25 <= eta <= 0.75."""
+    gx, gy, gz = np.gradient(g)
+    near = np.abs(g) < 0.05
+    grad = float(np.sqrt(gx[near] ** 2 + gy[near] ** 2 + gz[near] ** 2).mean()) if near.any() else float("nan")
+    w1090 = 2.0 * 0.9061938 * width / grad if width > 0 else 0.0
+    return {"interface_cells_10_90": w1090, "pure_phase_frac": float(((eta < 0.05) | (eta > 0.95)).mean()),
+            "mid_frac": float(((eta >= 0.25) & (eta <= 0.75)).mean())}
+
+
+def two_phase_problem(n=256, corr_cells=8, contrast=1000.0, seed=256, load=1e-3, width=0.5):
     """Two-phase isotropic field: lambda = mu = contrast**eta (1 in phase 0, `contrast` in
-    phase 1, geometric blend across the diffuse interface) with an affine dilatational
-    eigenstrain `load`*eta; all-Dirichlet (u = 0) box, so the forcing sits at the
-    interfaces.  The diffuse interface (width 2 standard deviations of the field) keeps
-    the reference's multigrid convergent at 1:1000: with sharper interfaces the restricted
-    coefficient fields of the coarse levels resolve the contrast so poorly that damped point
-    Jacobi there has spectral radius above 1 and the cycle diverges."""
+    phase 1, geometric blend across the interface) with an affine dilatational eigenstrain
+    `load`*eta; all-Dirichlet (u = 0) box, so the forcing sits at the interfaces.
+
+    eta = (1 + erf(g / width)) / 2 for the standardised Gaussian random field g.  The default
+    width 0.5 puts the 10-90 % transition across about 4 cells at correlation length 8 (see
+    interface_stats).  Sharper interfaces (width <= 0.25, two cells and less, and the thresholded
+    width 0) make the strong-form rows next to the interface lose their diagonal: the operator
+    has eigenvalues of D^-1 A in the left half plane and its discrete solution exceeds the
+    eigenstrain's physical displacement scale many times, so no multigrid-preconditioned Krylov
+    method converges on it."""
     ncell = (n, n, n)
-    eta = two_phase_indicator(ncell, corr_cells, seed, width)
+    g = grf(ncell, corr_cells, seed)
+    if width <= 0.0:
+        eta = (g > 0.0).astype(np.float64)
+    else:
+        from scipy.special import erf
+        eta = 0.5 * (1.0 + erf(g / width))
+    stats = interface_stats(g, eta, width)
+    del g
     lam = np.exp(eta * math.log(contrast))
     C = iso_field(lam, lam)
     F0 = np.zeros((eta.size, 9))
     F0[:, 0] = F0[:, 4] = F0[:, 8] = (eta * load).ravel()
     bct, bcv = bc_constant()
-    return dict(ncell=ncell, dx=(1.0 / n,) * 3, C=C, F0=F0.ravel(), eta=eta, bct=bct, bcv=bcv)
+    return dict(ncell=ncell, dx=(1.0 / n,) * 3, C=C, F0=F0.ravel(), eta=eta, bct=bct, bcv=bcv, width=width,
+                interface=stats)

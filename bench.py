@@ -2,12 +2,15 @@
 Fapply HBM bandwidth.
 
 A step is one complete solve A u = b (initial guess 0) of the 256^3-cell / 257^3-node
-two-phase isotropic field (Gaussian random field, correlation length 8 cells, diffuse
-interface, stiffness contrast 1:1000, dilatational eigenstrain in the stiff phase, u = 0 on
-all 26 boundary regions), through the C ABI, with b, C and u resident in HBM.
+two-phase isotropic field (Gaussian random field g, correlation length 8 cells, phase fraction
+eta = (1 + erf(g / width)) / 2 with width 0.5: a 10-90 % interface of about 4 cells; stiffness
+lambda = mu = 1000^eta, dilatational eigenstrain 1e-3 eta, u = 0 on all 26 boundary regions),
+through the C ABI, with b, C and u resident in HBM.
 
-Prints one JSON line (rank 0).  `value` is the time-to-solution of the fastest solver
-method in seconds (lower is better); the plain FP64 multigrid time is reported beside it.
+Prints one JSON line (rank 0).  `value` is the time-to-solution of the fastest converged solver
+method in seconds (lower is better; null when none converged); every method, converged or not,
+is listed in config.methods with its iterations, final residual and relative L2 distance to an
+FP64 reference solve.
 """
 import argparse
 import json
@@ -31,8 +34,14 @@ def parse():
     p.add_argument("--corr", type=float, default=8.0)
     p.add_argument("--contrast", type=float, default=1000.0)
     p.add_argument("--seed", type=int, default=256)
+    p.add_argument("--width", type=float, default=0.5,
+                   help="interface parameter w of eta = (1 + erf(g / w)) / 2 (0: thresholded)")
+    p.add_argument("--krylov-smoother", default="l1", choices=["jacobi", "l1"],
+                   help="smoother of the FGMRES V-cycle (plain multigrid methods: 'jacobi' unless the method "
+                        "name ends in -l1)")
+    p.add_argument("--mg-max-iter", type=int, default=300, help="cycle limit of the plain multigrid methods")
     p.add_argument("--fapply-reps", type=int, default=50)
-    p.add_argument("--restart", type=int, default=8, help="FGMRES restart length")
+    p.add_argument("--restart", type=int, default=16, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
     p.add_argument("--krylov-sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
@@ -41,9 +50,10 @@ def parse():
     p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES V-cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
-    p.add_argument("--methods", default="fp64v,mixedv,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
+    p.add_argument("--methods", default="fp64v,fp64v-l1,mixedv-l1,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
-                        "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps); "
+                        "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps), the "
+                        "reference's damped point Jacobi, or the l1 smoother with the suffix -l1; "
                         "fgmres-*: flexible GMRES(--restart) preconditioned by one V-cycle with --krylov-smooth "
                         "Fsmooth calls of --krylov-sweeps Jacobi sweeps each side; "
                         "fgmres-learned-*: the same with the learned correction stage after each cycle")
@@ -66,7 +76,7 @@ def fapply_only(a):
     dev = torch.device("cuda:0")
     n = a.n
     N = (n + 1) ** 3
-    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed)
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed, width=a.width)
     op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]))
     op.SetModel(pr["C"])
     op.prepareForSolve()
@@ -107,13 +117,23 @@ def main():
     dev = torch.device("cuda:0")
     n = a.n
     t0 = time.time()
-    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed)
-    print("problem generated in %.1fs" % (time.time() - t0), file=sys.stderr, flush=True)
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed, width=a.width)
+    print("problem generated in %.1fs: %s" % (time.time() - t0, json.dumps(pr["interface"])), file=sys.stderr,
+          flush=True)
     N = (n + 1) ** 3
     bc = (pr["bct"], pr["bcv"])
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
 
+    def make_op(prec, vcyc, krylov, smoother):
+        nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
+        return Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
+                       pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
+                       sweeps_finest=a.krylov_sweeps_finest if krylov else 0, smoother=smoother,
+                       max_iter=a.mg_max_iter, verbose=a.verbose,
+                       bottom_solver=a.krylov_bottom if krylov else "bicgstab")
+
     results = {}
+    sols = {}
     rhs = None
     fapply = None
     net, train_info = None, None
@@ -123,16 +143,15 @@ def main():
         learned = base.startswith("learned-")
         if learned:
             base = base[len("learned-"):]
-            if net is None:
-                net, train_info = train_network(a, P, Elastic, Net, PREC_MIXED if base.startswith("mixed") else PREC_FP64,
-                                                CYCLE_V if base.endswith("v") else CYCLE_F)
+        l1 = base.endswith("-l1")
+        if l1:
+            base = base[:-3]
+        smoother = a.krylov_smoother if krylov else ("l1" if l1 else "jacobi")
         prec = PREC_MIXED if base.startswith("mixed") else PREC_FP64
         vcyc = base.endswith("v")
-        nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
-        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
-                     pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
-                     sweeps_finest=a.krylov_sweeps_finest if krylov else 0,
-                     verbose=a.verbose, bottom_solver=a.krylov_bottom if krylov else "bicgstab")
+        if learned and net is None:
+            net, train_info = train_network(a, P, Elastic, Net, prec, CYCLE_V if vcyc else CYCLE_F, smoother)
+        op = make_op(prec, vcyc, krylov, smoother)
         op.SetModel(pr["C"])
         op.prepareForSolve()
         if rhs is None:
@@ -144,75 +163,74 @@ def main():
         sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
         if learned:
             net.attach(op)
-        lin = (Krylov(op, restart=a.restart, precond=PRECOND_LEARNED if learned else PRECOND_MG, verbose=a.verbose)
-               if krylov else Linear(op))
+        lin = (Krylov(op, restart=a.restart, precond=PRECOND_LEARNED if learned else PRECOND_MG,
+                      max_iter=a.mg_max_iter, verbose=a.verbose) if krylov else Linear(op))
         ok = True
-        for _ in range(a.warmup):
+        st = None
+        for _ in range(max(1, a.warmup)):
             sol.zero_()
             op.prepareForSolve()
-            ok = lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False)["converged"] == 1
-        torch.cuda.synchronize()
-        stats = []
-        ts = time.perf_counter()
-        for _ in range(a.steps if ok else 1):
-            # a step: coefficient restriction to every level, diagonals, then the solve from u0 = 0
-            sol.zero_()
-            op.prepareForSolve()
-            stats.append(lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False))
-        torch.cuda.synchronize()
-        te = time.perf_counter()
-        results[meth] = dict(seconds=(te - ts) / len(stats), iterations=stats[-1]["iterations"],
-                             final_rel_res=stats[-1]["final_res_inf"] / max(stats[-1]["rhs_inf"], 1e-300),
-                             converged=all(s["converged"] for s in stats))
+            torch.cuda.synchronize()
+            tw = time.perf_counter()
+            st = lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False)
+            torch.cuda.synchronize()
+            tw = time.perf_counter() - tw
+            ok = st["converged"] == 1
+            if not ok:
+                break
+        rec = dict(precision="f64" if prec == PREC_FP64 else "f32-in-f64", smoother=smoother)
+        if ok:
+            stats = []
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            for _ in range(a.steps):
+                # a step: coefficient restriction to every level, diagonals, then the solve from u0 = 0
+                sol.zero_()
+                op.prepareForSolve()
+                stats.append(lin.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False))
+            torch.cuda.synchronize()
+            te = time.perf_counter()
+            st = stats[-1]
+            rec.update(seconds=(te - ts) / len(stats), iterations=st["iterations"],
+                       final_rel_res=st["final_res_inf"] / max(st["rhs_inf"], 1e-300),
+                       converged=all(x["converged"] for x in stats))
+        else:
+            # not converged (or diverged): one attempt, not timed as a step
+            rec.update(seconds=None, attempt_seconds=tw, iterations=st["iterations"],
+                       final_rel_res=st["final_res_inf"] / max(st["rhs_inf"], 1e-300), converged=False,
+                       rc=st["rc"])
+        results[meth] = rec
+        if rec["converged"]:
+            sols[meth] = sol.detach().clone()
         if learned:
-            results[meth]["fallback_events"] = stats[-1]["fallback_events"]
-            results[meth]["layers"] = layer_report(net.layer_times(op, reps=5), n)
-        print("%s: %.4f s/solve, %d iterations" % (meth, results[meth]["seconds"], results[meth]["iterations"]),
+            rec["fallback_events"] = st["fallback_events"]
+            rec["layers"] = layer_report(net.layer_times(op, reps=5), n)
+        print("%s: %s s/solve, %d iterations, final %.2e%s" % (meth, "%.4f" % rec["seconds"] if rec["seconds"] else "-",
+                                                               rec["iterations"], rec["final_rel_res"],
+                                                               "" if rec["converged"] else " (not converged)"),
               file=sys.stderr, flush=True)
-        if meth == "fp64" or fapply is None:
-            # Fapply (level 0) bandwidth with HIP events on the launch stream (the default stream)
-            u = torch.rand(3 * N, dtype=torch.float64, device=dev)
-            f = torch.empty_like(u)
-            op.Fapply(0, 0, f, u)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(a.fapply_reps):
-                op.Fapply(0, 0, f, u)
-            e1.record()
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / a.fapply_reps
-            bytes_per_launch = 64.0 * N  # (3 u + 3 f + 2 C) x 8 B per node, isotropic FP64
-            fapply = dict(ms=ms, gbs=bytes_per_launch / (ms * 1e-3) / 1e9, bytes=bytes_per_launch)
-            # Fsmooth (level 0, FP64): fused damped-Jacobi sweeps, the diagonal recomputed from
-            # lambda, mu, so (3 x + 3 b + 2 C + 3 x') x 8 B per node and sweep
-            b = torch.rand_like(u)
-            op.Fsmooth(0, 0, f, b)
-            torch.cuda.synchronize()
-            reps = max(1, a.fapply_reps // 5)
-            e0.record()
-            for _ in range(reps):
-                op.Fsmooth(0, 0, f, b)
-            e1.record()
-            torch.cuda.synchronize()
-            sw = op.desc.sweeps_per_smooth
-            ms_sweep = e0.elapsed_time(e1) / reps / sw
-            fapply["fsmooth_sweep_ms"] = ms_sweep
-            fapply["fsmooth_sweep_gbs"] = 88.0 * N / (ms_sweep * 1e-3) / 1e9
-            del u, f, b
+        if fapply is None:
+            fapply = measure_fapply(a, op, N, dev)
         del op, lin, sol
         torch.cuda.empty_cache()
 
+    # reference solution: FP64 FGMRES to 1e-13 (untimed); relative L2 distance of every method's solution
+    ref = reference_solve(a, make_op, pr["C"], rhs, N, dev)
+    for k, x in sols.items():
+        results[k]["rel_l2_vs_fp64_ref"] = float(torch.linalg.norm(x - ref) / torch.linalg.norm(ref))
+
     conv = [k for k in results if results[k]["converged"]]
-    best = min(conv or results, key=lambda k: results[k]["seconds"])
-    val = results[best]["seconds"]
+    best = min(conv, key=lambda k: results[k]["seconds"]) if conv else None
+    val = results[best]["seconds"] if best else None
+
+    # the value method's solution, checked by the oracle's FP64 operator on the host
+    oracle_res = None
+    if best is not None and not a.skip_cpu:
+        oracle_res = oracle_residual(pr, sols[best].cpu().numpy(), rhs.cpu().numpy())
 
     cpu = None
-    if not a.skip_cpu:
-        ref_meth = next((k for k in ("fp64", "fp64v") if k in results and results[k]["converged"]), None)
-        if ref_meth is not None:
-            cpu = cpu_baseline(pr, rhs.cpu().numpy(), results[ref_meth]["iterations"],
-                               cycle=0 if ref_meth.endswith("v") else 1)
+    if not a.skip_cpu and best is not None:
+        cpu = cpu_baseline(pr, rhs.cpu().numpy(), a, results)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "fapply_iso_f64_257_traffic.json")
@@ -231,23 +249,28 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f32-in-f64",
+        "dtype": results[best]["precision"] if best else None,
         "data": "synthetic",
         "config": {
-            "workload": "two-phase GRF (corr %g cells, contrast 1:%g, seed %d) isotropic field, %d^3 cells / %d^3 "
-                        "nodes, Dirichlet box, eigenstrain load; solve to |b-Au|inf/|b|inf <= 1e-10"
-                        % (a.corr, a.contrast, a.seed, n, n + 1),
+            "workload": "two-phase GRF (corr %g cells, contrast 1:%g, seed %d) isotropic field, eta = (1 + "
+                        "erf(g/%g))/2: 10-90%% interface %.1f cells, %.0f%% of nodes within 5%% of a pure phase; %d^3 "
+                        "cells / %d^3 nodes, Dirichlet box, eigenstrain load; solve to |b-Au|inf/|b|inf <= 1e-10"
+                        % (a.corr, a.contrast, a.seed, a.width, pr["interface"]["interface_cells_10_90"],
+                           100 * pr["interface"]["pure_phase_frac"], n, n + 1),
+            "interface": pr["interface"],
             "method": best,
             "fgmres": {"restart": a.restart, "vcycle_smooth_calls": a.krylov_smooth,
-                       "jacobi_sweeps_per_call": a.krylov_sweeps,
-                       "jacobi_sweeps_per_call_finest": a.krylov_sweeps_finest or a.krylov_sweeps,
-                       "bottom_solver": a.krylov_bottom},
+                       "sweeps_per_call": a.krylov_sweeps,
+                       "sweeps_per_call_finest": a.krylov_sweeps_finest or a.krylov_sweeps,
+                       "smoother": a.krylov_smoother, "bottom_solver": a.krylov_bottom},
+            "oracle_true_rel_res": oracle_res,
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
             "fsmooth_sweep_ms": fapply["fsmooth_sweep_ms"] if fapply else None,
             "fsmooth_sweep_gbs": fapply["fsmooth_sweep_gbs"] if fapply else None,
             "fapply_ms": fapply["ms"] if fapply else None,
+            "fapply_timing": "HIP events around %d launches on the launch stream" % a.fapply_reps,
         },
         "roofline": {
             "bound": "hbm",
@@ -260,6 +283,66 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def measure_fapply(a, op, N, dev):
+    """Fapply (level 0, FP64) and one FP64 Jacobi sweep: mean durations from HIP events on the
+    launch stream (the default stream, where the library launches)"""
+    import torch
+    u = torch.rand(3 * N, dtype=torch.float64, device=dev)
+    f = torch.empty_like(u)
+    op.Fapply(0, 0, f, u)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.fapply_reps):
+        op.Fapply(0, 0, f, u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.fapply_reps
+    bytes_per_launch = 64.0 * N  # (3 u + 3 f + 2 C) x 8 B per node, isotropic FP64
+    out = dict(ms=ms, gbs=bytes_per_launch / (ms * 1e-3) / 1e9, bytes=bytes_per_launch)
+    # Fsmooth (level 0, FP64): fused sweeps, (3 x + 3 b + 2 C + 3 x') x 8 B per node and sweep (the
+    # point diagonal is recomputed from lambda, mu; the l1 diagonal from them and their differences)
+    b = torch.rand_like(u)
+    op.Fsmooth(0, 0, f, b)
+    torch.cuda.synchronize()
+    reps = max(1, a.fapply_reps // 5)
+    e0.record()
+    for _ in range(reps):
+        op.Fsmooth(0, 0, f, b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_sweep = e0.elapsed_time(e1) / reps / op.desc.sweeps_per_smooth
+    out["fsmooth_sweep_ms"] = ms_sweep
+    out["fsmooth_sweep_gbs"] = 88.0 * N / (ms_sweep * 1e-3) / 1e9
+    del u, f, b
+    return out
+
+
+def reference_solve(a, make_op, C, rhs, N, dev):
+    """FP64 FGMRES with the l1 V-cycle to |b - A u|_inf <= 1e-13 |b|_inf (not timed)"""
+    import torch
+    from alamo_amd.operator import Krylov, PREC_FP64, PRECOND_MG
+    op = make_op(PREC_FP64, True, True, "l1")
+    op.SetModel(C)
+    sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+    st = Krylov(op, restart=a.restart, precond=PRECOND_MG, max_iter=2000).solve(sol, rhs, 1e-13, 0.0,
+                                                                                 raise_on_fail=False)
+    print("reference solve: %d iterations, final %.2e" % (st["iterations"], st["final_res_inf"] / st["rhs_inf"]),
+          file=sys.stderr, flush=True)
+    del op
+    return sol
+
+
+def oracle_residual(pr, x, b):
+    """|b - A x|_inf / |b|_inf with the oracle's FP64 operator (checker only)"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle_ctypes as ora
+    from alamo_amd import problems as P
+    r = ora.residual(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], x, b)
+    return float(np.abs(r).max() / np.abs(b).max())
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of MI355X
@@ -281,20 +364,20 @@ def layer_report(ms, n):
     return out
 
 
-def train_network(a, P, Elastic, Net, precision, cycle):
+def train_network(a, P, Elastic, Net, precision, cycle, smoother):
     """Train the correction network on random two-phase fields of train_n^3 cells (seeds other
-    than the benchmark field's), outside the timed region."""
+    than the benchmark field's, the same interface width), outside the timed region."""
     t0 = time.time()
     net = Net(seed=12345)
     stats = []
     for f in range(a.train_fields):
-        pr = P.two_phase_problem(n=a.train_n, corr_cells=a.corr * a.train_n / a.n, contrast=a.contrast,
-                                 seed=a.seed + 1 + f)
+        pr = P.two_phase_problem(n=a.train_n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed + 1 + f,
+                                 width=a.width)
         vc = cycle == 0
         nsm = a.krylov_smooth if vc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
                      pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps,
-                     sweeps_finest=a.krylov_sweeps_finest)
+                     sweeps_finest=a.krylov_sweeps_finest, smoother=smoother, bottom_solver=a.krylov_bottom)
         op.SetModel(pr["C"])
         stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
                                solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))

@@ -40,6 +40,7 @@ enum { ALAMO_BOTTOM_BICGSTAB = 0, /* Jacobi-preconditioned BiCGStab on the coars
        ALAMO_BOTTOM_DENSE = 2     /* direct: the coarsest operator assembled and inverted at prepare
                                      (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle */ };
 #define ALAMO_DENSE_BOTTOM_MAX 1200
+enum { ALAMO_SMOOTHER_JACOBI = 0, ALAMO_SMOOTHER_L1 = 1 };
 enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, ALAMO_ERR_NOCONV = 4,
        ALAMO_ERR_NAN = 5 };
 
@@ -65,14 +66,12 @@ typedef struct {
     int bottom_max_iter;       /* solver.bottom_max_iter */
     double bottom_tol_rel;     /* solver.bottom_tol_rel */
     int verbose;
-    int row_align;             /* 0 or 1: compact node layout on every level.  A >= 2: on each level
-                                  whose x rows hold >= 4 A nodes, the row pitch is rounded up to a
-                                  multiple of A nodes (32: every row starts on a 256-byte boundary),
-                                  which lets the row kernels load whole cache lines; every field
-                                  passed for the handle on such a level then uses that layout
-                                  (alamo_mg_level_layout), with zeros in the pad entries.  Host
-                                  arrays (the *_host entry points) are always compact.  No
-                                  reference counterpart (an amrex::FArrayBox is compact). */
+    int smoother;              /* ALAMO_SMOOTHER_*: the sweeps of Fsmooth and of the cycles.  JACOBI:
+                                  the reference's damped point Jacobi (Operator.cpp:109-146).  L1:
+                                  x += (b - A x) / d with d_i = sign(a_ii) sum_j |a_ij| (l1-Jacobi, no
+                                  damping), a smoother that stays stable on rows whose off-diagonal
+                                  sum exceeds the diagonal, as the strong-form rows next to a sharp
+                                  stiffness jump do; no reference counterpart, not with psi. */
     int fixed_iter;            /* solver.fixed_iter (Linear.H:181,267): > 0 runs exactly that many
                                   multigrid iterations and returns without a convergence failure
                                   (amrex::MLMG::setFixedIter); 0 off */
@@ -113,7 +112,8 @@ int alamo_mg_set_stream(alamo_mg* h, void* stream);
 int alamo_mg_num_levels(alamo_mg* h, int* nlev);
 int alamo_mg_level_ncell(alamo_mg* h, int mglev, int* ncell3);
 /* Node layout of level mglev: node (i,j,k) of component c is element i + pitch[0] j + pitch[1] k
- * + pitch[2] c; pitch[2] elements per component (the size of one component's array). */
+ * + pitch[2] c; pitch[2] elements per component (the size of one component's array).  Levels are
+ * compact: pitch = (N0, N0 N1, N0 N1 N2) with N = n_cell + 1. */
 int alamo_mg_level_layout(alamo_mg* h, int mglev, int64_t* pitch3);
 
 /* Elastic::SetModel(amrlev, MultiTab) (Elastic.cpp:87-116): copy the per-node stiffness of
@@ -165,13 +165,13 @@ int alamo_blend_models(int64_t nnodes, int nmodels, int nrec, const double* eta,
 /* Linear::setMaxIter / setBottomMaxIter / setFixedIter / setVerbose / setPreSmooth /
  * setPostSmooth and Operator::SetOmega (Linear.H:178-184, Operator.H:57) on a built handle: copies
  * cycle, pre_smooth, post_smooth, sweeps_per_smooth, omega, max_iter, bottom_max_iter,
- * bottom_tol_rel, fixed_iter and verbose from d; the fields that shape the hierarchy (n_cell, dx,
- * sym, bc, precision, max_coarsening_level, row_align) must be unchanged. */
+ * bottom_tol_rel, fixed_iter, verbose, bottom_solver, max_fmg_iter, sweeps_finest and smoother
+ * from d; the fields that shape the hierarchy (n_cell, dx, sym, bc, precision,
+ * max_coarsening_level) must be unchanged. */
 int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d);
 
 /* Elastic::SetPsi(amrlev, psi, psi_small) (src/Operator/Elastic.H:55-57, Elastic.cpp:118-137): the
- * cell field psi (n_cell[0] n_cell[1] n_cell[2] doubles, i fastest; compact also on pitched
- * handles).  From then on every row carries the psi terms of Elastic.cpp:191-195,243-260 and the
+ * cell field psi (n_cell[0] n_cell[1] n_cell[2] doubles, i fastest).  From then on every row carries the psi terms of Elastic.cpp:191-195,243-260 and the
  * diagonal its psi_avg (:340-363); prepare restricts psi to the coarse levels (:839-907) and
  * alamo_newton_rhs uses the psi variant of Newton.H:234-266.  psi_small in [0, 1) (the
  * reference's default is 1e-8).  Device pointer; _host copies from host memory. */

@@ -188,6 +188,14 @@ public:
         opt().bottom_solver = b;
         push();
     }
+    // smoother of the cycles: "jacobi" (the reference's damped point Jacobi) or "l1" (l1-Jacobi;
+    // no reference counterpart)
+    void setSmoother(const std::string& name) {
+        int v = name == "jacobi" ? ALAMO_SMOOTHER_JACOBI : name == "l1" ? ALAMO_SMOOTHER_L1 : -1;
+        if (v < 0) throw alamo::Abort(ALAMO_ERR_ARG, "unknown smoother " + name);
+        opt().smoother = v;
+        push();
+    }
     // Linear::apply (Linear.H:169-174): rhs = A sol on the finest level
     void apply(double* rhs, const double* sol) { defined().Apply(0, 0, rhs, sol); }
     // Linear::solveaffine (Linear.H:53-130) subtracts A 0 under inhomogeneous boundaries first;

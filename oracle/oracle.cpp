@@ -410,6 +410,113 @@ static void jacobi(const Level& L, int sym, const double* C, const int* bct, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// l1-Jacobi smoother (no reference counterpart; a preconditioner choice that leaves the operator
+// unchanged): d_(n,q) = sign(a_nq,nq) sum_j |a_nq,j| over the row of node n, component q.  The
+// rows are read by probing this file's fapply: the stencil reaches one node in each direction,
+// so the probes of the 27 node classes (i mod 3, j mod 3, k mod 3) x 3 components each hit one
+// column of every row.  x += (b - A x) / d per sweep (a zero row keeps x).
+// Pinned by tests/test_oracle_l1.py (dense brute force of the row sums).
+// ---------------------------------------------------------------------------
+static void l1_diag(const Level& L, int sym, const double* C, const int* bct, double* d) {
+    const int64_t n = 3 * L.tot;
+    std::vector<double> e(n), y(n), D(n);
+    diagonal(L, sym, C, bct, D.data());
+    std::fill(d, d + n, 0.0);
+    for (int c = 0; c < 27; c++)
+        for (int p = 0; p < 3; p++) {
+            std::fill(e.begin(), e.end(), 0.0);
+            for (int k = c / 9; k < L.N[2]; k += 3)
+                for (int j = (c / 3) % 3; j < L.N[1]; j += 3)
+                    for (int i = c % 3; i < L.N[0]; i += 3) e[p * L.tot + L.id(i, j, k)] = 1.0;
+            fapply(L, sym, C, bct, e.data(), y.data());
+            for (int64_t q = 0; q < n; q++) d[q] += std::fabs(y[q]);
+        }
+    for (int64_t q = 0; q < n; q++)
+        if (D[q] < 0.0) d[q] = -d[q];
+}
+
+static void jacobi_l1(const Level& L, int sym, const double* C, const int* bct, const double* d, int sweeps,
+                      double* x, const double* b) {
+    std::vector<double> Ax(3 * L.tot);
+    for (int s = 0; s < sweeps; s++) {
+        fapply(L, sym, C, bct, x, Ax.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < 3 * L.tot; q++)
+            if (d[q] != 0.0) x[q] += (b[q] - Ax[q]) / d[q];
+    }
+}
+
+// The same l1 diagonal assembled row by row from the stencils of fapply_node (interior: second
+// differences, four-corner mixed differences and central differences of u and C; boundary: the BC
+// row with one-sided first derivatives), at the cost of one pass; the CPU baseline's smoother.
+// Pinned against l1_diag (probing) by tests/test_oracle_l1.py.
+static void l1_diag_rows(const Level& L, int sym, const double* C, const int* bct, double* d) {
+    const int nc = ncoef(sym);
+    int lo[3] = {0, 0, 0}, hi[3] = {L.n[0], L.n[1], L.n[2]};
+    const int sl[3] = {1, 3, 9};
+    const int64_t st[3] = {1, L.N[0], (int64_t)L.N[0] * L.N[1]};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < L.N[2]; k++)
+        for (int j = 0; j < L.N[1]; j++)
+            for (int i = 0; i < L.N[0]; i++) {
+                const int64_t m = L.id(i, j, k);
+                const double* c = C + m * nc;
+                const int idx[3] = {i, j, k};
+                for (int q = 0; q < 3; q++) {
+                    double acc[27][3] = {{0}};
+                    if (L.bnd(i, j, k)) {
+                        const int face = classify(i, j, k, lo, hi);
+                        double nrm[3];
+                        normal_of(face, nrm);
+                        auto d1w = [&](int l, double coef, int comp) {
+                            if (idx[l] == 0) { acc[13 + sl[l]][comp] += coef / L.dx[l]; acc[13][comp] -= coef / L.dx[l]; }
+                            else if (idx[l] == L.n[l]) { acc[13][comp] += coef / L.dx[l]; acc[13 - sl[l]][comp] -= coef / L.dx[l]; }
+                            else { acc[13 + sl[l]][comp] += 0.5 * coef / L.dx[l]; acc[13 - sl[l]][comp] -= 0.5 * coef / L.dx[l]; }
+                        };
+                        const int t = bct[3 * face + q];
+                        if (t == 0) acc[13][q] = 1.0;  // Displacement
+                        else if (t == 1) {             // Traction: (C : grad u) n
+                            for (int kk = 0; kk < 3; kk++)
+                                for (int l = 0; l < 3; l++) {
+                                    double coef = 0.0;
+                                    for (int e = 0; e < 3; e++) coef += nrm[e] * C4(sym, c, q, e, kk, l);
+                                    d1w(l, coef, kk);
+                                }
+                        } else if (t == 3) {  // Neumann: grad u n
+                            for (int l = 0; l < 3; l++) d1w(l, nrm[l], q);
+                        }
+                    } else {
+                        for (int kk = 0; kk < 3; kk++)
+                            for (int jj = 0; jj < 3; jj++)
+                                for (int ll = 0; ll < 3; ll++) {
+                                    const double cf = C4(sym, c, q, jj, kk, ll);
+                                    if (jj == ll) {
+                                        const double w = cf / (L.dx[jj] * L.dx[jj]);
+                                        acc[13 + sl[jj]][kk] += w;
+                                        acc[13 - sl[jj]][kk] += w;
+                                        acc[13][kk] -= 2.0 * w;
+                                    } else {
+                                        const double w = 0.25 * cf / (L.dx[jj] * L.dx[ll]);
+                                        acc[13 + sl[jj] + sl[ll]][kk] += w;
+                                        acc[13 - sl[jj] - sl[ll]][kk] += w;
+                                        acc[13 + sl[jj] - sl[ll]][kk] -= w;
+                                        acc[13 - sl[jj] + sl[ll]][kk] -= w;
+                                    }
+                                    const double dc = (C4(sym, C + (m + st[jj]) * nc, q, jj, kk, ll) -
+                                                       C4(sym, C + (m - st[jj]) * nc, q, jj, kk, ll)) * 0.5 / L.dx[jj];
+                                    acc[13 + sl[ll]][kk] += dc * 0.5 / L.dx[ll];
+                                    acc[13 - sl[ll]][kk] -= dc * 0.5 / L.dx[ll];
+                                }
+                    }
+                    double sum = 0.0;
+                    for (int o = 0; o < 27; o++)
+                        for (int kk = 0; kk < 3; kk++) sum += std::fabs(acc[o][kk]);
+                    d[q * L.tot + m] = acc[13][q] < 0.0 ? -sum : sum;
+                }
+            }
+}
+
 // a11: solutionResidual / correctionResidual  r = b - A x  (Operator.cpp:635-657)
 static void residual(const Level& L, int sym, const double* C, const int* bct, const double* x, const double* b,
                      double* r) {
@@ -540,7 +647,7 @@ struct Hier {
     int sym;
     const int* bct;
     std::vector<Level> L;
-    std::vector<std::vector<double>> C, D, psi;
+    std::vector<std::vector<double>> C, D, psi, Dl1;
 };
 
 // Restriction of psi to the next multigrid level (Operator/Elastic.cpp:839-907): coarse cell
@@ -635,6 +742,8 @@ struct MGOpts {
     int max_iter;
     int bottom_max_iter;
     double bottom_tol;
+    int smoother = 0;       // 0: damped point Jacobi (the reference), 1: l1-Jacobi (Hier::Dl1)
+    int sweeps_finest = 0;  // > 0: sweeps per smoothing call on level 0
 };
 
 static void apply_l(Hier& H, int l, const double* x, double* y) {
@@ -688,8 +797,11 @@ struct MGWork {
 };
 
 static void smooth(Hier& H, const MGOpts& o, int l, double* x, const double* b, int ncalls) {
-    for (int c = 0; c < ncalls; c++)
-        jacobi(H.L[l], H.sym, H.C[l].data(), H.bct, H.D[l].data(), o.omega, o.sweeps, x, b);
+    const int sw = (l == 0 && o.sweeps_finest > 0) ? o.sweeps_finest : o.sweeps;
+    for (int c = 0; c < ncalls; c++) {
+        if (o.smoother == 1) jacobi_l1(H.L[l], H.sym, H.C[l].data(), H.bct, H.Dl1[l].data(), sw, x, b);
+        else jacobi(H.L[l], H.sym, H.C[l].data(), H.bct, H.D[l].data(), o.omega, sw, x, b);
+    }
 }
 
 static void vcycle(Hier& H, const MGOpts& o, MGWork& W, int top) {
@@ -753,6 +865,87 @@ static int mg_solve(Hier& H, const MGOpts& o, double* x, const double* b, double
     }
     if (final_res) *final_res = rn;
     return rn <= target ? it : -it;
+}
+
+// ---------------------------------------------------------------------------
+// Flexible GMRES(m) right-preconditioned by one V-cycle from zero (no reference counterpart: the
+// CPU baseline of the library's Krylov solver).  Modified Gram-Schmidt in FP64; a restart cycle
+// ends early when its least-squares residual estimate, scaled by the max/2-norm ratio of the
+// restart's true residual, reaches the target; convergence is decided by the true residual's max
+// norm at the restarts, as in the library.
+// ---------------------------------------------------------------------------
+static int fgmres(Hier& H, const MGOpts& o, int m, double* x, const double* b, double tol_rel, double tol_abs,
+                  double* final_res) {
+    const Level& L = H.L[0];
+    const int64_t n = 3 * L.tot;
+    MGWork W;
+    W.cor.resize(H.L.size()); W.res.resize(H.L.size()); W.rescor.resize(H.L.size());
+    for (size_t l = 0; l < H.L.size(); l++) {
+        W.cor[l].assign(3 * H.L[l].tot, 0.0);
+        W.res[l].assign(3 * H.L[l].tot, 0.0);
+        W.rescor[l].assign(3 * H.L[l].tot, 0.0);
+    }
+    std::vector<std::vector<double>> V(m + 1, std::vector<double>(n)), Z(m, std::vector<double>(n));
+    std::vector<double> r(n);
+    residual(L, H.sym, H.C[0].data(), H.bct, x, b, r.data());
+    const double r0 = norm_inf(r.data(), n), bn = norm_inf(b, n);
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
+    double rinf = r0, beta = std::sqrt(dot(r.data(), r.data(), n));
+    std::vector<std::vector<double>> Hm(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m), sn(m), gv(m + 1), y(m);
+    int it = 0;
+    while (rinf > target && it < o.max_iter && std::isfinite(rinf) && beta > 0.0) {
+        const double ratio = rinf / beta;
+        for (int64_t q = 0; q < n; q++) V[0][q] = r[q] / beta;
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m; j++) {
+            W.res[0] = V[j];
+            vcycle(H, o, W, 0);
+            Z[j] = W.cor[0];
+            apply_l(H, 0, Z[j].data(), V[j + 1].data());
+            for (int i = 0; i <= j; i++) {
+                const double hij = dot(V[j + 1].data(), V[i].data(), n);
+                Hm[i][j] = hij;
+#pragma omp parallel for schedule(static)
+                for (int64_t q = 0; q < n; q++) V[j + 1][q] -= hij * V[i][q];
+            }
+            const double hn = std::sqrt(dot(V[j + 1].data(), V[j + 1].data(), n));
+            Hm[j + 1][j] = hn;
+            if (hn > 0.0)
+#pragma omp parallel for schedule(static)
+                for (int64_t q = 0; q < n; q++) V[j + 1][q] /= hn;
+            for (int i = 0; i < j; i++) {
+                const double a = cs[i] * Hm[i][j] + sn[i] * Hm[i + 1][j];
+                Hm[i + 1][j] = -sn[i] * Hm[i][j] + cs[i] * Hm[i + 1][j];
+                Hm[i][j] = a;
+            }
+            const double den = std::hypot(Hm[j][j], Hm[j + 1][j]);
+            cs[j] = den > 0 ? Hm[j][j] / den : 1.0;
+            sn[j] = den > 0 ? Hm[j + 1][j] / den : 0.0;
+            Hm[j][j] = den;
+            Hm[j + 1][j] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            it++;
+            jend = j + 1;
+            if (std::fabs(gv[j + 1]) * ratio <= target || it >= o.max_iter || hn == 0.0 || !std::isfinite(hn)) break;
+        }
+        for (int i = jend - 1; i >= 0; i--) {
+            double sum = gv[i];
+            for (int k = i + 1; k < jend; k++) sum -= Hm[i][k] * y[k];
+            y[i] = Hm[i][i] != 0.0 ? sum / Hm[i][i] : 0.0;
+        }
+        for (int i = 0; i < jend; i++)
+#pragma omp parallel for schedule(static)
+            for (int64_t q = 0; q < n; q++) x[q] += y[i] * Z[i][q];
+        residual(L, H.sym, H.C[0].data(), H.bct, x, b, r.data());
+        rinf = norm_inf(r.data(), n);
+        beta = std::sqrt(dot(r.data(), r.data(), n));
+    }
+    if (final_res) *final_res = rinf;
+    return rinf <= target ? it : -it;
 }
 
 // ---------------------------------------------------------------------------
@@ -955,6 +1148,15 @@ void ora_jacobi(const int* ncell, const double* dx, int sym, const double* C, co
     Level L(ncell, dx);
     jacobi(L, sym, C, bct, D, omega, sweeps, x, b);
 }
+void ora_l1_diag(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* d) {
+    Level L(ncell, dx);
+    l1_diag(L, sym, C, bct, d);
+}
+void ora_jacobi_l1(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* d,
+                   int sweeps, double* x, const double* b) {
+    Level L(ncell, dx);
+    jacobi_l1(L, sym, C, bct, d, sweeps, x, b);
+}
 void ora_residual(const int* ncell, const double* dx, int sym, const double* C, const int* bct, const double* x,
                   const double* b, double* r) {
     Level L(ncell, dx);
@@ -985,6 +1187,30 @@ int ora_mg_solve(const int* ncell, const double* dx, int sym, const double* C, c
     if (nlevels) *nlevels = (int)H.L.size();
     MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, 1e-4};
     return mg_solve(H, o, x, b, tol_rel, tol_abs, hist, hist_cap, final_res);
+}
+// FGMRES(restart) with one V-cycle (nu1 / nu2 smoothing calls of `sweeps` sweeps, `sweeps_finest` on
+// level 0 when > 0; smoother 0 damped point Jacobi, 1 l1-Jacobi; BiCGStab bottom to bottom_tol) as the
+// right preconditioner.  Returns the iterations, negative when max_iter ended the solve.
+int ora_krylov_solve(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* x,
+                     const double* b, double tol_rel, double tol_abs, int restart, int max_iter, int nu1, int nu2,
+                     int sweeps, int sweeps_finest, int smoother, double omega, double bottom_tol, double* final_res) {
+    Hier H;
+    build_hier(H, ncell, dx, sym, bct, C, -1);
+    if (smoother == 1) {
+        H.Dl1.resize(H.L.size());
+        for (size_t l = 0; l < H.L.size(); l++) {
+            H.Dl1[l].resize(3 * H.L[l].tot);
+            l1_diag_rows(H.L[l], sym, H.C[l].data(), bct, H.Dl1[l].data());
+        }
+    }
+    MGOpts o{0, nu1, nu2, sweeps, omega, max_iter, 200, bottom_tol};
+    o.smoother = smoother;
+    o.sweeps_finest = sweeps_finest;
+    return fgmres(H, o, restart, x, b, tol_rel, tol_abs, final_res);
+}
+void ora_l1_diag_rows(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* d) {
+    Level L(ncell, dx);
+    l1_diag_rows(L, sym, C, bct, d);
 }
 // Multigrid hierarchy stiffness at level l (for parity of the coefficient restriction).
 void ora_coeff_level(const int* ncell, const double* dx, int sym, const double* C, int level, double* out) {

@@ -84,6 +84,43 @@ def jacobi(ncell, dx, sym, C, bct, Dg, omega, sweeps, x, b):
     return x
 
 
+def l1_diag(ncell, dx, sym, C, bct):
+    """l1-Jacobi smoother diagonal: sign(a_ii) sum_j |a_ij| per row"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    N = np.prod(nodes(ncell))
+    d = np.zeros(3 * N)
+    lib().ora_l1_diag(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(d))
+    return d
+
+
+def jacobi_l1(ncell, dx, sym, C, bct, d, sweeps, x, b):
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    d = _c(d); x = _c(x).copy(); b = _c(b)
+    lib().ora_jacobi_l1(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(d), sweeps, _d(x), _d(b))
+    return x
+
+
+def l1_diag_rows(ncell, dx, sym, C, bct):
+    """the l1 smoother diagonal assembled row by row from the stencils (the CPU baseline's)"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    N = np.prod(nodes(ncell))
+    d = np.zeros(3 * N)
+    lib().ora_l1_diag_rows(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(d))
+    return d
+
+
+def krylov_solve(ncell, dx, sym, C, bct, b, x0=None, tol_rel=1e-10, tol_abs=0.0, restart=16, max_iter=400, nu1=1,
+                 nu2=1, sweeps=3, sweeps_finest=1, smoother=1, omega=2.0 / 3.0, bottom_tol=1e-10):
+    """FGMRES(restart) with one V-cycle as right preconditioner; returns (x, iterations, final |r|_inf)"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); b = _c(b)
+    x = np.zeros_like(b) if x0 is None else _c(x0).copy()
+    fr = ctypes.c_double(0)
+    it = lib().ora_krylov_solve(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(x), _d(b), ctypes.c_double(tol_rel),
+                                ctypes.c_double(tol_abs), restart, max_iter, nu1, nu2, sweeps, sweeps_finest, smoother,
+                                ctypes.c_double(omega), ctypes.c_double(bottom_tol), ctypes.byref(fr))
+    return x, it, fr.value
+
+
 def residual(ncell, dx, sym, C, bct, x, b):
     ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); x = _c(x); b = _c(b)
     r = np.zeros_like(b)

@@ -55,9 +55,9 @@ def eshelby_rhs(ora, n):
     return pr, (bct, bcv), rhs
 
 
-def two_phase_rhs(ora, n, corr_cells=4, contrast=1000.0, seed=7):
+def two_phase_rhs(ora, n, corr_cells=4, contrast=1000.0, seed=7, width=2.0):
     """Two-phase 1:contrast problem of n^3 cells and its Newton right-hand side (oracle-built)."""
-    pr = P.two_phase_problem(n, corr_cells=corr_cells, contrast=contrast, seed=seed)
+    pr = P.two_phase_problem(n, corr_cells=corr_cells, contrast=contrast, seed=seed, width=width)
     N = (n + 1) ** 3
     b = ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"])
     rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], pr["bct"], np.zeros(3 * N), b)

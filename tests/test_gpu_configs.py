@@ -98,7 +98,7 @@ def test_config_two_phase_513(cuda):
     import torch
     n = 512
     print("generating the 513^3 two-phase field", flush=True)
-    pr = P.two_phase_problem(n=n, corr_cells=16, contrast=1000.0, seed=5)
+    pr = P.two_phase_problem(n=n, corr_cells=16, contrast=1000.0, seed=5, width=2.0)
     N = (n + 1) ** 3
     C = to_dev(pr["C"], cuda)
     F0 = to_dev(pr["F0"], cuda)

@@ -1,8 +1,8 @@
-"""The selectable isotropic interior kernels stay parity-green: the 62-output row kernel
-(ALAMO_ROWS_TILE=62) and the 64x4 LDS-tile kernel (ALAMO_ISO_KERNEL=tiles, also the path of
-levels beyond the row kernel's 32-bit offsets).  The selection is read when the library loads, so
-each variant runs in a child process; each child checks Fapply, the residual and one Fsmooth call
-against the oracle at 1e-12 / 1e-11 of the field scale, on general and on Dirichlet x faces."""
+"""The 64x4 LDS-tile kernel, the path of isotropic levels beyond the row kernel's 32-bit byte
+offsets (arrays of 4 GiB and more), stays parity-green: ALAMO_ISO_KERNEL=tiles routes every level
+to it.  The selection is read when the library loads, so it runs in a child process, which checks
+Fapply, the residual and one Fsmooth call against the oracle at 1e-12 / 1e-11 of the field scale,
+on general and on Dirichlet x faces."""
 import os
 import subprocess
 import sys
@@ -45,7 +45,7 @@ print("variant ok")
 '''
 
 
-@pytest.mark.parametrize("env", [{"ALAMO_ROWS_TILE": "62"}, {"ALAMO_ISO_KERNEL": "tiles"}])
+@pytest.mark.parametrize("env", [{"ALAMO_ISO_KERNEL": "tiles"}])
 def test_kernel_variant_parity(cuda, env):
     e = dict(os.environ)
     e.update(env)

@@ -213,3 +213,44 @@ def test_abi_names_of_the_reference_calls(cuda, ora):
     op.Reflux(lo, hi, m, c1, fine)
     amr.reflux(NC, lo, hi, m, c2, fine)
     assert np.array_equal(to_host(c1), to_host(c2))
+
+
+def test_newton_rhs_closed_forms(cuda):
+    """The GPU Newton right-hand side (Newton.H:226-230) against closed forms with a varying DW,
+    independent of the oracle: quadratic u with constant isotropic C (interior rows two or more
+    from the boundary: rhs = b - C : grad grad u) and u = 0 with a linear eigenstrain (every
+    interior row: rhs = b + div(C : F0)).  Anisotropic spacings."""
+    ncell, dx = (66, 40, 36), (1 / 66, 1 / 50, 1 / 30)
+    Nn = [n + 1 for n in ncell]
+    k, j, i = np.meshgrid(*[np.arange(n) for n in Nn[::-1]], indexing="ij")
+    I, J, K = i.ravel(), j.ravel(), k.ravel()
+    X = np.stack([I * dx[0], J * dx[1], K * dx[2]])
+    N = X.shape[1]
+    r = rng(17)
+    lam, mu = 1.5, 0.4
+    C = P.iso_field(np.full(N, lam), np.full(N, mu))
+    bct, bcv = P.bc_constant()
+    op = Elastic(ncell, dx, sym=P.ISO, bc=(bct, bcv))
+    op.SetModel(C)
+    # quadratic displacement, no eigenstrain
+    H = [0.5 * (h + h.T) for h in r.normal(size=(3, 3, 3))]
+    u = np.stack([0.5 * np.einsum("in,ij,jn->n", X, H[a], X) for a in range(3)]).ravel()
+    b = r.normal(size=3 * N)
+    rhs = to_dev(np.zeros(3 * N), cuda)
+    op.NewtonRHS(to_dev(u, cuda), to_dev(b, cuda), to_dev(np.zeros(9 * N), cuda), rhs)
+    got = to_host(rhs).reshape(3, N)
+    deep = (I >= 2) & (I <= ncell[0] - 2) & (J >= 2) & (J <= ncell[1] - 2) & (K >= 2) & (K <= ncell[2] - 2)
+    div = np.array([lam * sum(H[q][a, q] for q in range(3))
+                    + mu * (sum(H[a][d, d] for d in range(3)) + sum(H[d][a, d] for d in range(3))) for a in range(3)])
+    want = b.reshape(3, N)[:, deep] - div[:, None]
+    assert np.abs(got[:, deep] - want).max() < 1e-9 * np.abs(want).max()
+    # linear eigenstrain, u = 0
+    F00 = r.normal(size=(3, 3))
+    G = r.normal(size=(3, 3, 3))
+    F0 = (F00[None] + np.einsum("en,eab->nab", X, G)).reshape(N, 9)
+    b0 = np.zeros(3 * N)
+    op.NewtonRHS(to_dev(np.zeros(3 * N), cuda), to_dev(b0, cuda), to_dev(F0.ravel(), cuda), rhs)
+    got = to_host(rhs).reshape(3, N)
+    inner = (I > 0) & (I < ncell[0]) & (J > 0) & (J < ncell[1]) & (K > 0) & (K < ncell[2])
+    divCF = np.array([lam * np.trace(G[a]) + mu * sum(G[d][a, d] + G[d][d, a] for d in range(3)) for a in range(3)])
+    assert np.abs(got[:, inner] - divCF[:, None]).max() < 1e-10 * np.abs(divCF).max()

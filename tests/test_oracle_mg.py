@@ -117,7 +117,7 @@ def test_sharp_high_contrast_diverges(ora):
 
 
 def test_diffuse_high_contrast_converges(ora):
-    pr = P.two_phase_problem(n=16, corr_cells=4, contrast=1000.0, seed=3)
+    pr = P.two_phase_problem(n=16, corr_cells=4, contrast=1000.0, seed=3, width=2.0)
     N = nodes_of(pr["ncell"])
     b = ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"])
     rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], pr["bct"], np.zeros(3 * N), b)

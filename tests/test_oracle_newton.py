@@ -58,3 +58,86 @@ def test_bc_init_values(ora):
     assert b[2, 0, 1, 1] == 3 and b[0, 1, 1, 1] == 0.0
     bh = ora.bc_init(ncell, bct, bcv, homogeneous=True).reshape(3, 4, 4, 4)
     assert bh[0, 1, 1, 3] == 0.5 and bh[1, 1, 1, 3] == 0.0 and bh[2, 0, 1, 1] == 0.0
+
+
+# ---------------------------------------------------------------------------
+# Varying DW: the interior term sum_d d_d DW[a][d] (Newton.H:226-230 through
+# Numeric/Stencil.H:588-657) against closed forms.  Anisotropic spacings so that a
+# wrong 1/(2 dx) factor, a swapped direction or a wrong stride changes the result.
+# ---------------------------------------------------------------------------
+def _grid(ncell, dx):
+    Nn = [n + 1 for n in ncell]
+    k, j, i = np.meshgrid(*[np.arange(n) for n in Nn[::-1]], indexing="ij")
+    return Nn, np.stack([i.ravel() * dx[0], j.ravel() * dx[1], k.ravel() * dx[2]]), (i.ravel(), j.ravel(), k.ravel())
+
+
+def _iso_div_quadratic(lam, mu, H):
+    """div(C : grad u) for u_a = x.H_a.x / 2 (+ linear terms), isotropic C."""
+    out = np.zeros(3)
+    for a in range(3):
+        out[a] = (lam * sum(H[k][a, k] for k in range(3))
+                  + mu * (sum(H[a][d, d] for d in range(3)) + sum(H[d][a, d] for d in range(3))))
+    return out
+
+
+def test_rhs_quadratic_displacement(ora):
+    """F0 = 0, constant C, quadratic u: DW = C : grad u is linear in x, so on nodes two or
+    more away from the boundary (their neighbours' gradients are central) the interior rows
+    are exactly rhs = b - C : grad grad u."""
+    ncell, dx = (6, 7, 8), (0.2, 0.25, 0.1)
+    Nn, X, (I, J, K) = _grid(ncell, dx)
+    N = X.shape[1]
+    rng = np.random.default_rng(11)
+    H = [0.5 * (h + h.T) for h in rng.normal(size=(3, 3, 3))]
+    gl = rng.normal(size=(3, 3))
+    u = np.stack([0.5 * np.einsum("in,ij,jn->n", X, H[a], X) + gl[a] @ X for a in range(3)]).ravel()
+    lam, mu = 2.0, 0.7
+    C = P.iso_field(np.full(N, lam), np.full(N, mu))
+    bct, bcv = P.bc_constant()
+    b = rng.normal(size=3 * N)
+    rhs = ora.newton_rhs(ncell, dx, P.ISO, C, None, bct, u, b).reshape(3, N)
+    deep = (I >= 2) & (I <= ncell[0] - 2) & (J >= 2) & (J <= ncell[1] - 2) & (K >= 2) & (K <= ncell[2] - 2)
+    assert deep.sum() > 20
+    want = b.reshape(3, N)[:, deep] - _iso_div_quadratic(lam, mu, H)[:, None]
+    assert np.abs(rhs[:, deep] - want).max() < 1e-10 * np.abs(want).max()
+
+
+def test_rhs_quadratic_displacement_cubic(ora):
+    """The same with an unrotated cubic MajorMinor C (C11, C12, C44 slots): div(C : grad u)_a =
+    C11 u_a,aa + C12 sum_{k != a} u_k,ka + C44 sum_{d != a} (u_a,dd + u_d,ad)."""
+    ncell, dx = (6, 6, 7), (0.25, 0.2, 0.125)
+    Nn, X, (I, J, K) = _grid(ncell, dx)
+    N = X.shape[1]
+    rng = np.random.default_rng(5)
+    H = [0.5 * (h + h.T) for h in rng.normal(size=(3, 3, 3))]
+    u = np.stack([0.5 * np.einsum("in,ij,jn->n", X, H[a], X) for a in range(3)]).ravel()
+    C11, C12, C44 = 1.68, 1.21, 0.75
+    C = np.tile(P.cubic_mm(C11, C12, C44), N)
+    bct, bcv = P.bc_constant()
+    b = np.zeros(3 * N)
+    rhs = ora.newton_rhs(ncell, dx, P.MAJORMINOR, C, None, bct, u, b).reshape(3, N)
+    deep = (I >= 2) & (I <= ncell[0] - 2) & (J >= 2) & (J <= ncell[1] - 2) & (K >= 2) & (K <= ncell[2] - 2)
+    div = np.array([C11 * H[a][a, a] + C12 * sum(H[k][k, a] for k in range(3) if k != a)
+                    + C44 * sum(H[a][d, d] + H[d][a, d] for d in range(3) if d != a) for a in range(3)])
+    assert np.abs(rhs[:, deep] + div[:, None]).max() < 1e-10 * np.abs(div).max()
+
+
+def test_rhs_linear_eigenstrain(ora):
+    """u = 0, constant isotropic C, eigenstrain F0(x) = F00 + sum_e x_e G_e: DW = -C : F0 is
+    linear, so every interior row is exactly rhs = b + div(C : F0)."""
+    ncell, dx = (5, 6, 7), (0.2, 0.125, 0.25)
+    Nn, X, (I, J, K) = _grid(ncell, dx)
+    N = X.shape[1]
+    rng = np.random.default_rng(3)
+    F00 = rng.normal(size=(3, 3))
+    G = rng.normal(size=(3, 3, 3))
+    F0 = (F00[None] + np.einsum("en,eab->nab", X, G)).reshape(N, 9)
+    lam, mu = 1.5, 0.4
+    C = P.iso_field(np.full(N, lam), np.full(N, mu))
+    bct, bcv = P.bc_constant()
+    b = ora.bc_init(ncell, bct, bcv)
+    rhs = ora.newton_rhs(ncell, dx, P.ISO, C, F0.ravel(), bct, np.zeros(3 * N), b).reshape(3, N)
+    inner = (I > 0) & (I < ncell[0]) & (J > 0) & (J < ncell[1]) & (K > 0) & (K < ncell[2])
+    divCF = np.array([lam * np.trace(G[a]) + mu * sum(G[d][a, d] + G[d][d, a] for d in range(3))
+                      for a in range(3)])
+    assert np.abs(rhs[:, inner] - divCF[:, None]).max() < 1e-11 * np.abs(divCF).max()
