@@ -20,7 +20,8 @@ class MGDesc(ctypes.Structure):
                 ("max_coarsening_level", c_int), ("precision", c_int), ("cycle", c_int),
                 ("pre_smooth", c_int), ("post_smooth", c_int), ("sweeps_per_smooth", c_int),
                 ("omega", c_double), ("max_iter", c_int), ("bottom_max_iter", c_int),
-                ("bottom_tol_rel", c_double), ("verbose", c_int), ("smoother", c_int), ("fixed_iter", c_int),
+                ("bottom_tol_rel", c_double), ("verbose", c_int), ("smoother", c_int), ("coarse_op", c_int),
+                ("fixed_iter", c_int),
                 ("bottom_solver", c_int), ("max_fmg_iter", c_int), ("sweeps_finest", c_int)]
 
 

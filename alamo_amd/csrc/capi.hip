@@ -24,6 +24,7 @@
 #include "krylov.h"
 #include "amr.h"
 #include "post.h"
+#include "galerkin.h"
 
 using namespace alamo;
 
@@ -74,6 +75,8 @@ struct LevelBuf {
     float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
     double* dsm = nullptr;  // l1-Jacobi smoother diagonal (3 planes), allocated when that smoother is used
     float* dsmf = nullptr;
+    double* S = nullptr;    // Galerkin stencil of a coarse level (GAL_NPLANE planes), coarse_op GALERKIN
+    float* Sf = nullptr;
     double* psi = nullptr;  // Elastic::m_psi_mf of the level (cells), when set
     float* psif = nullptr;
     uint16_t* gid = nullptr;  // per-grain model (finest level): grain ids and the grain table
@@ -109,6 +112,7 @@ struct alamo_mg {
     float* kVf = nullptr;     // FP32 basis and preconditioned vectors of the mixed-precision inner iterations
     float* kZf = nullptr;
     int kry_mf = 0;
+    bool gal = false;         // levels >= 1 run the prepared Galerkin stencils
     bool psi_set = false;     // Elastic::m_psi_set
     bool uniform = false;     // Elastic::m_uniform
     double psi_small = 1e-8;  // Elastic::m_psi_small
@@ -257,6 +261,7 @@ template <> struct LB<double> {
     static double*& tmp(LevelBuf& l) { return l.tmp; }
     static double*& hold(LevelBuf& l) { return l.hold; }
     static double*& dsm(LevelBuf& l) { return l.dsm; }
+    static double*& S(LevelBuf& l) { return l.S; }
 };
 template <> struct LB<float> {
     static float*& C(LevelBuf& l) { return l.Cf; }
@@ -267,7 +272,17 @@ template <> struct LB<float> {
     static float*& tmp(LevelBuf& l) { return l.tmpf; }
     static float*& hold(LevelBuf& l) { return l.holdf; }
     static float*& dsm(LevelBuf& l) { return l.dsmf; }
+    static float*& S(LevelBuf& l) { return l.Sf; }
 };
+
+// The operator of level l in the cycle precision: the level's rows (level 0 always; every level
+// with coarse_op RESTRICT, the reference's re-discretisation), or its Galerkin stencil.
+template <typename T>
+static void level_op(alamo_mg* h, int l, int mode, const T* u, const T* b, T* out, T omega, hipStream_t s) {
+    LevelBuf& lb = h->L[l];
+    if (l > 0 && h->gal) launch_stencil<T>(mode, lb.g, LB<T>::S(lb), u, b, out, omega, s);
+    else launch_node<T>(h->sym, mode, lb.g, h->bc, LB<T>::C(lb), u, b, out, omega, s);
+}
 
 // Diagonal and damping of a first sweep from a zero guess, x = w b / d: the prepared point
 // diagonal with omega (the reference smoother), or the l1 smoother diagonal with weight 1.
@@ -281,8 +296,7 @@ template <typename T>
 static void sweeps(alamo_mg* h, int l, int count) {
     LevelBuf& lb = h->L[l];
     for (int s = 0; s < count; s++) {
-        launch_node<T>(h->sym, JACOBI, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::tmp(lb),
-                       T(h->d.omega), h->stream);
+        level_op<T>(h, l, JACOBI, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::tmp(lb), T(h->d.omega), h->stream);
         std::swap(LB<T>::cor(lb), LB<T>::tmp(lb));
     }
 }
@@ -357,8 +371,7 @@ static void vcycle(alamo_mg* h, int top, bool first_done = false) {
     for (int l = top; l < bot; l++) {
         LevelBuf& lb = h->L[l];
         presmooth_from_zero<T>(h, l, done);
-        launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
-                       T(0), h->stream);
+        level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
         LevelBuf& lc = h->L[l + 1];
         // the restriction also applies the next level's first pre-smoothing sweep (from zero)
         done = l + 1 < bot && h->d.pre_smooth * sps(h, l + 1) > 0;
@@ -391,8 +404,7 @@ static void fcycle(alamo_mg* h) {
         vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
         k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
         // res = res - L(cor)
-        launch_node<T>(h->sym, RESID, lb.g, h->bc, LB<T>::C(lb), LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb),
-                       T(0), h->stream);
+        level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
         std::swap(LB<T>::res(lb), LB<T>::rescor(lb));
         std::swap(LB<T>::hold(lb), LB<T>::cor(lb));
         vcycle<T>(h, l);
@@ -465,16 +477,38 @@ static int dense_bottom_factor(alamo_mg* h) {
         TRY(dalloc(h, &h->dense_inv, n * n));
         h->dense_n = n;
     }
-    k_unit_columns<<<gs_blocks(n * n), 256, 0, h->stream>>>(h->dense_inv, n);
-    const unsigned cols = (unsigned)n;
-    if (h->sym == ISO) k_bottom_columns<ISO><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
-    else if (h->sym == MAJORMINOR)
-        k_bottom_columns<MAJORMINOR><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
-    else k_bottom_columns<DIAGONAL><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
-    HIPCHK(hipGetLastError());
-    std::vector<double> A(n * n), inv;
-    HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> A(n * n, 0.0), inv;
+    if (h->gal && h->L.size() > 1) {
+        // the coarsest Galerkin stencil written out column-major
+        const Geom& g = lb.g;
+        std::vector<double> S((size_t)GAL_NPLANE * g.tot);
+        HIPCHK(hipMemcpyAsync(S.data(), lb.S, S.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int k = 0; k <= g.n[2]; k++)
+            for (int j = 0; j <= g.n[1]; j++)
+                for (int i = 0; i <= g.n[0]; i++) {
+                    const int64_t m = i + g.sy * j + g.sz * k;
+                    for (int o = 0; o < 27; o++) {
+                        const int ii = i + o % 3 - 1, jj = j + (o / 3) % 3 - 1, kk = k + o / 9 - 1;
+                        if (ii < 0 || jj < 0 || kk < 0 || ii > g.n[0] || jj > g.n[1] || kk > g.n[2]) continue;
+                        const int64_t mn = ii + g.sy * jj + g.sz * kk;
+                        for (int q = 0; q < 3; q++)
+                            for (int pp = 0; pp < 3; pp++)
+                                A[(pp * g.tot + mn) * n + q * g.tot + m] = S[(size_t)gal_plane(o, q, pp) * g.tot + m];
+                    }
+                }
+    } else {
+        k_unit_columns<<<gs_blocks(n * n), 256, 0, h->stream>>>(h->dense_inv, n);
+        const unsigned cols = (unsigned)n;
+        if (h->sym == ISO)
+            k_bottom_columns<ISO><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+        else if (h->sym == MAJORMINOR)
+            k_bottom_columns<MAJORMINOR><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+        else k_bottom_columns<DIAGONAL><<<cols, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, h->dense_inv, h->dense_A);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     if (!dense_invert(A, n, inv))
         FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest-level operator is singular (no displacement "
                             "condition pins it); use bottom_solver bicgstab or smoother");
@@ -506,17 +540,47 @@ static int prepare(alamo_mg* h) {
         }
     }
     const bool l1 = h->d.smoother == ALAMO_SMOOTHER_L1;
+    const bool mixed = h->d.precision == ALAMO_PREC_MIXED;
     if (l1 && h->psi_set) FAIL(ALAMO_ERR_ARG, "the l1 smoother has no psi terms; use smoother JACOBI with SetPsi");
+    h->gal = false;
+    if (h->d.coarse_op == ALAMO_COARSE_GALERKIN && nl > 1) {
+        if (h->psi_set) FAIL(ALAMO_ERR_ARG, "coarse_op GALERKIN has no psi terms; use coarse_op RESTRICT with SetPsi");
+        if (h->d.bottom_solver == ALAMO_BOTTOM_BICGSTAB)
+            FAIL(ALAMO_ERR_ARG, "coarse_op GALERKIN needs bottom_solver dense or smoother");
+        // A_l = R A_{l-1} P: level 1 from the level-0 rows, deeper levels from the stencil above
+        for (int l = 1; l < nl; l++) {
+            LevelBuf& lc = h->L[l];
+            LevelBuf& lf = h->L[l - 1];
+            if (!lc.S) TRY(dalloc(h, &lc.S, (int64_t)GAL_NPLANE * lc.g.tot));
+            if (mixed && !lc.Sf) TRY(dalloc(h, &lc.Sf, (int64_t)GAL_NPLANE * lc.g.tot));
+            const unsigned nb = nblocks(27 * lc.g.tot);
+            if (l == 1) {
+                if (h->sym == ISO) k_rap0<ISO><<<nb, 256, 0, h->stream>>>(lc.g, lf.g, h->bc, lf.C, lc.S);
+                else if (h->sym == MAJORMINOR)
+                    k_rap0<MAJORMINOR><<<nb, 256, 0, h->stream>>>(lc.g, lf.g, h->bc, lf.C, lc.S);
+                else k_rap0<DIAGONAL><<<nb, 256, 0, h->stream>>>(lc.g, lf.g, h->bc, lf.C, lc.S);
+            } else {
+                k_rap<<<nb, 256, 0, h->stream>>>(lc.g, lf.g, lf.S, lc.S);
+            }
+            if (mixed)
+                k_convert<double, float><<<gs_blocks(GAL_NPLANE * lc.g.tot), 256, 0, h->stream>>>(
+                    lc.S, lc.Sf, (int64_t)GAL_NPLANE * lc.g.tot);
+        }
+        h->gal = true;
+    }
     for (int l = 0; l < nl; l++) {
         LevelBuf& lb = h->L[l];
-        launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, lb.D, 0.0, h->stream);
+        const bool st = h->gal && l > 0;
+        if (st) launch_stencil<double>(DIAG, lb.g, lb.S, nullptr, nullptr, lb.D, 0.0, h->stream);
+        else launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, lb.D, 0.0, h->stream);
         lb.g.dsm = nullptr;
         lb.g.dsmf = nullptr;
         if (l1) {
-            // the smoother diagonal, from the level's coefficients and boundary table
+            // the smoother diagonal, from the level's coefficients and boundary table (or stencil)
             if (!lb.dsm) TRY(dalloc(h, &lb.dsm, 3 * lb.g.tot));
             const unsigned nb = nblocks(lb.g.tot);
-            if (h->sym == ISO) k_l1_diag<double, ISO><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
+            if (st) k_stencil_l1<double><<<nb, 256, 0, h->stream>>>(lb.g, lb.S, lb.dsm);
+            else if (h->sym == ISO) k_l1_diag<double, ISO><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
             else if (h->sym == MAJORMINOR)
                 k_l1_diag<double, MAJORMINOR><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
             else k_l1_diag<double, DIAGONAL><<<nb, 256, 0, h->stream>>>(lb.g, h->bc, lb.C, lb.dsm);
@@ -1017,6 +1081,7 @@ int alamo_mg_desc_default(alamo_mg_desc* d) {
     d->bottom_tol_rel = 1e-4;
     d->verbose = 0;
     d->smoother = ALAMO_SMOOTHER_JACOBI;
+    d->coarse_op = ALAMO_COARSE_RESTRICT;
     d->fixed_iter = 0;
     d->bottom_solver = ALAMO_BOTTOM_BICGSTAB;
     d->max_fmg_iter = 100000000;
@@ -1043,6 +1108,8 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
     if (d->smoother != ALAMO_SMOOTHER_JACOBI && d->smoother != ALAMO_SMOOTHER_L1) FAIL(ALAMO_ERR_ARG, "unknown smoother");
+    if (d->coarse_op != ALAMO_COARSE_RESTRICT && d->coarse_op != ALAMO_COARSE_GALERKIN)
+        FAIL(ALAMO_ERR_ARG, "unknown coarse_op");
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -1152,9 +1219,13 @@ int alamo_mg_update_opts(alamo_mg* h, const alamo_mg_desc* d) {
         FAIL(ALAMO_ERR_ARG, "unknown bottom_solver");
     if (d->sweeps_finest < 0) FAIL(ALAMO_ERR_ARG, "sweeps_finest must be >= 0");
     if (d->smoother != ALAMO_SMOOTHER_JACOBI && d->smoother != ALAMO_SMOOTHER_L1) FAIL(ALAMO_ERR_ARG, "unknown smoother");
-    // the dense inverse and the smoother diagonals are built at prepare
-    if (d->bottom_solver != h->d.bottom_solver || d->smoother != h->d.smoother) h->prepared = false;
+    if (d->coarse_op != ALAMO_COARSE_RESTRICT && d->coarse_op != ALAMO_COARSE_GALERKIN)
+        FAIL(ALAMO_ERR_ARG, "unknown coarse_op");
+    // the dense inverse, the smoother diagonals and the Galerkin stencils are built at prepare
+    if (d->bottom_solver != h->d.bottom_solver || d->smoother != h->d.smoother || d->coarse_op != h->d.coarse_op)
+        h->prepared = false;
     h->d.smoother = d->smoother;
+    h->d.coarse_op = d->coarse_op;
     h->d.bottom_solver = d->bottom_solver;
     h->d.max_fmg_iter = d->max_fmg_iter;
     h->d.sweeps_finest = d->sweeps_finest;
@@ -1344,7 +1415,7 @@ int alamo_mg_error0x(alamo_mg* h, int amrlev, int mglev, double* R0x, const doub
     // D0x = x / D, R0x = A D0x, R0x = x - R0x
     HIPCHK(hipMemcpyAsync(d0x, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     k_divide<<<gs_blocks(n), 256, 0, s>>>(d0x, lb.D, n);
-    launch_node<double>(h->sym, RESID, lb.g, h->bc, lb.C, d0x, x, R0x, 0.0, s);
+    level_op<double>(h, mglev, RESID, d0x, x, R0x, 0.0, s);
     HIPCHK(hipFreeAsync(d0x, s));
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
@@ -1383,7 +1454,8 @@ int alamo_mg_get_coeff(alamo_mg* h, int mglev, double* C_soa) {
 int alamo_mg_fapply(alamo_mg* h, int amrlev, int mglev, double* out, const double* in, void* stream) {
     TRY(check_level(h, amrlev, mglev));
     LevelBuf& lb = h->L[mglev];
-    launch_node<double>(h->sym, APPLY, lb.g, h->bc, lb.C, in, nullptr, out, 0.0, (hipStream_t)stream);
+    (void)lb;
+    level_op<double>(h, mglev, APPLY, in, nullptr, out, 0.0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -1391,7 +1463,8 @@ int alamo_mg_fapply(alamo_mg* h, int amrlev, int mglev, double* out, const doubl
 int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* stream) {
     TRY(check_level(h, amrlev, mglev));
     LevelBuf& lb = h->L[mglev];
-    launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, diag, 0.0, (hipStream_t)stream);
+    (void)lb;
+    level_op<double>(h, mglev, DIAG, nullptr, nullptr, diag, 0.0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -1414,7 +1487,7 @@ int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double
     double* a = x;
     double* c = tmp;
     for (int s = 0; s < h->d.sweeps_per_smooth; s++) {
-        launch_node<double>(h->sym, JACOBI, lb.g, h->bc, lb.C, a, b, c, h->d.omega, (hipStream_t)stream);
+        level_op<double>(h, mglev, JACOBI, a, b, c, h->d.omega, (hipStream_t)stream);
         std::swap(a, c);
     }
     if (a != x) HIPCHK(hipMemcpyAsync(x, a, 3 * lb.g.tot * sizeof(double), hipMemcpyDeviceToDevice,
@@ -1428,7 +1501,8 @@ int alamo_mg_residual(alamo_mg* h, int amrlev, int mglev, double* r, const doubl
                       void* stream) {
     TRY(check_level(h, amrlev, mglev));
     LevelBuf& lb = h->L[mglev];
-    launch_node<double>(h->sym, RESID, lb.g, h->bc, lb.C, x, b, r, 0.0, (hipStream_t)stream);
+    (void)lb;
+    level_op<double>(h, mglev, RESID, x, b, r, 0.0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }

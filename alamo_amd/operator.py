@@ -32,6 +32,19 @@ def _ptr(t):
 
 BOTTOM_SOLVERS = {"bicgstab": 0, "smoother": 1, "dense": 2}
 SMOOTHERS = {"jacobi": 0, "l1": 1}
+COARSE_OPS = {"restrict": 0, "galerkin": 1}
+
+
+def coarse_op_code(name):
+    """operators of the coarse levels: "restrict" (re-discretised from restricted coefficients, the
+    reference, Elastic.cpp:746-836) or "galerkin" (R A P; no reference counterpart)."""
+    if isinstance(name, int):
+        if name not in COARSE_OPS.values():
+            raise ValueError("unknown coarse_op %d" % name)
+        return name
+    if name not in COARSE_OPS:
+        raise ValueError("coarse_op must be one of %s, not %r" % (sorted(COARSE_OPS), name))
+    return COARSE_OPS[name]
 
 
 def smoother_code(name):
@@ -66,7 +79,7 @@ class Elastic:
     def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
                  post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
                  bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, smoother="jacobi", fixed_iter=0,
-                 bottom_solver="bicgstab", sweeps_finest=0):
+                 bottom_solver="bicgstab", sweeps_finest=0, coarse_op="restrict"):
         d = MGDesc()
         check(lib().alamo_mg_desc_default(ctypes.byref(d)))
         for q in range(3):
@@ -86,6 +99,7 @@ class Elastic:
         d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
         d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
         d.smoother = smoother_code(smoother)
+        d.coarse_op = coarse_op_code(coarse_op)
         d.fixed_iter = int(fixed_iter)
         d.bottom_solver = bottom_solver_code(bottom_solver)
         d.sweeps_finest = int(sweeps_finest)
@@ -131,10 +145,10 @@ class Elastic:
     def UpdateOpts(self, **kw):
         """Change solver options of the built operator (cycle, pre_smooth, post_smooth,
         sweeps_per_smooth, omega, max_iter, bottom_max_iter, bottom_tol_rel, fixed_iter, verbose,
-        bottom_solver, max_fmg_iter, sweeps_finest, smoother)."""
+        bottom_solver, max_fmg_iter, sweeps_finest, smoother, coarse_op)."""
         allowed = {"cycle", "pre_smooth", "post_smooth", "sweeps_per_smooth", "omega", "max_iter",
                    "bottom_max_iter", "bottom_tol_rel", "fixed_iter", "verbose", "bottom_solver", "max_fmg_iter",
-                   "sweeps_finest", "smoother"}
+                   "sweeps_finest", "smoother", "coarse_op"}
         for k, v in kw.items():
             if k not in allowed:
                 raise ValueError("not a solver option: " + k)
@@ -142,6 +156,8 @@ class Elastic:
                 v = bottom_solver_code(v)
             if k == "smoother":
                 v = smoother_code(v)
+            if k == "coarse_op":
+                v = coarse_op_code(v)
             setattr(self.desc, k, v)
         check(lib().alamo_mg_update_opts(self.h, ctypes.byref(self.desc)))
 

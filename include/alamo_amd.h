@@ -41,6 +41,7 @@ enum { ALAMO_BOTTOM_BICGSTAB = 0, /* Jacobi-preconditioned BiCGStab on the coars
                                      (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle */ };
 #define ALAMO_DENSE_BOTTOM_MAX 1200
 enum { ALAMO_SMOOTHER_JACOBI = 0, ALAMO_SMOOTHER_L1 = 1 };
+enum { ALAMO_COARSE_RESTRICT = 0, ALAMO_COARSE_GALERKIN = 1 };
 enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, ALAMO_ERR_NOCONV = 4,
        ALAMO_ERR_NAN = 5 };
 
@@ -72,6 +73,14 @@ typedef struct {
                                   damping), a smoother that stays stable on rows whose off-diagonal
                                   sum exceeds the diagonal, as the strong-form rows next to a sharp
                                   stiffness jump do; no reference counterpart, not with psi. */
+    int coarse_op;             /* ALAMO_COARSE_*: the operators of the levels >= 1 of the cycles.
+                                  RESTRICT: re-discretised from the restricted coefficients (the
+                                  reference, Elastic.cpp:746-836).  GALERKIN: A_l = R A_{l-1} P with
+                                  the cycle's own restriction and prolongation, a 27-point stencil of
+                                  3 x 3 blocks per coarse node built at prepare; robust where the
+                                  restricted coefficients misrepresent sharp stiffness jumps.  Needs
+                                  bottom_solver dense or smoother; not with psi.  No reference
+                                  counterpart.  Level 0 keeps the reference operator either way. */
     int fixed_iter;            /* solver.fixed_iter (Linear.H:181,267): > 0 runs exactly that many
                                   multigrid iterations and returns without a convergence failure
                                   (amrex::MLMG::setFixedIter); 0 off */

@@ -196,6 +196,14 @@ public:
         opt().smoother = v;
         push();
     }
+    // operators of the coarse levels: "restrict" (the reference's re-discretisation) or "galerkin"
+    // (R A P; no reference counterpart)
+    void setCoarseOp(const std::string& name) {
+        int v = name == "restrict" ? ALAMO_COARSE_RESTRICT : name == "galerkin" ? ALAMO_COARSE_GALERKIN : -1;
+        if (v < 0) throw alamo::Abort(ALAMO_ERR_ARG, "unknown coarse_op " + name);
+        opt().coarse_op = v;
+        push();
+    }
     // Linear::apply (Linear.H:169-174): rhs = A sol on the finest level
     void apply(double* rhs, const double* sol) { defined().Apply(0, 0, rhs, sol); }
     // Linear::solveaffine (Linear.H:53-130) subtracts A 0 under inhomogeneous boundaries first;

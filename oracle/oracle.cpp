@@ -1188,6 +1188,74 @@ int ora_mg_solve(const int* ncell, const double* dx, int sym, const double* C, c
     MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, 1e-4};
     return mg_solve(H, o, x, b, tol_rel, tol_abs, hist, hist_cap, final_res);
 }
+// Galerkin coarse operators A_l = R A_{l-1} P (no reference counterpart; the library's coarse_op
+// GALERKIN), read off by probing the composition restrict_fw(A_{l-1}(interp_add(e))): the image
+// of a coarse unit vector reaches the coarse nodes within one node of it, so the 27 node classes
+// (I mod 3, J mod 3, K mod 3) x 3 components give every entry.  Level 0's A is fapply; deeper
+// levels apply the stencil of the level above.  Stencil layout: plane 9 o + 3 q + p (o = (dx+1) +
+// 3 (dy+1) + 9 (dz+1)) of tot values, the coefficient of u_p at node + o in row q.
+// Pinned by tests/test_oracle_galerkin.py (stencil products equal the composition on random
+// vectors; interior rows of constant-coefficient operators annihilate linear fields).
+static void stencil_apply(const Level& L, const double* S, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < L.N[2]; k++)
+        for (int j = 0; j < L.N[1]; j++)
+            for (int i = 0; i < L.N[0]; i++) {
+                const int64_t m = L.id(i, j, k);
+                double f[3] = {0, 0, 0};
+                for (int o = 0; o < 27; o++) {
+                    const int ii = i + o % 3 - 1, jj = j + (o / 3) % 3 - 1, kk = k + o / 9 - 1;
+                    if (ii < 0 || jj < 0 || kk < 0 || ii > L.n[0] || jj > L.n[1] || kk > L.n[2]) continue;
+                    const int64_t mn = L.id(ii, jj, kk);
+                    for (int q = 0; q < 3; q++)
+                        for (int pp = 0; pp < 3; pp++) f[q] += S[(9 * o + 3 * q + pp) * L.tot + m] * x[pp * L.tot + mn];
+                }
+                for (int q = 0; q < 3; q++) y[q * L.tot + m] = f[q];
+            }
+}
+
+static void galerkin_levels(Hier& H, int upto, std::vector<std::vector<double>>& S) {
+    S.assign(H.L.size(), {});
+    for (int l = 1; l <= upto && l < (int)H.L.size(); l++) {
+        const Level& Lc = H.L[l];
+        const Level& Lf = H.L[l - 1];
+        S[l].assign((size_t)243 * Lc.tot, 0.0);
+        std::vector<double> e(3 * Lc.tot), pf(3 * Lf.tot), af(3 * Lf.tot), y(3 * Lc.tot);
+        for (int c = 0; c < 27; c++)
+            for (int p = 0; p < 3; p++) {
+                std::fill(e.begin(), e.end(), 0.0);
+                for (int K = c / 9; K < Lc.N[2]; K += 3)
+                    for (int J = (c / 3) % 3; J < Lc.N[1]; J += 3)
+                        for (int I = c % 3; I < Lc.N[0]; I += 3) e[p * Lc.tot + Lc.id(I, J, K)] = 1.0;
+                std::fill(pf.begin(), pf.end(), 0.0);
+                interp_add(Lc, Lf, e.data(), pf.data());
+                if (l == 1) fapply(Lf, H.sym, H.C[0].data(), H.bct, pf.data(), af.data());
+                else stencil_apply(Lf, S[l - 1].data(), pf.data(), af.data());
+                restrict_fw(Lc, Lf, 1, 3, af.data(), y.data());
+                const int cc[3] = {c % 3, (c / 3) % 3, c / 9};
+                for (int K = 0; K < Lc.N[2]; K++)
+                    for (int J = 0; J < Lc.N[1]; J++)
+                        for (int I = 0; I < Lc.N[0]; I++) {
+                            const int idx[3] = {I, J, K};
+                            int off[3];
+                            bool ok = true;
+                            for (int d = 0; d < 3; d++) {
+                                // the node of class cc[d] among idx - 1 .. idx + 1
+                                off[d] = -2;
+                                for (int t = -1; t <= 1; t++)
+                                    if (((idx[d] + t) % 3 + 3) % 3 == cc[d]) off[d] = t;
+                                const int nb = idx[d] + off[d];
+                                ok = ok && nb >= 0 && nb <= Lc.n[d];
+                            }
+                            if (!ok) continue;
+                            const int o = (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1);
+                            const int64_t m = Lc.id(I, J, K);
+                            for (int q = 0; q < 3; q++) S[l][(size_t)(9 * o + 3 * q + p) * Lc.tot + m] = y[q * Lc.tot + m];
+                        }
+            }
+    }
+}
+
 // FGMRES(restart) with one V-cycle (nu1 / nu2 smoothing calls of `sweeps` sweeps, `sweeps_finest` on
 // level 0 when > 0; smoother 0 damped point Jacobi, 1 l1-Jacobi; BiCGStab bottom to bottom_tol) as the
 // right preconditioner.  Returns the iterations, negative when max_iter ended the solve.
@@ -1207,6 +1275,18 @@ int ora_krylov_solve(const int* ncell, const double* dx, int sym, const double* 
     o.smoother = smoother;
     o.sweeps_finest = sweeps_finest;
     return fgmres(H, o, restart, x, b, tol_rel, tol_abs, final_res);
+}
+// the Galerkin stencil of multigrid level `level` >= 1 (243 planes of that level's nodes)
+void ora_galerkin(const int* ncell, const double* dx, int sym, const double* C, const int* bct, int level, double* out) {
+    Hier H;
+    build_hier(H, ncell, dx, sym, bct, C, -1);
+    std::vector<std::vector<double>> S;
+    galerkin_levels(H, level, S);
+    if (level >= 1 && level < (int)H.L.size()) std::copy(S[level].begin(), S[level].end(), out);
+}
+void ora_stencil_apply(const int* ncell, const double* dx, const double* S, const double* x, double* y) {
+    Level L(ncell, dx);
+    stencil_apply(L, S, x, y);
 }
 void ora_l1_diag_rows(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* d) {
     Level L(ncell, dx);

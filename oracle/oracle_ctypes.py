@@ -121,6 +121,22 @@ def krylov_solve(ncell, dx, sym, C, bct, b, x0=None, tol_rel=1e-10, tol_abs=0.0,
     return x, it, fr.value
 
 
+def galerkin(ncell, dx, sym, C, bct, level):
+    """Galerkin stencil R A P of multigrid level `level` (243 planes: 9 o + 3 q + p)"""
+    ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32)
+    nc = tuple(int(n) // (2 ** level) for n in ncell)
+    out = np.zeros(243 * int(np.prod([n + 1 for n in nc])))
+    lib().ora_galerkin(_i(ncell), _d(dx), sym, _d(C), _i(bct), level, _d(out))
+    return out
+
+
+def stencil_apply(ncell, dx, S, x):
+    ncell = _c(ncell, np.int32); dx = _c(dx); S = _c(S); x = _c(x)
+    y = np.zeros_like(x)
+    lib().ora_stencil_apply(_i(ncell), _d(dx), _d(S), _d(x), _d(y))
+    return y
+
+
 def residual(ncell, dx, sym, C, bct, x, b):
     ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); x = _c(x); b = _c(b)
     r = np.zeros_like(b)
