@@ -89,6 +89,7 @@ SIGNATURES = {
     "alamo_mg_fapply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_diagonal": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_fsmooth": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_mg_fsmooth_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_error0x": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_mg_normalize": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "alamo_mg_residual": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/alamo_amd.h"
@@ -62,6 +63,32 @@ static const bool g_iso_tiles = [] {
         int r_ = (expr);           \
         if (r_ != ALAMO_OK) return r_; \
     } while (0)
+
+// Device scratch of alamo_mg_fsmooth_noalloc: the ping-pong partner of its sweeps,
+// ALAMO_NOALLOC_MAX_DOUBLES doubles (64 MiB) on every device the library runs on, from library load
+// to process end.  Host code never passes `scratch` itself to a kernel (on the host the name is a
+// stand-in, not a device address): scratch_ptr() looks up this device's copy once with
+// hipGetSymbolAddress and every use goes through that pointer.
+static __device__ __align__(256) double scratch[ALAMO_NOALLOC_MAX_DOUBLES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// the address of the current device's `scratch` (looked up on the first call on each device)
+static int scratch_ptr(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) FAIL(ALAMO_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) FAIL(ALAMO_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ALAMO_OK;
+}
 
 namespace {
 
@@ -1110,6 +1137,8 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     if (d->smoother != ALAMO_SMOOTHER_JACOBI && d->smoother != ALAMO_SMOOTHER_L1) FAIL(ALAMO_ERR_ARG, "unknown smoother");
     if (d->coarse_op != ALAMO_COARSE_RESTRICT && d->coarse_op != ALAMO_COARSE_GALERKIN)
         FAIL(ALAMO_ERR_ARG, "unknown coarse_op");
+    double* scr = nullptr;  // the no-allocation entry point's scratch on this device
+    TRY(scratch_ptr(&scr));
     alamo_mg* h = new alamo_mg();
     h->d = *d;
     h->sym = d->sym;
@@ -1493,6 +1522,34 @@ int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double
     if (a != x) HIPCHK(hipMemcpyAsync(x, a, 3 * lb.g.tot * sizeof(double), hipMemcpyDeviceToDevice,
                                       (hipStream_t)stream));
     HIPCHK(hipFreeAsync(tmp, (hipStream_t)stream));
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_fsmooth_noalloc(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream) {
+    if (!h || !x || !b) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (amrlev != 0) FAIL(ALAMO_ERR_ARG, "only amrlev 0 is supported by this handle");
+    if (mglev < 0 || mglev >= (int)h->L.size()) FAIL(ALAMO_ERR_ARG, "mglev out of range");
+    // preparing allocates (diagonals, stencils, dense bottom): the caller prepares beforehand
+    if (!h->prepared)
+        FAIL(ALAMO_ERR_STATE, "alamo_mg_fsmooth_noalloc: the handle is not prepared; call alamo_mg_prepare first");
+    LevelBuf& lb = h->L[mglev];
+    const int64_t n = 3 * lb.g.tot;
+    if (n > (int64_t)ALAMO_NOALLOC_MAX_DOUBLES)
+        FAIL(ALAMO_ERR_ARG, "alamo_mg_fsmooth_noalloc: level " + std::to_string(mglev) + " has " +
+                                std::to_string(lb.g.tot) + " nodes; the scratch holds 3 x " +
+                                std::to_string((int64_t)ALAMO_NOALLOC_MAX_DOUBLES / 3) + " values");
+    double* tmp = nullptr;
+    TRY(scratch_ptr(&tmp));
+    const hipStream_t s = (hipStream_t)stream;
+    // every sweep writes all 3 n entries of its output, so nothing in the scratch needs clearing
+    double* a = x;
+    double* c = tmp;
+    for (int sw = 0; sw < h->d.sweeps_per_smooth; sw++) {
+        level_op<double>(h, mglev, JACOBI, a, b, c, h->d.omega, s);
+        std::swap(a, c);
+    }
+    if (a != x) k_convert<double, double><<<gs_blocks(n), 256, 0, s>>>(a, x, n);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }

@@ -235,6 +235,12 @@ class Elastic:
         check(lib().alamo_mg_fsmooth(self.h, amrlev, mglev, self._vec(x, mglev, "Fsmooth x"),
                                       self._vec(b, mglev, "Fsmooth b"), stream))
 
+    def FsmoothNoAlloc(self, amrlev, mglev, x, b, stream=None):
+        """alamo_mg_fsmooth_noalloc: Fsmooth through the library's static device scratch (no
+        allocation; the handle must be prepared; levels of at most 8388608 / 3 nodes)."""
+        check(lib().alamo_mg_fsmooth_noalloc(self.h, amrlev, mglev, self._vec(x, mglev, "Fsmooth x"),
+                                             self._vec(b, mglev, "Fsmooth b"), stream))
+
     def correctionResidual(self, amrlev, mglev, resid, x, b, stream=None):
         check(lib().alamo_mg_residual(self.h, amrlev, mglev, self._vec(resid, mglev, "residual r"),
                                        self._vec(x, mglev, "residual x"), self._vec(b, mglev, "residual b"), stream))

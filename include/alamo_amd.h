@@ -40,6 +40,7 @@ enum { ALAMO_BOTTOM_BICGSTAB = 0, /* Jacobi-preconditioned BiCGStab on the coars
        ALAMO_BOTTOM_DENSE = 2     /* direct: the coarsest operator assembled and inverted at prepare
                                      (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle */ };
 #define ALAMO_DENSE_BOTTOM_MAX 1200
+#define ALAMO_NOALLOC_MAX_DOUBLES 8388608 /* 64 MiB: the device scratch of alamo_mg_fsmooth_noalloc */
 enum { ALAMO_SMOOTHER_JACOBI = 0, ALAMO_SMOOTHER_L1 = 1 };
 enum { ALAMO_COARSE_RESTRICT = 0, ALAMO_COARSE_GALERKIN = 1 };
 enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, ALAMO_ERR_NOCONV = 4,
@@ -200,6 +201,16 @@ int alamo_mg_diagonal(alamo_mg* h, int amrlev, int mglev, double* diag, void* st
 int alamo_mg_normalize(alamo_mg* h, int amrlev, int mglev, double* x, void* stream);
 /* Operator<Grid::Node>::Fsmooth (Operator.cpp:88-154): sweeps_per_smooth damped-Jacobi sweeps. */
 int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream);
+/* alamo_mg_fsmooth without any allocation: no reference counterpart.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, a serving loop, an external multigrid
+ * driver calling the smoother once per level and cycle): the sweeps ping-pong through a static
+ * device scratch array of ALAMO_NOALLOC_MAX_DOUBLES doubles (64 MiB) that every GPU the library
+ * runs on holds until the process ends, instead of a buffer allocated per call.  Same kernels and
+ * results as alamo_mg_fsmooth.  The handle must be prepared (alamo_mg_prepare) beforehand, since
+ * preparing allocates; levels with 3 x nodes > ALAMO_NOALLOC_MAX_DOUBLES are rejected
+ * (ALAMO_ERR_ARG).  All calls on one GPU share its scratch: calls that overlap on one GPU (other
+ * streams, other threads) race. */
+int alamo_mg_fsmooth_noalloc(alamo_mg* h, int amrlev, int mglev, double* x, const double* b, void* stream);
 /* Elastic::Error0x (src/Operator/Elastic.cpp:382-404): R0x = x - A (x / D) on level mglev. */
 int alamo_mg_error0x(alamo_mg* h, int amrlev, int mglev, double* R0x, const double* x, void* stream);
 /* solutionResidual / correctionResidual (Operator.cpp:635-657): r = b - A x. */
