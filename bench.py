@@ -34,29 +34,33 @@ def parse():
     p.add_argument("--corr", type=float, default=8.0)
     p.add_argument("--contrast", type=float, default=1000.0)
     p.add_argument("--seed", type=int, default=256)
-    p.add_argument("--width", type=float, default=0.5,
+    p.add_argument("--width", type=float, default=0.7,
                    help="interface parameter w of eta = (1 + erf(g / w)) / 2 (0: thresholded)")
     p.add_argument("--krylov-smoother", default="l1", choices=["jacobi", "l1"],
                    help="smoother of the FGMRES V-cycle (plain multigrid methods: 'jacobi' unless the method "
                         "name ends in -l1)")
-    p.add_argument("--mg-max-iter", type=int, default=300, help="cycle limit of the plain multigrid methods")
+    p.add_argument("--mg-max-iter", type=int, default=1500, help="cycle / iteration limit of the solves")
     p.add_argument("--fapply-reps", type=int, default=50)
     p.add_argument("--restart", type=int, default=16, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
-    p.add_argument("--krylov-sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--krylov-sweeps", type=int, default=4, help="smoothing sweeps per Fsmooth call in the FGMRES V-cycle")
     p.add_argument("--krylov-sweeps-finest", type=int, default=1,
                    help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --krylov-sweeps)")
     p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES V-cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
-    p.add_argument("--methods", default="fp64v,fp64v-l1,mixedv-l1,fgmres-fp64v,fgmres-mixedv,fgmres-learned-mixedv",
+    p.add_argument("--coarse-op", default="restrict", choices=["restrict", "galerkin"],
+                   help="coarse-level operators of the FGMRES V-cycle (methods ending in -gal: galerkin)")
+    p.add_argument("--methods",
+                   default="fp64v,fp64v-l1,mixedv-l1,fgmres-fp64v,fgmres-mixedv,fgmres-mixedv-gal,fgmres-learned-mixedv",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
                         "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps), the "
                         "reference's damped point Jacobi, or the l1 smoother with the suffix -l1; "
                         "fgmres-*: flexible GMRES(--restart) preconditioned by one V-cycle with --krylov-smooth "
                         "Fsmooth calls of --krylov-sweeps Jacobi sweeps each side; "
-                        "fgmres-learned-*: the same with the learned correction stage after each cycle")
+                        "fgmres-learned-*: the same with the learned correction stage after each cycle; "
+                        "the suffix -gal selects Galerkin coarse operators")
     p.add_argument("--train-n", type=int, default=128, help="cells per direction of the training fields")
     p.add_argument("--train-fields", type=int, default=2)
     p.add_argument("--train-pairs", type=int, default=8, help="(residual, error) pairs per training field")
@@ -124,13 +128,13 @@ def main():
     bc = (pr["bct"], pr["bcv"])
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
 
-    def make_op(prec, vcyc, krylov, smoother):
+    def make_op(prec, vcyc, krylov, smoother, coarse_op="restrict"):
         nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
         return Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
                        pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
                        sweeps_finest=a.krylov_sweeps_finest if krylov else 0, smoother=smoother,
-                       max_iter=a.mg_max_iter, verbose=a.verbose,
-                       bottom_solver=a.krylov_bottom if krylov else "bicgstab")
+                       max_iter=a.mg_max_iter, verbose=a.verbose, coarse_op=coarse_op,
+                       bottom_solver=a.krylov_bottom if krylov or coarse_op == "galerkin" else "bicgstab")
 
     results = {}
     sols = {}
@@ -143,15 +147,20 @@ def main():
         learned = base.startswith("learned-")
         if learned:
             base = base[len("learned-"):]
+        gal = base.endswith("-gal")
+        if gal:
+            base = base[:-4]
         l1 = base.endswith("-l1")
         if l1:
             base = base[:-3]
         smoother = a.krylov_smoother if krylov else ("l1" if l1 else "jacobi")
+        coarse_op = "galerkin" if gal else (a.coarse_op if krylov else "restrict")
         prec = PREC_MIXED if base.startswith("mixed") else PREC_FP64
         vcyc = base.endswith("v")
         if learned and net is None:
-            net, train_info = train_network(a, P, Elastic, Net, prec, CYCLE_V if vcyc else CYCLE_F, smoother)
-        op = make_op(prec, vcyc, krylov, smoother)
+            net, train_info = train_network(a, P, Elastic, Net, prec, CYCLE_V if vcyc else CYCLE_F, smoother,
+                                            coarse_op)
+        op = make_op(prec, vcyc, krylov, smoother, coarse_op)
         op.SetModel(pr["C"])
         op.prepareForSolve()
         if rhs is None:
@@ -178,7 +187,7 @@ def main():
             ok = st["converged"] == 1
             if not ok:
                 break
-        rec = dict(precision="f64" if prec == PREC_FP64 else "f32-in-f64", smoother=smoother)
+        rec = dict(precision="f64" if prec == PREC_FP64 else "f32-in-f64", smoother=smoother, coarse_op=coarse_op)
         if ok:
             stats = []
             torch.cuda.synchronize()
@@ -321,10 +330,10 @@ def measure_fapply(a, op, N, dev):
 
 
 def reference_solve(a, make_op, C, rhs, N, dev):
-    """FP64 FGMRES with the l1 V-cycle to |b - A u|_inf <= 1e-13 |b|_inf (not timed)"""
+    """FP64 FGMRES with the l1 Galerkin V-cycle to |b - A u|_inf <= 1e-13 |b|_inf (not timed)"""
     import torch
     from alamo_amd.operator import Krylov, PREC_FP64, PRECOND_MG
-    op = make_op(PREC_FP64, True, True, "l1")
+    op = make_op(PREC_FP64, True, True, "l1", "galerkin")
     op.SetModel(C)
     sol = torch.zeros(3 * N, dtype=torch.float64, device=dev)
     st = Krylov(op, restart=a.restart, precond=PRECOND_MG, max_iter=2000).solve(sol, rhs, 1e-13, 0.0,
@@ -364,7 +373,7 @@ def layer_report(ms, n):
     return out
 
 
-def train_network(a, P, Elastic, Net, precision, cycle, smoother):
+def train_network(a, P, Elastic, Net, precision, cycle, smoother, coarse_op):
     """Train the correction network on random two-phase fields of train_n^3 cells (seeds other
     than the benchmark field's, the same interface width), outside the timed region."""
     t0 = time.time()
@@ -377,7 +386,8 @@ def train_network(a, P, Elastic, Net, precision, cycle, smoother):
         nsm = a.krylov_smooth if vc else 2
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
                      pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps,
-                     sweeps_finest=a.krylov_sweeps_finest, smoother=smoother, bottom_solver=a.krylov_bottom)
+                     sweeps_finest=a.krylov_sweeps_finest, smoother=smoother, bottom_solver=a.krylov_bottom,
+                     coarse_op=coarse_op)
         op.SetModel(pr["C"])
         stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
                                solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))
@@ -389,42 +399,73 @@ def train_network(a, P, Elastic, Net, precision, cycle, smoother):
     return net, info
 
 
-def cpu_baseline(pr, rhs, gpu_iters, cycle=1):
-    """The oracle's FP64 multigrid (the same cycle as the GPU method it is set beside) on the host
-    cores: one setup plus one cycle at the full size, timed, and scaled to the GPU solve's
-    iteration count."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pr, rhs, a, results):
+    """The value method's algorithm (FGMRES(restart) right-preconditioned by the same V-cycle, l1
+    smoother, BiCGStab bottom to 1e-10) on the oracle, FP64, on the host cores:
+
+    * at the full 257^3 size: setup (hierarchy, diagonals) and one iteration timed, scaled to the
+      GPU method's iteration count, with every host thread and with one;
+    * a full solve of the same field type at 129^3 nodes (same correlation length, width and
+      seed), timed end to end with every host thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes
+    import numpy as np
     import oracle_ctypes as ora
     from alamo_amd import problems as P
-    import ctypes
     gomp = ctypes.CDLL("libgomp.so.1")  # the OpenMP runtime the oracle is linked against
     cores = int(gomp.omp_get_max_threads())
+    best = min((k for k in results if results[k]["converged"]), key=lambda k: results[k]["seconds"])
+    gpu_its = results[best]["iterations"]
+    kw = dict(restart=a.restart, nu1=a.krylov_smooth, nu2=a.krylov_smooth, sweeps=a.krylov_sweeps,
+              sweeps_finest=a.krylov_sweeps_finest, smoother=1 if a.krylov_smoother == "l1" else 0)
 
-    def timed():
+    def timed(p, b, max_iter):
         t0 = time.perf_counter()
-        ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=0)
-        t_setup = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-10, max_iter=1, cycle=cycle,
-                     nu1=1 if cycle == 0 else 2, nu2=1 if cycle == 0 else 2)
-        t_one = time.perf_counter() - t0
-        per_cycle = max(t_one - t_setup, 1e-9)
-        return t_setup + gpu_iters * per_cycle, t_setup, per_cycle
+        x, it, fr = ora.krylov_solve(p["ncell"], p["dx"], P.ISO, p["C"], p["bct"], b, tol_rel=1e-10,
+                                     max_iter=max_iter, **kw)
+        return time.perf_counter() - t0, it, fr
 
-    def sample(ts, tc, nthr):
-        return ("oracle FP64 %s-cycle multigrid on the full 257^3 field, %d thread(s): setup+residual (%.2fs) and "
-                "one cycle (%.2fs) timed, scaled to the GPU solve's %d cycles"
-                % ("V" if cycle == 0 else "F", nthr, ts, tc, gpu_iters))
+    def extrapolated(nthr):
+        t_setup, _, _ = timed(pr, rhs, 0)
+        print("cpu baseline: %d thread(s), setup %.1fs" % (nthr, t_setup), file=sys.stderr, flush=True)
+        t_one, _, _ = timed(pr, rhs, 1)
+        per_it = max(t_one - t_setup, 1e-9)
+        print("cpu baseline: %d thread(s), one iteration %.1fs" % (nthr, per_it), file=sys.stderr, flush=True)
+        return {"value": t_setup + gpu_its * per_it, "unit": "s", "cores": nthr,
+                "sample": "oracle FP64 FGMRES(%d) + l1 V-cycle on the full %d^3-node field, %d thread(s): setup+"
+                          "residual (%.2fs) and one iteration (%.2fs) timed, scaled to the GPU method's %d "
+                          "iterations" % (a.restart, a.n + 1, nthr, t_setup, per_it, gpu_its)}
 
-    est, ts, tc = timed()
-    out = {"value": est, "unit": "s", "cores": cores, "kind": "port", "sample": sample(ts, tc, cores)}
-    # the reference's own loops run one rank per core without threads: the same sample on one core
+    out = extrapolated(cores)
+    out.update(kind="port", cpu_model=cpu_model(), method=best,
+               note="the oracle runs the value method's algorithm in FP64 (the GPU method: %s)" % results[best][
+                   "precision"])
+    # a full solve at 129^3 nodes
+    nh = a.n // 2
+    ph = P.two_phase_problem(n=nh, corr_cells=a.corr, contrast=a.contrast, seed=a.seed, width=a.width)
+    Nh = (nh + 1) ** 3
+    bh = ora.newton_rhs(ph["ncell"], ph["dx"], P.ISO, ph["C"], ph["F0"], ph["bct"], np.zeros(3 * Nh),
+                        ora.bc_init(ph["ncell"], ph["bct"], ph["bcv"]))
+    t, it, fr = timed(ph, bh, 1000)
+    out["full_solve"] = {"nodes": "%d^3" % (nh + 1), "seconds": t, "iterations": abs(it), "converged": it > 0,
+                         "final_rel_res": fr / float(np.abs(bh).max()), "cores": cores}
+    print("cpu baseline: %d^3 full solve %.1fs, %d iterations" % (nh + 1, t, abs(it)), file=sys.stderr, flush=True)
+    # the reference's own loops run one rank per core without threads: the 257^3 sample on one core
     gomp.omp_set_num_threads(1)
     try:
-        est1, ts1, tc1 = timed()
+        out["single_core"] = extrapolated(1)
     finally:
         gomp.omp_set_num_threads(cores)
-    out["single_core"] = {"value": est1, "unit": "s", "cores": 1, "sample": sample(ts1, tc1, 1)}
     return out
 
 
