@@ -295,6 +295,24 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
             const T dm[3] = {mx, (sc[4][rn] - sc[4][rs]) * hy, (qn.mu - qm.mu) * hz};
             const T G[3][3] = {{ux[0], uy[0], uz[0]}, {ux[1], uy[1], uz[1]}, {ux[2], uy[2], uz[2]}};
             const T tr = G[0][0] + G[1][1] + G[2][2];
+            // l1 smoother: d_v = sign(D) (|D| + the row's off-centre absolute sum).  The row's
+            // coefficients: faces of u_v along a != v: mu / h_a^2 -+ d_a(mu) / 2h_a; faces of u_v
+            // along v: (lambda + 2 mu) / h_v^2 -+ (d_v(lambda) + 2 d_v(mu)) / 2h_v; faces of u_a along
+            // a: -+ d_v(lambda) / 2h_a; faces of u_a along v: -+ d_a(mu) / 2h_v; four corners of u_a in
+            // the (v, a) plane: (lambda + mu) / 4 h_v h_a.  Using |p + q| + |p - q| = 2 max(|p|, |q|),
+            // the parts shared by the three rows are formed once per node.
+            const T ih[3] = {idx, idy, idz}, hh[3] = {hx, hy, hz}, w2[3] = {ix2, iy2, iz2};
+            T Mf[3], am[3], SM = T(0), SA = T(0), SIH = T(0);
+            if (l1) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    am[a] = fabs(dm[a]);
+                    Mf[a] = T(2) * fmax(mu * w2[a], am[a] * hh[a]);
+                    SM += Mf[a];
+                    SA += am[a];
+                    SIH += ih[a];
+                }
+            }
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 T f = mu * (lap[v] + gd[v]) + lam * gd[v];
@@ -303,21 +321,12 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
                 if (MODE == APPLY) stb(O[v], mcb, f);
                 else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
                 else {
-                    const T w2[3] = {ix2, iy2, iz2};
                     const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
                     if (l1) {
-                        // l1 smoother: sign(D) (|D| + sum of the row's off-centre |a|), from the row's
-                        // coefficients: faces of u_v mu / h^2 -+ d(mu) / 2h ((lambda + 2 mu) / h^2 -+
-                        // (d lambda + 2 d mu) / 2h along v), faces of u_a -+ d(lambda)_v / 2h_a (along
-                        // a) and -+ d(mu)_a / 2h_v (along v), corners (lambda + mu) / 4 h_v h_a
-                        const T ih[3] = {idx, idy, idz}, hh[3] = {hx, hy, hz};
-                        T S = T(2) * fmax((lam + T(2) * mu) * w2[v], fabs(dl[v] + T(2) * dm[v]) * hh[v]);
-#pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            if (a == v) continue;
-                            S += T(2) * fmax(mu * w2[a], fabs(dm[a]) * hh[a]) + fabs(dl[v]) * ih[a] +
-                                 fabs(dm[a]) * ih[v] + fabs(lam + mu) * ih[v] * ih[a];
-                        }
+                        const T io = SIH - ih[v];  // sum of 1/h_a over a != v
+                        const T S = T(2) * fmax((lam + T(2) * mu) * w2[v], fabs(dl[v] + T(2) * dm[v]) * hh[v]) +
+                                    (SM - Mf[v]) + fabs(dl[v]) * io + ih[v] * (SA - am[v]) +
+                                    fabs(lam + mu) * ih[v] * io;
                         const T Dl = D < T(0) ? D - S : D + S;
                         stb(O[v], mcb, qc.u[v] + (bq[v] - f) / Dl);
                     } else {

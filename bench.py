@@ -40,7 +40,7 @@ def parse():
                    help="smoother of the FGMRES V-cycle (plain multigrid methods: 'jacobi' unless the method "
                         "name ends in -l1)")
     p.add_argument("--mg-max-iter", type=int, default=1500, help="cycle / iteration limit of the solves")
-    p.add_argument("--fapply-reps", type=int, default=50)
+    p.add_argument("--fapply-reps", type=int, default=100)
     p.add_argument("--restart", type=int, default=16, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
@@ -300,7 +300,9 @@ def measure_fapply(a, op, N, dev):
     import torch
     u = torch.rand(3 * N, dtype=torch.float64, device=dev)
     f = torch.empty_like(u)
-    op.Fapply(0, 0, f, u)
+    # warm-up launches first: the first launches after other work run ~15% slower (clocks, caches)
+    for _ in range(20):
+        op.Fapply(0, 0, f, u)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -314,7 +316,8 @@ def measure_fapply(a, op, N, dev):
     # Fsmooth (level 0, FP64): fused sweeps, (3 x + 3 b + 2 C + 3 x') x 8 B per node and sweep (the
     # point diagonal is recomputed from lambda, mu; the l1 diagonal from them and their differences)
     b = torch.rand_like(u)
-    op.Fsmooth(0, 0, f, b)
+    for _ in range(3):
+        op.Fsmooth(0, 0, f, b)
     torch.cuda.synchronize()
     reps = max(1, a.fapply_reps // 5)
     e0.record()
