@@ -122,6 +122,11 @@ SIGNATURES = {
     "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_amr_create": (c_int, [P(MGDesc), c_int, c_void_p, c_void_p, c_void_p, P(c_void_p)]),
+    "alamo_amr_destroy": (c_int, [c_void_p]),
+    "alamo_amr_level": (c_int, [c_void_p, c_int, P(c_void_p)]),
+    "alamo_amr_masks": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "alamo_amr_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(SolveStats)]),
     "alamo_net_layer_times": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
 }
 

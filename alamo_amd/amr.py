@@ -1,6 +1,7 @@
-"""Two-level AMR helpers of ``Operator<Grid::Node>`` (src/Operator/Operator.cpp) over the C ABI.
+"""AMR over the C ABI: the two-level helpers of ``Operator<Grid::Node>`` (src/Operator/Operator.cpp)
+and the composite multi-level solve (``Composite``).
 
-Fine boxes are inclusive cell boxes in fine index space (refinement ratio 2, lo even, hi odd);
+For the helpers, fine boxes are inclusive cell boxes in fine index space (refinement ratio 2, lo even, hi odd);
 fine fields live on the box's nodal box grown by one node, ``[lo - 1, hi + 2]^3``, 3 SoA planes.
 Fields are torch tensors on the GPU (memory only); masks are int32 tensors of the coarse nodes.
 """
@@ -9,6 +10,7 @@ import ctypes
 import numpy as np
 
 from ._lib import check, dev_ptr, lib
+from .problems import ISO
 
 CRSE_NODE, CRSE_FINE_NODE, FINE_NODE = 0, 1, 2  # amrex::nodelap_detail
 
@@ -62,3 +64,75 @@ def fixup_residual_mask(ncell_crse, mask, resmsk, stream=None):
     nc, pnc = _iv(ncell_crse)
     check(lib().alamo_amr_fixup_residual_mask(pnc, dev_ptr(mask, _nodes(nc), "fixup mask", "int32"),
                                               dev_ptr(resmsk, _nodes(nc), "fixup resmsk", "int32"), stream))
+
+
+class Composite:
+    """Solver::Nonlocal::Linear::solve over several AMR levels (Linear.H:131-168 with amrex::MLMG's
+    composite cycle) on an explicit mesh (Integrator.cpp:121-139: explicitmesh.lo<l> / hi<l>).
+
+    ``boxes[l - 1]`` lists the inclusive cell boxes ``(lo, hi)`` of AMR level l >= 1 in that level's
+    own index space (refinement ratio 2).  Every level is a full-domain field of 3 SoA planes of
+    ``(n_cell 2^l + 1)^3`` nodes; ``Level(l)`` is level l's ``Elastic`` operator (SetModel, BCInit,
+    Fapply, Stress ... on it), ``solve`` runs the composite V-cycles.  Options are ``Elastic``'s."""
+
+    def __init__(self, ncell, dx, boxes, sym=ISO, bc=None, **opts):
+        from .operator import Elastic, make_desc
+        self.desc = make_desc(ncell, dx, sym, bc, **opts)
+        self.nlevels = len(boxes) + 1
+        nbox = np.array([len(bl) for bl in boxes], dtype=np.int32)
+        lo = np.ascontiguousarray(np.array([b[0] for bl in boxes for b in bl], dtype=np.int32).reshape(-1))
+        hi = np.ascontiguousarray(np.array([b[1] for bl in boxes for b in bl], dtype=np.int32).reshape(-1))
+        if lo.size == 0:
+            lo = hi = np.zeros(3, dtype=np.int32)
+        self.boxes = [[(tuple(b[0]), tuple(b[1])) for b in bl] for bl in boxes]
+        h = ctypes.c_void_p()
+        check(lib().alamo_amr_create(ctypes.byref(self.desc), self.nlevels, nbox.ctypes.data_as(ctypes.c_void_p),
+                                     lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.byref(h)))
+        self.h = h
+        self.levels = []
+        for l in range(self.nlevels):
+            hl = ctypes.c_void_p()
+            check(lib().alamo_amr_level(h, l, ctypes.byref(hl)))
+            self.levels.append(Elastic._view(hl, self, [int(n) << l for n in ncell], [float(x) / (1 << l) for x in dx],
+                                             sym, self.desc))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                lib().alamo_amr_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def Level(self, amrlev):
+        return self.levels[amrlev]
+
+    def Nodes(self, amrlev):
+        return _nodes(self.levels[amrlev].ncell)
+
+    def Masks(self, amrlev, region=None, fine_mask=None, stream=None):
+        """the level's region mask (2 inside, 1 on the boundary, 0 outside) and its coverage by the
+        next level (the nodal fine mask: 2 fine, 1 coarse/fine, 0 coarse), int32 device tensors"""
+        n = self.Nodes(amrlev)
+        check(lib().alamo_amr_masks(self.h, amrlev,
+                                    None if region is None else dev_ptr(region, n, "Masks region", "int32"),
+                                    None if fine_mask is None else dev_ptr(fine_mask, n, "Masks fine_mask", "int32"),
+                                    stream))
+
+    def solve(self, sols, rhss, tol_rel, tol_abs, raise_on_fail=True):
+        """composite solve: sols[l] initial guess in, solution out; rhss[l] right-hand sides (BCInit
+        values in level 0's boundary nodes).  Returns (final |r|_inf, SolveStats)."""
+        from ._lib import SolveStats, AlamoError
+        if len(sols) != self.nlevels or len(rhss) != self.nlevels:
+            raise AlamoError(1, "need one solution and one right-hand side per AMR level")
+        ps = (ctypes.c_void_p * self.nlevels)(*[dev_ptr(sols[l], 3 * self.Nodes(l), "solve sol[%d]" % l)
+                                                for l in range(self.nlevels)])
+        pb = (ctypes.c_void_p * self.nlevels)(*[dev_ptr(rhss[l], 3 * self.Nodes(l), "solve rhs[%d]" % l)
+                                                for l in range(self.nlevels)])
+        st = SolveStats()
+        rc = lib().alamo_amr_solve(self.h, ps, pb, float(tol_rel), float(tol_abs), ctypes.byref(st))
+        if raise_on_fail:
+            check(rc)
+        return st.final_res_inf, st

@@ -120,5 +120,58 @@ __global__ void k_fixup_resmask(int64_t n, const int* __restrict__ mask, int* __
     if (q < n && mask[q] == 1) resmsk[q] = 1;
 }
 
+
+// ---------------------------------------------------------------------------
+// Composite multi-level solve (alamo_amr_*): every AMR level is one full-domain nodal array at
+// its own resolution, its region given by a node mask (k_fine_mask of the level's own boxes:
+// 2 region interior, 1 region boundary, 0 outside) and its coverage by the next level by a
+// second mask (k_fine_mask of the next level's boxes, coarsened).  The kernels below move fields
+// between the levels under those masks; mask arrays hold one int per node, fields 3 planes.
+// ---------------------------------------------------------------------------
+// dst = src on the nodes whose mask lies in [lo, hi] (3 planes of n nodes)
+__global__ void k_masked_copy(int64_t n, const int* __restrict__ mask, int lo, int hi, double* __restrict__ dst,
+                              const double* __restrict__ src) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int m = mask[q];
+    if (m < lo || m > hi) return;
+    for (int c = 0; c < 3; c++) dst[c * n + q] = src[c * n + q];
+}
+// x = 0 on the nodes whose mask lies outside [lo, hi]
+__global__ void k_masked_zero(int64_t n, const int* __restrict__ mask, int lo, int hi, double* __restrict__ x) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int m = mask[q];
+    if (m >= lo && m <= hi) return;
+    for (int c = 0; c < 3; c++) x[c * n + q] = 0.0;
+}
+// nodal injection crse(I) = fine(2 I) on the coarse nodes the next level covers (fine mask >= 1):
+// averageDownSolutionRHS (Operator.cpp:435-447) with full-domain level arrays
+__global__ void k_inject_covered(int64_t nc, int Nc0, int Nc1, int Nf0, int Nf1, int64_t nf,
+                                 const int* __restrict__ fmask, double* __restrict__ crse,
+                                 const double* __restrict__ fine) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nc || fmask[q] < 1) return;
+    const int I = (int)(q % Nc0), J = (int)((q / Nc0) % Nc1), K = (int)(q / ((int64_t)Nc0 * Nc1));
+    const int64_t mf = 2 * I + (int64_t)Nf0 * (2 * J + (int64_t)Nf1 * 2 * K);
+    for (int c = 0; c < 3; c++) crse[c * nc + q] = fine[c * nf + mf];
+}
+
+// out(I) = fine(2 I) on the coarse nodes strictly covered (fine mask 2), zero elsewhere: the finer
+// correction as the coarse rows at the coarse/fine interface read it
+__global__ void k_inject_strict(int64_t nc, int Nc0, int Nc1, int Nf0, int Nf1, int64_t nf,
+                                const int* __restrict__ fmask, double* __restrict__ out,
+                                const double* __restrict__ fine) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nc) return;
+    if (fmask[q] != 2) {
+        for (int c = 0; c < 3; c++) out[c * nc + q] = 0.0;
+        return;
+    }
+    const int I = (int)(q % Nc0), J = (int)((q / Nc0) % Nc1), K = (int)(q / ((int64_t)Nc0 * Nc1));
+    const int64_t mf = 2 * I + (int64_t)Nf0 * (2 * J + (int64_t)Nf1 * 2 * K);
+    for (int c = 0; c < 3; c++) out[c * nc + q] = fine[c * nf + mf];
+}
+
 }  // namespace amr
 }  // namespace alamo

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1187,7 +1188,8 @@ int alamo_mg_create(const alamo_mg_desc* d, alamo_mg** out) {
     }
     rc |= dalloc(h, &h->partial, 2 * h->nblk_red);
     rc |= dalloc(h, &h->red, 4);
-    rc |= dalloc(h, &h->bottom_work, 8 * 3 * gs.back().tot);
+    // BiCGStab bottom workspace: only when the coarsest level is small enough for a solve (<= 17^3)
+    if (gs.back().tot <= 17 * 17 * 17) rc |= dalloc(h, &h->bottom_work, 8 * 3 * gs.back().tot);
     rc |= dalloc(h, &h->bottom_iters, 1);
     if (mixed) rc |= dalloc(h, &h->sol_res, 3 * gs[0].tot);
     if (rc == ALAMO_OK && hipHostMalloc((void**)&h->red_host, (KRY_MAXV + 4) * sizeof(double)) != hipSuccess) {
@@ -2238,6 +2240,365 @@ int alamo_amr_fixup_residual_mask(const int* ncell_crse, const int* mask, int* r
     if (!mask || !resmsk) FAIL(ALAMO_ERR_ARG, "null argument");
     amr::k_fixup_resmask<<<nblocks(d.tot), 256, 0, (hipStream_t)stream>>>(d.tot, mask, resmsk);
     HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Composite multi-level (AMR) solve.  Design: every AMR level is a full-domain alamo_mg handle at
+// its own resolution (HBM is plentiful, and no ghost-exchange plan is needed: a level's region
+// and the nodes around it live in one array); the region of level l >= 1 is the union of its
+// boxes, given by a node mask, and its coverage by level l + 1 by a second mask (the nodal fine
+// mask of amrex::MLNodeLinOp::buildMasks, read at Operator.cpp:236,553).
+//
+// The composite system (FP64):
+//   * unknowns: level-0 nodes and the nodes strictly inside each fine region (region mask 2);
+//     fine nodes on or outside the region boundary (mask 1, 0) follow the coarser level,
+//     u_l = P u_{l-1} (the nodal prolongation of Operator.cpp:365-434, the reference's ghost-node
+//     coarse/fine strategy, Linear.H:259); coarse nodes covered by level l + 1 (fine mask 1, 2)
+//     hold the coincident fine values (averageDownSolutionRHS, Operator.cpp:435-447);
+//   * level l rows (Elastic::Fapply with its BC rows) on level 0 and on each fine region's
+//     interior; a coarse/fine interface node (fine mask 1) keeps its coarse row, which reads the
+//     fine values of the covered nodes next to it;
+//   * strictly covered coarse nodes (fine mask 2): the full-weighting restriction of level l + 1's
+//     residual (Operator<Grid::Node>::reflux, Operator.cpp:526-633, whose valid nodes all take the
+//     27-point interior stencil) -- the coarse-grid-correction equation of the fine level.
+// Solver: composite V-cycles (amrex::MLMG's multi-level cycle): down from the finest level,
+// masked smoothing of each level's correction and reflux of its residual, the level-0 multigrid
+// V-cycle, up again with prolonged corrections and smoothing; stop on the max norm of the
+// composite residual (Linear.H's tolerances).  Fine regions must lie inside the domain with one
+// coarse cell of margin and nest properly in the next coarser level's region.
+// ---------------------------------------------------------------------------
+struct alamo_amr {
+    int nl = 0;
+    std::vector<alamo_mg*> H;
+    std::vector<int*> region;  // level l >= 1 (level 0: null): 2 interior, 1 boundary, 0 outside
+    std::vector<int*> fmask;   // level l < nl - 1: coverage by level l + 1 (2, 1, 0)
+    std::vector<double*> e, r, c, rr, t;
+    double* e0 = nullptr;
+    std::vector<void*> allocs;
+};
+
+static int amr_alloc(alamo_amr* a, double** p, int64_t n) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, std::max<int64_t>(n, 1) * sizeof(double)));
+    HIPCHK(hipMemset(q, 0, std::max<int64_t>(n, 1) * sizeof(double)));
+    a->allocs.push_back(q);
+    *p = (double*)q;
+    return ALAMO_OK;
+}
+
+// e <- masked sweeps of level l's smoother on e with right-hand side c: only region-interior
+// nodes change (the rest keeps its value: zero in the down sweep, the prolonged coarse
+// correction in the up sweep)
+static int amr_smooth(alamo_amr* a, int l, double* e, const double* c, int nsweeps) {
+    alamo_mg* h = a->H[l];
+    const Geom& g = h->L[0].g;
+    for (int s = 0; s < nsweeps; s++) {
+        level_op<double>(h, 0, JACOBI, e, c, a->t[l], h->d.omega, h->stream);
+        amr::k_masked_copy<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->region[l], 2, 2, e, a->t[l]);
+    }
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+// covered coarse nodes (fine mask 1, 2) take the coincident fine values, finest level first
+static void amr_inject(alamo_amr* a, double* const* u) {
+    for (int l = a->nl - 2; l >= 0; l--) {
+        const Geom& g = a->H[l]->L[0].g;
+        const Geom& gf = a->H[l + 1]->L[0].g;
+        amr::k_inject_covered<<<nblocks(g.tot), 256, 0, a->H[l]->stream>>>(g.tot, g.N[0], g.N[1], gf.N[0], gf.N[1],
+                                                                          gf.tot, a->fmask[l], u[l], u[l + 1]);
+    }
+}
+
+// covered coarse nodes first take the fine values; then r_l = b_l - A_l u_l on level 0 and on each
+// fine region's interior (zero elsewhere), the strictly covered nodes replaced by the restriction
+// of r_{l+1} (reflux)
+static int amr_residual(alamo_amr* a, double* const* u, const double* const* b) {
+    amr_inject(a, u);
+    for (int l = a->nl - 1; l >= 0; l--) {
+        alamo_mg* h = a->H[l];
+        const Geom& g = h->L[0].g;
+        level_op<double>(h, 0, RESID, u[l], b[l], a->r[l], 0.0, h->stream);
+        if (l > 0) amr::k_masked_zero<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->region[l], 2, 2, a->r[l]);
+        if (l < a->nl - 1) {
+            const Geom& gf = a->H[l + 1]->L[0].g;
+            k_restrict<double><<<nblocks(g.tot), 256, 0, h->stream>>>(g, gf, 3, a->r[l + 1], a->t[l]);
+            amr::k_masked_copy<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->fmask[l], 2, 2, a->r[l], a->t[l]);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+// one composite V-cycle (correction scheme) on the residuals a->r; corrections added to u
+static int amr_vcycle(alamo_amr* a, double* const* u) {
+    const int L = a->nl - 1;
+    alamo_mg* h0 = a->H[0];
+    const int nsw = h0->d.pre_smooth * h0->d.sweeps_per_smooth;
+    for (int l = L; l >= 0; l--) {
+        alamo_mg* h = a->H[l];
+        const Geom& g = h->L[0].g;
+        HIPCHK(hipMemcpyAsync(a->c[l], a->r[l], 3 * g.tot * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        if (l < L) {
+            // strictly covered nodes: the restricted residual of the finer correction equation;
+            // coarse/fine nodes: their coarse rows less the effect of the finer correction injected
+            // at the strictly covered nodes (rr[l] serves as scratch: it is written after smoothing)
+            const Geom& gf = a->H[l + 1]->L[0].g;
+            k_restrict<double><<<nblocks(g.tot), 256, 0, h->stream>>>(g, gf, 3, a->rr[l + 1], a->t[l]);
+            amr::k_masked_copy<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->fmask[l], 2, 2, a->c[l], a->t[l]);
+            amr::k_inject_strict<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, g.N[0], g.N[1], gf.N[0], gf.N[1], gf.tot,
+                                                                       a->fmask[l], a->rr[l], a->e[l + 1]);
+            level_op<double>(h, 0, RESID, a->rr[l], a->c[l], a->t[l], 0.0, h->stream);
+            amr::k_masked_copy<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->fmask[l], 1, 1, a->c[l], a->t[l]);
+        }
+        if (l == 0) break;
+        vfill<double>(a->e[l], 3 * g.tot, 0.0, h->stream);
+        TRY(amr_smooth(a, l, a->e[l], a->c[l], nsw));
+        level_op<double>(h, 0, RESID, a->e[l], a->c[l], a->rr[l], 0.0, h->stream);
+        amr::k_masked_zero<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->region[l], 2, 2, a->rr[l]);
+    }
+    TRY(precond_apply(h0, ALAMO_PRECOND_MG, a->c[0], a->e[0]));
+    for (int l = 1; l <= L; l++) {
+        alamo_mg* h = a->H[l];
+        const Geom& gc = a->H[l - 1]->L[0].g;
+        const Geom& g = h->L[0].g;
+        k_interp_add<double><<<interp_blocks(gc, g), 256, 0, h->stream>>>(gc, g, a->e[l - 1], a->e[l]);
+        TRY(amr_smooth(a, l, a->e[l], a->c[l], h0->d.post_smooth * h0->d.sweeps_per_smooth));
+    }
+    for (int l = 0; l <= L; l++) {
+        const int64_t n = 3 * a->H[l]->L[0].g.tot;
+        k_axpy<double, double><<<gs_blocks(n), 256, 0, a->H[l]->stream>>>(u[l], a->e[l], n, 1.0);
+    }
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+static int amr_norm(alamo_amr* a, double* const* v, double* ninf) {
+    double m = 0.0;
+    for (int l = 0; l < a->nl; l++) {
+        double x = 0.0;
+        TRY(norms_dev<double>(a->H[l], v[l], 3 * a->H[l]->L[0].g.tot, &x, nullptr));
+        if (!(x <= m)) m = x;  // a NaN propagates
+    }
+    *ninf = m;
+    return ALAMO_OK;
+}
+
+extern "C" {
+
+int alamo_amr_create(const alamo_mg_desc* base, int nlevels, const int* nbox, const int* box_lo, const int* box_hi,
+                     alamo_amr** out) {
+    if (!base || !out) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (nlevels < 1 || nlevels > 6) FAIL(ALAMO_ERR_ARG, "nlevels must be in [1, 6]");
+    if (nlevels > 1 && (!nbox || !box_lo || !box_hi)) FAIL(ALAMO_ERR_ARG, "null box list");
+    // boxes per level (levels >= 1), inclusive cell boxes in the level's own index space
+    std::vector<std::vector<std::array<int, 6>>> B(nlevels);
+    int off = 0;
+    for (int l = 1; l < nlevels; l++) {
+        if (nbox[l - 1] < 1 || nbox[l - 1] > amr::MAX_BOXES) FAIL(ALAMO_ERR_ARG, "each fine level needs 1 to 64 boxes");
+        const int f = 1 << l;
+        for (int t = 0; t < nbox[l - 1]; t++, off++) {
+            std::array<int, 6> bx{};
+            for (int q = 0; q < 3; q++) {
+                bx[q] = box_lo[3 * off + q];
+                bx[3 + q] = box_hi[3 * off + q];
+                if (bx[q] % 2 != 0 || bx[3 + q] % 2 != 1 || bx[q] > bx[3 + q])
+                    FAIL(ALAMO_ERR_ARG, "fine boxes must cover whole coarse cells (lo even, hi odd, lo <= hi)");
+                // one coarse cell of margin to the domain boundary
+                if (bx[q] < 2 || bx[3 + q] > f * base->n_cell[q] - 3)
+                    FAIL(ALAMO_ERR_ARG, "fine boxes must stay one coarse cell inside the domain");
+            }
+            B[l].push_back(bx);
+        }
+        if (l >= 2) {
+            // proper nesting: every box, coarsened and grown by one cell, lies in the union of the
+            // coarser level's boxes (so the coarse/fine interface nodes carry coarser-level rows)
+            const int64_t n0 = (int64_t)base->n_cell[0] << (l - 1), n1 = (int64_t)base->n_cell[1] << (l - 1);
+            const int64_t n2 = (int64_t)base->n_cell[2] << (l - 1);
+            std::vector<uint8_t> cov((size_t)(n0 * n1 * n2), 0);
+            for (const auto& cb : B[l - 1])
+                for (int64_t k = cb[2]; k <= cb[5]; k++)
+                    for (int64_t j = cb[1]; j <= cb[4]; j++)
+                        for (int64_t i = cb[0]; i <= cb[3]; i++) cov[(size_t)(i + n0 * (j + n1 * k))] = 1;
+            for (const auto& bx : B[l])
+                for (int64_t k = bx[2] / 2 - 1; k <= (bx[5] + 1) / 2; k++)
+                    for (int64_t j = bx[1] / 2 - 1; j <= (bx[4] + 1) / 2; j++)
+                        for (int64_t i = bx[0] / 2 - 1; i <= (bx[3] + 1) / 2; i++)
+                            if (!cov[(size_t)(i + n0 * (j + n1 * k))])
+                                FAIL(ALAMO_ERR_ARG, "each box, coarsened and grown by one cell, must lie in the next "
+                                                    "coarser level's boxes");
+        }
+    }
+    alamo_amr* a = new alamo_amr();
+    a->nl = nlevels;
+    int rc = ALAMO_OK;
+    for (int l = 0; l < nlevels && rc == ALAMO_OK; l++) {
+        alamo_mg_desc d = *base;
+        for (int q = 0; q < 3; q++) {
+            d.n_cell[q] = base->n_cell[q] << l;
+            d.dx[q] = base->dx[q] / (double)(1 << l);
+        }
+        if (l > 0) d.max_coarsening_level = 0;  // fine levels are smoothed, not coarsened
+        alamo_mg* h = nullptr;
+        rc = alamo_mg_create(&d, &h);
+        if (rc != ALAMO_OK) break;
+        a->H.push_back(h);
+    }
+    const std::string e0 = rc == ALAMO_OK ? std::string() : g_err;
+    if (rc == ALAMO_OK) {
+        a->region.assign(nlevels, nullptr);
+        a->fmask.assign(nlevels, nullptr);
+        a->e.assign(nlevels, nullptr);
+        a->r = a->c = a->rr = a->t = a->e;
+        for (int l = 0; l < nlevels && rc == ALAMO_OK; l++) {
+            const Geom& g = a->H[l]->L[0].g;
+            rc |= amr_alloc(a, &a->e[l], 3 * g.tot);
+            rc |= amr_alloc(a, &a->r[l], 3 * g.tot);
+            rc |= amr_alloc(a, &a->c[l], 3 * g.tot);
+            rc |= amr_alloc(a, &a->rr[l], 3 * g.tot);
+            rc |= amr_alloc(a, &a->t[l], 3 * g.tot);
+            amr::Dom dom;
+            dom.nc[0] = g.n[0]; dom.nc[1] = g.n[1]; dom.nc[2] = g.n[2];
+            dom.N0 = g.N[0]; dom.N1 = g.N[1]; dom.tot = g.tot;
+            // the level's own region, and its coverage by the next level
+            for (int which = 0; which < 2 && rc == ALAMO_OK; which++) {
+                const int src = which == 0 ? l : l + 1;
+                if ((which == 0 && l == 0) || src >= nlevels) continue;
+                double* m = nullptr;
+                rc |= amr_alloc(a, &m, (g.tot + 1) / 2);
+                amr::Boxes bx;
+                bx.n = (int)B[src].size();
+                for (int t = 0; t < bx.n; t++)
+                    for (int q = 0; q < 3; q++) {
+                        bx.lo[t][q] = which == 0 ? B[src][t][q] : B[src][t][q] / 2;
+                        bx.hi[t][q] = which == 0 ? B[src][t][3 + q] : (B[src][t][3 + q] + 1) / 2 - 1;
+                    }
+                amr::k_fine_mask<<<nblocks(g.tot), 256>>>(dom, bx, (int*)m);
+                if (which == 0) a->region[l] = (int*)m;
+                else a->fmask[l] = (int*)m;
+            }
+        }
+        if (rc == ALAMO_OK && hipDeviceSynchronize() != hipSuccess) rc = ALAMO_ERR_HIP;
+    }
+    if (rc != ALAMO_OK) {
+        std::string e = e0.empty() ? g_err : e0;
+        alamo_amr_destroy(a);
+        g_err = e;
+        return rc;
+    }
+    *out = a;
+    return ALAMO_OK;
+}
+
+int alamo_amr_destroy(alamo_amr* a) {
+    if (!a) return ALAMO_OK;
+    (void)hipDeviceSynchronize();
+    for (alamo_mg* h : a->H) alamo_mg_destroy(h);
+    for (void* p : a->allocs) (void)hipFree(p);
+    delete a;
+    return ALAMO_OK;
+}
+
+int alamo_amr_level(alamo_amr* a, int amrlev, alamo_mg** h) {
+    if (!a || !h) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (amrlev < 0 || amrlev >= a->nl) FAIL(ALAMO_ERR_ARG, "amrlev out of range");
+    *h = a->H[amrlev];
+    return ALAMO_OK;
+}
+
+int alamo_amr_masks(alamo_amr* a, int amrlev, int* region, int* fine_mask, void* stream) {
+    if (!a) FAIL(ALAMO_ERR_ARG, "null handle");
+    if (amrlev < 0 || amrlev >= a->nl) FAIL(ALAMO_ERR_ARG, "amrlev out of range");
+    const Geom& g = a->H[amrlev]->L[0].g;
+    const hipStream_t s = (hipStream_t)stream;
+    if (region) {
+        if (a->region[amrlev]) HIPCHK(hipMemcpyAsync(region, a->region[amrlev], g.tot * sizeof(int), hipMemcpyDeviceToDevice, s));
+        else k_fill<int><<<gs_blocks(g.tot), 256, 0, s>>>(region, g.tot, 2);
+    }
+    if (fine_mask) {
+        if (a->fmask[amrlev]) HIPCHK(hipMemcpyAsync(fine_mask, a->fmask[amrlev], g.tot * sizeof(int), hipMemcpyDeviceToDevice, s));
+        else k_fill<int><<<gs_blocks(g.tot), 256, 0, s>>>(fine_mask, g.tot, 0);
+    }
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_amr_solve(alamo_amr* a, double* const* sol, const double* const* rhs, double tol_rel, double tol_abs,
+                    alamo_solve_stats* st) {
+    if (!a || !sol || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
+    for (int l = 0; l < a->nl; l++) {
+        if (!sol[l] || !rhs[l]) FAIL(ALAMO_ERR_ARG, "null level field");
+        if (!a->H[l]->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called on every AMR level");
+        if (!a->H[l]->prepared) TRY(prepare(a->H[l]));
+        if (a->H[l]->gal && l > 0) FAIL(ALAMO_ERR_ARG, "coarse_op GALERKIN is for the level-0 hierarchy only");
+    }
+    alamo_mg* h0 = a->H[0];
+    if (h0->L.back().g.tot > 17 * 17 * 17)
+        FAIL(ALAMO_ERR_ARG, "coarsest level has more than 17^3 nodes; use n_cell divisible by a larger power of 2");
+    HIPCHK(hipDeviceSynchronize());
+    // every level's work goes on level 0's stream, in order (restored on return)
+    std::vector<hipStream_t> keep(a->nl);
+    for (int l = 0; l < a->nl; l++) {
+        keep[l] = a->H[l]->stream;
+        a->H[l]->stream = h0->stream;
+    }
+    struct Restore {
+        alamo_amr* a;
+        std::vector<hipStream_t>& k;
+        ~Restore() { for (int l = 0; l < a->nl; l++) a->H[l]->stream = k[l]; }
+    } restore{a, keep};
+    auto t0 = std::chrono::steady_clock::now();
+    // fine-level nodes outside the region interior take the prolonged coarse solution
+    for (int l = 1; l < a->nl; l++) {
+        alamo_mg* h = a->H[l];
+        const Geom& g = h->L[0].g;
+        const Geom& gc = a->H[l - 1]->L[0].g;
+        vfill<double>(a->t[l], 3 * g.tot, 0.0, h->stream);
+        k_interp_add<double><<<interp_blocks(gc, g), 256, 0, h->stream>>>(gc, g, sol[l - 1], a->t[l]);
+        amr::k_masked_copy<<<nblocks(g.tot), 256, 0, h->stream>>>(g.tot, a->region[l], 0, 1, sol[l], a->t[l]);
+    }
+    // |b|: level 0 and the fine levels' region interiors
+    for (int l = 0; l < a->nl; l++) {
+        const Geom& g = a->H[l]->L[0].g;
+        HIPCHK(hipMemcpyAsync(a->c[l], rhs[l], 3 * g.tot * sizeof(double), hipMemcpyDeviceToDevice, a->H[l]->stream));
+        if (l > 0) amr::k_masked_zero<<<nblocks(g.tot), 256, 0, a->H[l]->stream>>>(g.tot, a->region[l], 2, 2, a->c[l]);
+    }
+    double bn = 0.0, r0 = 0.0;
+    TRY(amr_norm(a, a->c.data(), &bn));
+    TRY(amr_residual(a, sol, rhs));
+    TRY(amr_norm(a, a->r.data(), &r0));
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
+    double rn = r0;
+    int it = 0;
+    while (rn > target && it < h0->d.max_iter && std::isfinite(rn)) {
+        TRY(amr_vcycle(a, sol));
+        TRY(amr_residual(a, sol, rhs));
+        TRY(amr_norm(a, a->r.data(), &rn));
+        it++;
+        if (h0->d.verbose) fprintf(stderr, "alamo amr: iter %d |r|inf %.6e target %.3e\n", it, rn, target);
+    }
+    // averageDownSolutionRHS: covered coarse nodes take the fine values
+    amr_inject(a, sol);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    auto t1 = std::chrono::steady_clock::now();
+    const bool conv = rn <= target;
+    if (st) {
+        st->iterations = it;
+        st->converged = conv ? 1 : 0;
+        st->initial_res_inf = r0;
+        st->final_res_inf = rn;
+        st->final_res_l2 = 0.0;
+        st->rhs_inf = bn;
+        st->target = target;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->fallback_events = 0;
+    }
+    if (!std::isfinite(rn)) FAIL(ALAMO_ERR_NAN, "alamo_amr_solve: residual is not finite (diverged)");
+    if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_amr_solve: max_iter reached before the tolerance");
     return ALAMO_OK;
 }
 

@@ -71,38 +71,44 @@ def bottom_solver_code(name):
     return BOTTOM_SOLVERS[name]
 
 
+def make_desc(ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2, post_smooth=2,
+              sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200, bottom_tol_rel=1e-4,
+              max_coarsening_level=-1, verbose=0, smoother="jacobi", fixed_iter=0, bottom_solver="bicgstab",
+              sweeps_finest=0, coarse_op="restrict"):
+    """alamo_mg_desc from the operator's options (defaults: Linear.H:183-244 and Elastic.H)"""
+    d = MGDesc()
+    check(lib().alamo_mg_desc_default(ctypes.byref(d)))
+    for q in range(3):
+        d.n_cell[q] = int(ncell[q])
+        d.dx[q] = float(dx[q])
+    d.sym = int(sym)
+    if bc is not None:
+        t, v = bc
+        t = np.asarray(t, dtype=np.int32).reshape(26, 3)
+        v = np.asarray(v, dtype=np.float64).reshape(26, 3)
+        for f in range(26):
+            for c in range(3):
+                d.bc_type[f][c] = int(t[f, c])
+                d.bc_val[f][c] = float(v[f, c])
+    d.precision, d.cycle = precision, cycle
+    d.pre_smooth, d.post_smooth, d.sweeps_per_smooth = pre_smooth, post_smooth, sweeps_per_smooth
+    d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
+    d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
+    d.smoother = smoother_code(smoother)
+    d.coarse_op = coarse_op_code(coarse_op)
+    d.fixed_iter = int(fixed_iter)
+    d.bottom_solver = bottom_solver_code(bottom_solver)
+    d.sweeps_finest = int(sweeps_finest)
+    return d
+
+
 class Elastic:
     """Operator::Elastic<SYM> on one box covering a non-periodic nodal domain.
 
     ``bc`` is the pair (bc_type[26][3], bc_val[26][3]) of BC::Operator::Elastic::Constant."""
 
-    def __init__(self, ncell, dx, sym=ISO, bc=None, precision=PREC_FP64, cycle=CYCLE_F, pre_smooth=2,
-                 post_smooth=2, sweeps_per_smooth=2, omega=2.0 / 3.0, max_iter=200, bottom_max_iter=200,
-                 bottom_tol_rel=1e-4, max_coarsening_level=-1, verbose=0, smoother="jacobi", fixed_iter=0,
-                 bottom_solver="bicgstab", sweeps_finest=0, coarse_op="restrict"):
-        d = MGDesc()
-        check(lib().alamo_mg_desc_default(ctypes.byref(d)))
-        for q in range(3):
-            d.n_cell[q] = int(ncell[q])
-            d.dx[q] = float(dx[q])
-        d.sym = int(sym)
-        if bc is not None:
-            t, v = bc
-            t = np.asarray(t, dtype=np.int32).reshape(26, 3)
-            v = np.asarray(v, dtype=np.float64).reshape(26, 3)
-            for f in range(26):
-                for c in range(3):
-                    d.bc_type[f][c] = int(t[f, c])
-                    d.bc_val[f][c] = float(v[f, c])
-        d.precision, d.cycle = precision, cycle
-        d.pre_smooth, d.post_smooth, d.sweeps_per_smooth = pre_smooth, post_smooth, sweeps_per_smooth
-        d.omega, d.max_iter, d.bottom_max_iter, d.bottom_tol_rel = omega, max_iter, bottom_max_iter, bottom_tol_rel
-        d.max_coarsening_level, d.verbose = max_coarsening_level, verbose
-        d.smoother = smoother_code(smoother)
-        d.coarse_op = coarse_op_code(coarse_op)
-        d.fixed_iter = int(fixed_iter)
-        d.bottom_solver = bottom_solver_code(bottom_solver)
-        d.sweeps_finest = int(sweeps_finest)
+    def __init__(self, ncell, dx, sym=ISO, bc=None, **opts):
+        d = make_desc(ncell, dx, sym, bc, **opts)
         self.desc = d
         self.sym = sym
         self.ncell = tuple(int(n) for n in ncell)
@@ -112,6 +118,17 @@ class Elastic:
         self.h = h
         self._size = [self.LevelLayout(l)[2] for l in range(self.NMGLevels())]
 
+    @classmethod
+    def _view(cls, h, owner, ncell, dx, sym, desc):
+        """an operator on a handle that `owner` holds and destroys (an AMR level of Composite)"""
+        self = cls.__new__(cls)
+        self.desc, self.sym, self._owner = desc, sym, owner
+        self.ncell = tuple(int(n) for n in ncell)
+        self.dx = tuple(float(x) for x in dx)
+        self.h = h
+        self._size = [self.LevelLayout(l)[2] for l in range(self.NMGLevels())]
+        return self
+
     def _vec(self, t, mglev, what, ncomp=3):
         """checked pointer of a device field of ncomp planes on level mglev"""
         if not 0 <= mglev < len(self._size):
@@ -120,6 +137,8 @@ class Elastic:
 
     def __del__(self):
         h = getattr(self, "h", None)
+        if getattr(self, "_owner", None) is not None:
+            return
         if h is not None and h.value:
             try:
                 lib().alamo_mg_destroy(h)
@@ -612,6 +631,8 @@ class Net:
 
     def __del__(self):
         h = getattr(self, "h", None)
+        if getattr(self, "_owner", None) is not None:
+            return
         if h is not None and h.value:
             try:
                 lib().alamo_net_destroy(h)

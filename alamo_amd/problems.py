@@ -361,3 +361,109 @@ def two_phase_problem(n=256, corr_cells=8, contrast=1000.0, seed=256, load=1e-3,
     bct, bcv = bc_constant()
     return dict(ncell=ncell, dx=(1.0 / n,) * 3, C=C, F0=F0.ravel(), eta=eta, bct=bct, bcv=bcv, width=width,
                 interface=stats)
+
+
+# ---------------------------------------------------------------------------
+# The reference's multi-level regression problems (tests/UniaxialTension, tests/TrigTest) and the
+# AMR Eshelby configuration, on explicit meshes (explicitmesh.lo<l> / hi<l>, Integrator.cpp:121-139)
+# ---------------------------------------------------------------------------
+def _face_key(name):
+    """the reference's lower-case region key ("xloylo", "zhixlo", ...) -> FACES name"""
+    parts = [name[i:i + 3] for i in range(0, len(name), 3)]
+    return "_".join(p.upper() for p in parts)
+
+
+def explicit_mesh(n, nlevels=3):
+    """tests/UniaxialTension/input:42-46 and tests/TrigTest/input:31-35 scaled to n base cells: level
+    1 the middle half of the domain, level 2 the middle half of level 1 (level l box in level-l
+    cells).  n = 32 gives the reference's lo1 16 16 16 / hi1 47 47 47, lo2 48 48 48 / hi2 79 79 79."""
+    boxes = []
+    for l in range(1, nlevels):
+        N = n << l                      # level-l cells across the domain
+        w = N >> l                      # the box spans 1/2^l of the domain
+        lo = (N - w) // 2
+        boxes.append([((lo,) * 3, (lo + w - 1,) * 3)])
+    return boxes
+
+
+def uniaxial_tension_problem(n=32, E=210.0, nu=0.3, disp=0.1):
+    """tests/UniaxialTension/input: the box [-8, 8]^3 of n^3 cells, isotropic E = 210, nu = 0.3
+    (Model/Solid/Linear/Isotropic.H:166-170: lambda = E nu / (1 + nu) / (1 - 2 nu), mu = E / 2 / (1 + nu)),
+    u_x = 0 at x = -8, u_x = 0.1 at x = 8, u_y = 0 at y = -8, u_z = 0 at z = -8, traction-free otherwise
+    (the input's 26-region table, :72-125).  Exact solution (tests/UniaxialTension/test:26-27):
+    u = eps ((x + 8), -nu (y + 8), -nu (z + 8)) with eps = 0.1 / 16."""
+    lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
+    mu = E / 2.0 / (1.0 + nu)
+    D, T = DISPLACEMENT, TRACTION
+    table = {
+        "xlo": (D, T, T, 0), "xhi": (D, T, T, disp), "ylo": (T, D, T, 0), "yhi": (T, T, T, 0),
+        "zlo": (T, T, D, 0), "zhi": (T, T, T, 0),
+        "xloylo": (D, T, T, 0), "xloyhi": (D, T, T, 0), "xhiylo": (D, T, T, disp), "xhiyhi": (D, T, T, disp),
+        "zloxlo": (D, T, T, 0), "zhixlo": (D, T, T, 0), "zloxhi": (D, T, T, disp), "zhixhi": (D, T, T, disp),
+        "ylozlo": (T, D, D, 0), "ylozhi": (T, D, T, 0), "yhizlo": (T, T, D, 0), "yhizhi": (T, T, T, 0),
+        "xloylozlo": (D, D, D, 0), "xloylozhi": (D, D, T, 0), "xloyhizlo": (D, T, D, 0), "xloyhizhi": (D, T, T, 0),
+        "xhiylozlo": (D, D, D, disp), "xhiylozhi": (D, D, T, disp), "xhiyhizlo": (D, T, D, disp),
+        "xhiyhizhi": (D, T, T, disp),
+    }
+    t, v = bc_constant()
+    for key, (a, b, c, vx) in table.items():
+        f = F[_face_key(key)]
+        t[f] = (a, b, c)
+        v[f] = (vx, 0.0, 0.0)
+    eps = disp / 16.0
+
+    def exact(ncell):
+        X, Y, Z, _ = node_coords(ncell, (-8, -8, -8), (8, 8, 8))
+        return np.concatenate([(eps * (X + 8)).ravel(), (-nu * eps * (Y + 8)).ravel(), (-nu * eps * (Z + 8)).ravel()])
+
+    def model(ncell):
+        N = int(np.prod([c + 1 for c in ncell]))
+        return iso_field(np.full(N, lam), np.full(N, mu))
+
+    return dict(ncell=(n, n, n), dx=(16.0 / n,) * 3, bct=t, bcv=v, lam=lam, mu=mu, exact=exact, model=model)
+
+
+def trig_problem(n=32):
+    """tests/TrigTest/input: the unit box of n^3 cells, the Laplacian model with alpha = 1
+    (Model/Solid/Linear/Laplacian.H:26-60: Matrix4<Diagonal> of ones, f_i = u_i,jj), u = 0 on all 26
+    regions, body force b_x = sin(pi x) sin(pi y) sin(pi z) at the nodes (IC/Trig.H:112-158 with ni =
+    1 1 1).  Exact solution (tests/TrigTest/test:24): u_x = -sin(pi x) sin(pi y) sin(pi z) / (3 pi^2)."""
+    t, v = bc_constant()
+
+    def rhs(ncell):
+        X, Y, Z, _ = node_coords(ncell)
+        b = np.zeros((3, X.size))
+        b[0] = (np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)).ravel()
+        return b.ravel()
+
+    def exact(ncell):
+        X, Y, Z, _ = node_coords(ncell)
+        u = np.zeros((3, X.size))
+        u[0] = (-np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z) / (3.0 * np.pi ** 2)).ravel()
+        return u.ravel()
+
+    def model(ncell):
+        N = int(np.prod([c + 1 for c in ncell]))
+        return np.ones(9 * N)
+
+    return dict(ncell=(n, n, n), dx=(1.0 / n,) * 3, bct=t, bcv=v, rhs=rhs, exact=exact, model=model)
+
+
+def eshelby_amr_mesh(n=64):
+    """The AMR Eshelby configuration's explicit mesh (base n^3 cells; boxes of at most 32^3 cells):
+    level 1 covers coarse cells [n/8, 7n/8) (the spherical interface r = 0.25 with its erf shell
+    and a margin), level 2 the coarse cells [n/4, 3n/4) (the interface), each as a grid of 32^3
+    boxes (n = 64: 27 and 64 boxes)."""
+    def grid(lo, hi, s=32):
+        out = []
+        edges = list(range(lo, hi + 1, s))
+        if edges[-1] != hi + 1:
+            edges.append(hi + 1)
+        for z0, z1 in zip(edges[:-1], edges[1:]):
+            for y0, y1 in zip(edges[:-1], edges[1:]):
+                for x0, x1 in zip(edges[:-1], edges[1:]):
+                    out.append(((x0, y0, z0), (x1 - 1, y1 - 1, z1 - 1)))
+        return out
+    l1 = grid(2 * (n // 8), 2 * (7 * n // 8) - 1)
+    l2 = grid(4 * (n // 4), 4 * (3 * n // 4) - 1)
+    return [l1, l2]

@@ -48,6 +48,7 @@ enum { ALAMO_OK = 0, ALAMO_ERR_ARG = 1, ALAMO_ERR_HIP = 2, ALAMO_ERR_STATE = 3, 
 
 typedef struct alamo_mg alamo_mg;
 typedef struct alamo_net alamo_net;
+typedef struct alamo_amr alamo_amr;
 
 /* Geometry, boundary table and solver options of one hierarchy.  Field names
  * follow the reference's input keys (solver.* of Solver/Nonlocal/Linear.H:296-382,
@@ -344,6 +345,35 @@ int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z);
 int alamo_net_layer_times(alamo_net* net, alamo_mg* h, int reps, double* ms);
 /* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
+
+/* ---------------------------------------------------------------------------------------------
+ * Composite multi-level (AMR) solve: Solver::Nonlocal::Linear::solve over several AMR levels
+ * (Linear.H:131-168 with amrex::MLMG's composite cycle) on an explicit mesh
+ * (Integrator.cpp:121-139: explicitmesh.lo<l> / hi<l>).  Level l has n_cell = base n_cell 2^l and
+ * dx = base dx / 2^l; nbox[l - 1] boxes for each level l >= 1, inclusive cell boxes in that level's
+ * own index space, concatenated in box_lo / box_hi (3 ints each), covering whole coarse cells (lo
+ * even, hi odd, at most 64 per level), one coarse cell inside the domain and properly nested (each
+ * box, coarsened and grown by one cell, inside the union of the next coarser level's boxes).
+ * Every level is a full-domain field at its resolution (3 SoA planes of
+ * (n_cell + 1)^3 nodes); a level's values outside its region follow the coarser level.  The
+ * composite equations and the V-cycle are described in the library source (capi.hip).  Options
+ * (cycle, smoothing, smoother, tolerances, max_iter) come from `base`.
+ * --------------------------------------------------------------------------------------------- */
+int alamo_amr_create(const alamo_mg_desc* base, int nlevels, const int* nbox, const int* box_lo, const int* box_hi,
+                     alamo_amr** amr);
+int alamo_amr_destroy(alamo_amr* amr);
+/* the operator handle of AMR level amrlev (Elastic::SetModel(amrlev, ...), alamo_bc_init,
+ * alamo_newton_rhs, Fapply, post-processing all work on it) */
+int alamo_amr_level(alamo_amr* amr, int amrlev, alamo_mg** h);
+/* device int arrays of the level's nodes: region (2 inside the level's region, 1 on its boundary,
+ * 0 outside; level 0: all 2) and coverage by the next level (the nodal fine mask: 2 fine, 1
+ * coarse-fine, 0 coarse; the finest level: all 0); either may be null */
+int alamo_amr_masks(alamo_amr* amr, int amrlev, int* region, int* fine_mask, void* stream);
+/* sol[l] (initial guess in, solution out) and rhs[l]: device fields of every level.  Stops when the
+ * composite residual's max norm is <= max(tol_abs, tol_rel max(|b|_inf, |r0|_inf)); covered coarse
+ * nodes receive the fine solution (averageDownSolutionRHS, Operator.cpp:435-447). */
+int alamo_amr_solve(alamo_amr* amr, double* const* sol, const double* const* rhs, double tol_rel, double tol_abs,
+                    alamo_solve_stats* stats);
 
 #ifdef __cplusplus
 }

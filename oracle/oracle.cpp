@@ -1256,6 +1256,224 @@ static void galerkin_levels(Hier& H, int upto, std::vector<std::vector<double>>&
     }
 }
 
+// ---------------------------------------------------------------------------
+// Composite multi-level (AMR) system and solve, the library's alamo_amr_* (no single reference
+// function: Linear::solve over several AMR levels, Linear.H:131-168, drives amrex::MLMG's composite
+// cycle with Operator<Grid::Node>::reflux, Operator.cpp:526-633, and averageDownSolutionRHS,
+// :435-447; AMReX is not available, so the C/F treatment is this restatement's own, identical
+// to the library's).  Every level is a full-domain field at resolution n_cell 2^l:
+//   * unknowns: level 0 and the nodes strictly inside each fine region (region mask 2); fine
+//     nodes on or outside the region boundary follow the coarser level, u_l = P u_{l-1}; coarse
+//     nodes covered by the next level (nodal fine mask 1, 2) hold the coincident fine values;
+//   * level l rows (fapply) on level 0 and on each fine region's interior, the coarse/fine
+//     interface nodes (fine mask 1) keeping their coarse rows;
+//   * strictly covered coarse nodes (fine mask 2) take the full-weighting restriction of the finer
+//     residual (the reflux of the correction scheme).
+// The solve: composite V-cycles (masked smoothing and reflux down the levels, the level-0
+// multigrid V-cycle, prolongation and smoothing up), then covered coarse nodes take the fine
+// solution.  Pinned by tests/test_oracle_amr_composite.py (the reference's UniaxialTension
+// closed form: linear fields are reproduced exactly; the TrigTest closed form to O(h^2)).
+// ---------------------------------------------------------------------------
+extern "C" void oamr_fine_mask(const int* nc, int nbox, const int* flo, const int* fhi, int* mask);
+
+struct Composite {
+    int nl = 0;
+    std::vector<Hier> H;
+    std::vector<std::vector<int>> region, fmask;
+    MGOpts o{};
+};
+
+static void composite_build(Composite& A, const int* ncell0, const double* dx0, int sym, const int* bct, int nl,
+                            const double* const* C, const int* nbox, const int* box_lo, const int* box_hi,
+                            const MGOpts& o) {
+    A.nl = nl;
+    A.o = o;
+    A.H.resize(nl);
+    A.region.resize(nl);
+    A.fmask.resize(nl);
+    std::vector<int> off(nl + 1, 0);
+    for (int l = 1; l < nl; l++) off[l + 1] = off[l] + nbox[l - 1];
+    for (int l = 0; l < nl; l++) {
+        int nc[3];
+        double h[3];
+        for (int d = 0; d < 3; d++) { nc[d] = ncell0[d] << l; h[d] = dx0[d] / (double)(1 << l); }
+        build_hier(A.H[l], nc, h, sym, bct, C[l], l == 0 ? -1 : 0);
+        if (o.smoother == 1) {
+            A.H[l].Dl1.resize(A.H[l].L.size());
+            for (size_t m = 0; m < A.H[l].L.size(); m++) {
+                A.H[l].Dl1[m].resize(3 * A.H[l].L[m].tot);
+                l1_diag_rows(A.H[l].L[m], sym, A.H[l].C[m].data(), bct, A.H[l].Dl1[m].data());
+            }
+        }
+        const int64_t tot = A.H[l].L[0].tot;
+        if (l > 0) {
+            // the level's own boxes, written as the boxes of a virtual level above it
+            std::vector<int> flo(3 * nbox[l - 1]), fhi(3 * nbox[l - 1]);
+            for (int t = 0; t < nbox[l - 1]; t++)
+                for (int d = 0; d < 3; d++) {
+                    flo[3 * t + d] = 2 * box_lo[3 * (off[l] + t) + d];
+                    fhi[3 * t + d] = 2 * box_hi[3 * (off[l] + t) + d] + 1;
+                }
+            A.region[l].resize(tot);
+            oamr_fine_mask(nc, nbox[l - 1], flo.data(), fhi.data(), A.region[l].data());
+        }
+        if (l < nl - 1) {
+            A.fmask[l].resize(tot);
+            oamr_fine_mask(nc, nbox[l], box_lo + 3 * off[l + 1], box_hi + 3 * off[l + 1], A.fmask[l].data());
+        }
+    }
+}
+
+static void masked_copy(const std::vector<int>& mask, int lo, int hi, int64_t n, double* dst, const double* src) {
+    for (int64_t q = 0; q < n; q++)
+        if (mask[q] >= lo && mask[q] <= hi)
+            for (int c = 0; c < 3; c++) dst[c * n + q] = src[c * n + q];
+}
+static void masked_zero(const std::vector<int>& mask, int lo, int hi, int64_t n, double* x) {
+    for (int64_t q = 0; q < n; q++)
+        if (mask[q] < lo || mask[q] > hi)
+            for (int c = 0; c < 3; c++) x[c * n + q] = 0.0;
+}
+
+// covered coarse nodes (nodal fine mask 1, 2) take the coincident fine values, finest first
+static void composite_inject(Composite& A, double* const* u) {
+    for (int l = A.nl - 2; l >= 0; l--) {
+        const Level& Lc = A.H[l].L[0];
+        const Level& Lf = A.H[l + 1].L[0];
+        for (int K = 0; K < Lc.N[2]; K++)
+            for (int J = 0; J < Lc.N[1]; J++)
+                for (int I = 0; I < Lc.N[0]; I++) {
+                    const int64_t m = Lc.id(I, J, K);
+                    if (A.fmask[l][m] < 1) continue;
+                    const int64_t mf = Lf.id(2 * I, 2 * J, 2 * K);
+                    for (int q = 0; q < 3; q++) u[l][q * Lc.tot + m] = u[l + 1][q * Lf.tot + mf];
+                }
+    }
+}
+
+// e~ = the fine field injected at the nodes strictly covered (fine mask 2), zero elsewhere
+static void inject_strict(const Level& Lc, const Level& Lf, const std::vector<int>& fm, const double* fine, double* out) {
+    std::fill(out, out + 3 * Lc.tot, 0.0);
+    for (int K = 0; K < Lc.N[2]; K++)
+        for (int J = 0; J < Lc.N[1]; J++)
+            for (int I = 0; I < Lc.N[0]; I++) {
+                const int64_t m = Lc.id(I, J, K);
+                if (fm[m] != 2) continue;
+                const int64_t mf = Lf.id(2 * I, 2 * J, 2 * K);
+                for (int q = 0; q < 3; q++) out[q * Lc.tot + m] = fine[q * Lf.tot + mf];
+            }
+}
+
+// covered coarse nodes first take the fine values; then r_l = b_l - A_l u_l on level 0 and on each
+// fine region's interior (zero elsewhere), the strictly covered nodes replaced by the restriction of
+// r_{l+1}
+static void composite_residual(Composite& A, double* const* u, const double* const* b, double* const* r) {
+    composite_inject(A, u);
+    for (int l = A.nl - 1; l >= 0; l--) {
+        const Level& L = A.H[l].L[0];
+        residual(L, A.H[l].sym, A.H[l].C[0].data(), A.H[l].bct, u[l], b[l], r[l]);
+        if (l > 0) masked_zero(A.region[l], 2, 2, L.tot, r[l]);
+        if (l < A.nl - 1) {
+            std::vector<double> t(3 * L.tot);
+            restrict_fw(L, A.H[l + 1].L[0], 1, 3, r[l + 1], t.data());
+            masked_copy(A.fmask[l], 2, 2, L.tot, r[l], t.data());
+        }
+    }
+}
+
+static void composite_smooth(Composite& A, int l, double* e, const double* c, int nsweeps) {
+    const Level& L = A.H[l].L[0];
+    std::vector<double> t(e, e + 3 * L.tot);
+    for (int s = 0; s < nsweeps; s++) {
+        if (A.o.smoother == 1) jacobi_l1(L, A.H[l].sym, A.H[l].C[0].data(), A.H[l].bct, A.H[l].Dl1[0].data(), 1, t.data(), c);
+        else jacobi(L, A.H[l].sym, A.H[l].C[0].data(), A.H[l].bct, A.H[l].D[0].data(), A.o.omega, 1, t.data(), c);
+        masked_copy(A.region[l], 2, 2, L.tot, e, t.data());
+        std::copy(e, e + 3 * L.tot, t.begin());
+    }
+}
+
+static int composite_solve(Composite& A, double* const* u, const double* const* b, double tol_rel, double tol_abs,
+                           double* final_res) {
+    const int nl = A.nl;
+    std::vector<std::vector<double>> r(nl), c(nl), e(nl), rr(nl);
+    std::vector<double*> rp(nl);
+    for (int l = 0; l < nl; l++) {
+        const int64_t n = 3 * A.H[l].L[0].tot;
+        r[l].assign(n, 0.0); c[l].assign(n, 0.0); e[l].assign(n, 0.0); rr[l].assign(n, 0.0);
+        rp[l] = r[l].data();
+    }
+    for (int l = 1; l < nl; l++) {
+        const int64_t n = A.H[l].L[0].tot;
+        std::vector<double> t(3 * n, 0.0);
+        interp_add(A.H[l - 1].L[0], A.H[l].L[0], u[l - 1], t.data());
+        masked_copy(A.region[l], 0, 1, n, u[l], t.data());
+    }
+    double bn = 0.0;
+    for (int l = 0; l < nl; l++) {
+        const int64_t n = A.H[l].L[0].tot;
+        std::vector<double> t(b[l], b[l] + 3 * n);
+        if (l > 0) masked_zero(A.region[l], 2, 2, n, t.data());
+        bn = std::max(bn, norm_inf(t.data(), 3 * n));
+    }
+    auto rnorm = [&]() {
+        double m = 0.0;
+        for (int l = 0; l < nl; l++) m = std::max(m, norm_inf(r[l].data(), 3 * A.H[l].L[0].tot));
+        return m;
+    };
+    composite_residual(A, u, b, rp.data());
+    const double r0 = rnorm();
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * std::max(r0, bn));
+    double rn = r0;
+    int it = 0;
+    MGWork W;
+    W.cor.resize(A.H[0].L.size()); W.res.resize(A.H[0].L.size()); W.rescor.resize(A.H[0].L.size());
+    for (size_t m = 0; m < A.H[0].L.size(); m++) {
+        W.cor[m].assign(3 * A.H[0].L[m].tot, 0.0);
+        W.res[m].assign(3 * A.H[0].L[m].tot, 0.0);
+        W.rescor[m].assign(3 * A.H[0].L[m].tot, 0.0);
+    }
+    const int nsw = A.o.nu1 * A.o.sweeps, nsw2 = A.o.nu2 * A.o.sweeps;
+    while (rn > target && it < A.o.max_iter && std::isfinite(rn)) {
+        const int L = nl - 1;
+        for (int l = L; l >= 0; l--) {
+            const Level& Lv = A.H[l].L[0];
+            c[l] = r[l];
+            if (l < L) {
+                // strictly covered nodes: the restricted residual of the finer correction equation;
+                // coarse/fine nodes: their coarse rows less the effect of the finer correction
+                // injected at the strictly covered nodes
+                std::vector<double> t(3 * Lv.tot), et(3 * Lv.tot);
+                restrict_fw(Lv, A.H[l + 1].L[0], 1, 3, rr[l + 1].data(), t.data());
+                masked_copy(A.fmask[l], 2, 2, Lv.tot, c[l].data(), t.data());
+                inject_strict(Lv, A.H[l + 1].L[0], A.fmask[l], e[l + 1].data(), et.data());
+                residual(Lv, A.H[l].sym, A.H[l].C[0].data(), A.H[l].bct, et.data(), c[l].data(), t.data());
+                masked_copy(A.fmask[l], 1, 1, Lv.tot, c[l].data(), t.data());
+            }
+            if (l == 0) break;
+            std::fill(e[l].begin(), e[l].end(), 0.0);
+            composite_smooth(A, l, e[l].data(), c[l].data(), nsw);
+            residual(Lv, A.H[l].sym, A.H[l].C[0].data(), A.H[l].bct, e[l].data(), c[l].data(), rr[l].data());
+            masked_zero(A.region[l], 2, 2, Lv.tot, rr[l].data());
+        }
+        W.res[0] = c[0];
+        if (A.o.cycle == 1) fcycle(A.H[0], A.o, W);
+        else vcycle(A.H[0], A.o, W, 0);
+        e[0] = W.cor[0];
+        for (int l = 1; l < nl; l++) {
+            interp_add(A.H[l - 1].L[0], A.H[l].L[0], e[l - 1].data(), e[l].data());
+            composite_smooth(A, l, e[l].data(), c[l].data(), nsw2);
+        }
+        for (int l = 0; l < nl; l++)
+            for (size_t q = 0; q < e[l].size(); q++) u[l][q] += e[l][q];
+        composite_residual(A, u, b, rp.data());
+        rn = rnorm();
+        it++;
+    }
+    composite_inject(A, u);
+    if (final_res) *final_res = rn;
+    return rn <= target ? it : -it;
+}
+
 // FGMRES(restart) with one V-cycle (nu1 / nu2 smoothing calls of `sweeps` sweeps, `sweeps_finest` on
 // level 0 when > 0; smoother 0 damped point Jacobi, 1 l1-Jacobi; BiCGStab bottom to bottom_tol) as the
 // right preconditioner.  Returns the iterations, negative when max_iter ended the solve.
@@ -1287,6 +1505,57 @@ void ora_galerkin(const int* ncell, const double* dx, int sym, const double* C, 
 void ora_stencil_apply(const int* ncell, const double* dx, const double* S, const double* x, double* y) {
     Level L(ncell, dx);
     stencil_apply(L, S, x, y);
+}
+// Composite AMR solve / residual.  Level fields are concatenated: level l has 3 (n_cell 2^l + 1)^3
+// values (C: ncoef per node, AoS); boxes as alamo_amr_create.
+static void composite_args(int nl, const int* ncell0, int sym, const double* C, const double* v,
+                           std::vector<const double*>& Cl, std::vector<double*>& vl, int nrec_v) {
+    int64_t oc = 0, ov = 0;
+    Cl.resize(nl);
+    vl.resize(nl);
+    for (int l = 0; l < nl; l++) {
+        const int64_t tot = (int64_t)((ncell0[0] << l) + 1) * ((ncell0[1] << l) + 1) * ((ncell0[2] << l) + 1);
+        Cl[l] = C + oc;
+        vl[l] = (double*)v + ov;
+        oc += tot * ncoef(sym);
+        ov += tot * nrec_v;
+    }
+}
+int ora_amr_solve(const int* ncell0, const double* dx0, int sym, const double* C, const int* bct, int nl,
+                  const int* nbox, const int* box_lo, const int* box_hi, double* u, const double* b, double tol_rel,
+                  double tol_abs, int cycle, int nu1, int nu2, int sweeps, int smoother, double omega, int max_iter,
+                  double* final_res) {
+    std::vector<const double*> Cl;
+    std::vector<double*> ul, bl;
+    composite_args(nl, ncell0, sym, C, u, Cl, ul, 3);
+    std::vector<const double*> tmp;
+    composite_args(nl, ncell0, sym, C, b, tmp, bl, 3);
+    MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, 1e-4};
+    o.smoother = smoother;
+    Composite A;
+    composite_build(A, ncell0, dx0, sym, bct, nl, Cl.data(), nbox, box_lo, box_hi, o);
+    std::vector<const double*> bc(bl.begin(), bl.end());
+    return composite_solve(A, ul.data(), bc.data(), tol_rel, tol_abs, final_res);
+}
+void ora_amr_residual(const int* ncell0, const double* dx0, int sym, const double* C, const int* bct, int nl,
+                      const int* nbox, const int* box_lo, const int* box_hi, const double* u, const double* b,
+                      double* r) {
+    std::vector<const double*> Cl, tmp;
+    std::vector<double*> ul, bl, rl;
+    composite_args(nl, ncell0, sym, C, u, Cl, ul, 3);
+    composite_args(nl, ncell0, sym, C, b, tmp, bl, 3);
+    composite_args(nl, ncell0, sym, C, r, tmp, rl, 3);
+    MGOpts o{0, 1, 1, 2, 2.0 / 3.0, 0, 200, 1e-4};
+    Composite A;
+    composite_build(A, ncell0, dx0, sym, bct, nl, Cl.data(), nbox, box_lo, box_hi, o);
+    std::vector<const double*> bc(bl.begin(), bl.end());
+    // the residual injects the fine values into covered coarse nodes: work on a copy of u
+    int64_t nu = 0;
+    for (int l = 0; l < nl; l++) nu += 3 * A.H[l].L[0].tot;
+    std::vector<double> uc(u, u + nu);
+    std::vector<double*> ucl;
+    composite_args(nl, ncell0, sym, C, uc.data(), Cl, ucl, 3);
+    composite_residual(A, ucl.data(), bc.data(), rl.data());
 }
 void ora_l1_diag_rows(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* d) {
     Level L(ncell, dx);

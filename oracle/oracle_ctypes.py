@@ -497,3 +497,44 @@ def amr_fixup_residual_mask(mask, resmsk):
     f.argtypes = [ctypes.c_int64, I, I]
     f(len(out), _i(_c(mask, np.int32)), _i(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Composite multi-level (AMR) solve: levels l = 0 .. nl-1 at n_cell 2^l, boxes (per level >= 1, a
+# list of (lo, hi) inclusive cell boxes in the level's own index space); C, u, b: lists of per-level
+# arrays (C AoS ncoef per node, u / b 3 SoA planes)
+# ---------------------------------------------------------------------------
+def _amr_boxes(boxes):
+    nbox = np.array([len(bl) for bl in boxes], dtype=np.int32)
+    lo = np.array([b[0] for bl in boxes for b in bl], dtype=np.int32).reshape(-1)
+    hi = np.array([b[1] for bl in boxes for b in bl], dtype=np.int32).reshape(-1)
+    return nbox, lo, hi
+
+
+def amr_solve(ncell0, dx0, sym, Cs, bct, boxes, bs, us=None, tol_rel=1e-10, tol_abs=0.0, cycle=0, nu1=2, nu2=2,
+              sweeps=2, smoother=0, omega=2.0 / 3.0, max_iter=400):
+    nl = len(Cs)
+    ncell0 = _c(ncell0, np.int32); dx0 = _c(dx0); bct = _c(bct, np.int32)
+    nbox, lo, hi = _amr_boxes(boxes)
+    C = _c(np.concatenate([np.asarray(c, dtype=np.float64).ravel() for c in Cs]))
+    b = _c(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in bs]))
+    u = np.zeros_like(b) if us is None else _c(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in us])).copy()
+    fr = ctypes.c_double(0)
+    it = lib().ora_amr_solve(_i(ncell0), _d(dx0), sym, _d(C), _i(bct), nl, _i(nbox), _i(lo), _i(hi), _d(u), _d(b),
+                             ctypes.c_double(tol_rel), ctypes.c_double(tol_abs), cycle, nu1, nu2, sweeps, smoother,
+                             ctypes.c_double(omega), max_iter, ctypes.byref(fr))
+    sizes = [3 * int(np.prod([(int(n) << l) + 1 for n in ncell0])) for l in range(nl)]
+    return np.split(u, np.cumsum(sizes)[:-1]), it, fr.value
+
+
+def amr_residual(ncell0, dx0, sym, Cs, bct, boxes, us, bs):
+    nl = len(Cs)
+    ncell0 = _c(ncell0, np.int32); dx0 = _c(dx0); bct = _c(bct, np.int32)
+    nbox, lo, hi = _amr_boxes(boxes)
+    C = _c(np.concatenate([np.asarray(c, dtype=np.float64).ravel() for c in Cs]))
+    u = _c(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in us]))
+    b = _c(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in bs]))
+    r = np.zeros_like(b)
+    lib().ora_amr_residual(_i(ncell0), _d(dx0), sym, _d(C), _i(bct), nl, _i(nbox), _i(lo), _i(hi), _d(u), _d(b), _d(r))
+    sizes = [3 * int(np.prod([(int(n) << l) + 1 for n in ncell0])) for l in range(nl)]
+    return np.split(r, np.cumsum(sizes)[:-1])

@@ -71,3 +71,18 @@ def test_grain_table_shapes():
         op.SetModelGrains(np.zeros(n - 1, dtype=np.uint16), np.zeros((2, 21)))
     with pytest.raises(AlamoError, match="table must be"):
         op.SetModelGrains(np.zeros(n, dtype=np.uint16), np.zeros(42))
+
+
+def test_amr_create_rejects_bad_meshes():
+    """alamo_amr_create validates the explicit mesh before touching the device"""
+    from alamo_amd.amr import Composite
+    n = 16
+    bad = [
+        [[((3, 4, 4), (11, 11, 11))]],                               # lo odd
+        [[((0, 4, 4), (11, 11, 11))]],                               # no coarse cell of margin
+        [[((4, 4, 4), (11, 11, 11))], [((8, 8, 8), (23, 23, 23))]],  # level 2 not nested with margin
+        [[((4, 4, 4), (11, 11, 11))] * 65],                          # too many boxes
+    ]
+    for boxes in bad:
+        with pytest.raises(AlamoError):
+            Composite((n, n, n), (1.0 / n,) * 3, boxes)
