@@ -122,6 +122,7 @@ SIGNATURES = {
     "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_stream_probe": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p]),
     "alamo_amr_create": (c_int, [P(MGDesc), c_int, c_void_p, c_void_p, c_void_p, P(c_void_p)]),
     "alamo_amr_destroy": (c_int, [c_void_p]),
     "alamo_amr_level": (c_int, [c_void_p, c_int, P(c_void_p)]),

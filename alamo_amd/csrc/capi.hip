@@ -27,6 +27,7 @@
 #include "amr.h"
 #include "post.h"
 #include "galerkin.h"
+#include "calib.h"
 
 using namespace alamo;
 
@@ -1524,6 +1525,14 @@ int alamo_mg_fsmooth(alamo_mg* h, int amrlev, int mglev, double* x, const double
     if (a != x) HIPCHK(hipMemcpyAsync(x, a, 3 * lb.g.tot * sizeof(double), hipMemcpyDeviceToDevice,
                                       (hipStream_t)stream));
     HIPCHK(hipFreeAsync(tmp, (hipStream_t)stream));
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_stream_probe(const double* a, double* o, int64_t n, void* stream) {
+    if (!a || !o) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (n < 1) FAIL(ALAMO_ERR_ARG, "n must be positive");
+    k_stream_5r3w<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(n, a, o);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }

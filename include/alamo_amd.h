@@ -346,6 +346,11 @@ int alamo_net_layer_times(alamo_net* net, alamo_mg* h, int reps, double* ms);
 /* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
 
+/* Attainable-bandwidth calibration for roofline reports (no counterpart in the reference): o[v n + q]
+ * = a[v n + q] a[3 n + q] + a[4 n + q], v = 0..2, for q < n -- the isotropic FP64 Fapply's five
+ * reads and three writes per node as a plain stream.  a: 5 n doubles, o: 3 n doubles (device). */
+int alamo_stream_probe(const double* a, double* o, int64_t n, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Composite multi-level (AMR) solve: Solver::Nonlocal::Linear::solve over several AMR levels
  * (Linear.H:131-168 with amrex::MLMG's composite cycle) on an explicit mesh
