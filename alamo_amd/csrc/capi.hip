@@ -742,7 +742,7 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     if (kind == ALAMO_PRECOND_LEARNED) {
         // one multigrid cycle, then the network's correction of the cycle's remaining error
         TRY(precond_apply(h, ALAMO_PRECOND_MG, v, z));
-        if (!h->lq) TRY(dalloc(h, &h->lq, 3 * n));
+        if (!h->lq) TRY(dalloc(h, &h->lq, 5 * n));
         return net_correct(h->net, h, v, z, h->lq);
     }
     if (kind == ALAMO_PRECOND_NONE) {
@@ -843,8 +843,10 @@ static int upload_coef(alamo_mg* h, const double* c, int nv) {
 }
 
 // One FP32 preconditioner application of the mixed-precision inner iterations: z = cycle(v), FP32
-// in and out (the cycle's own FP32 level-0 buffers stage them).
-static int precond_f32(alamo_mg* h, const float* v, float* z) {
+// in and out (the cycle's own FP32 level-0 buffers stage them); with the learned kind the network
+// stage follows on FP64 copies of v and z (its residual, network and step are FP64 / BF16 as in the
+// FP64 iteration).  Returns NET_NONFINITE (z holding the plain cycle's result) like precond_apply.
+static int precond_f32(alamo_mg* h, int kind, const float* v, float* z) {
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
     if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
@@ -854,11 +856,21 @@ static int precond_f32(alamo_mg* h, const float* v, float* z) {
         one_cycle<float>(h);
         HIPCHK(hipMemcpyAsync(z, l0.corf, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     }
+    if (kind != ALAMO_PRECOND_LEARNED) return ALAMO_OK;
+    if (!h->lq) TRY(dalloc(h, &h->lq, 5 * n));
+    double* v64 = h->lq + 3 * n;
+    double* z64 = h->lq + 4 * n;
+    k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(v, v64, n);
+    k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(z, z64, n);
+    const int rc = net_correct(h->net, h, v64, z64, h->lq);
+    if (rc != ALAMO_OK) return rc;
+    k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(z64, z, n);
+    HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
 
 // FGMRES(m) with the Krylov vectors in precision TB.  TB = double: the FP64 iteration, any
-// preconditioner kind.  TB = float (mixed precision, cycle preconditioner): the basis, the
+// preconditioner kind.  TB = float (mixed precision, cycle or learned preconditioner): the basis, the
 // preconditioned vectors and A z are FP32 inside a restart cycle, a GMRES-based iterative
 // refinement (Carson and Higham, SIAM J. Sci. Comput. 40 (2018)): every restart recomputes the
 // FP64 true residual from the FP64 solution, and only that residual decides convergence.
@@ -920,22 +932,19 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
             TB* vj = V + (int64_t)j * n;
             TB* zj = Z + (int64_t)j * n;
             TB* w = V + (int64_t)(j + 1) * n;
-            if constexpr (F32) {
-                TRY(precond_f32(h, vj, zj));
-                launch_node<float>(h->sym, APPLY, l0.g, h->bc, l0.Cf, zj, nullptr, w, 0.f, h->stream);
-            } else {
-                const int prc = precond_apply(h, kind, vj, zj);
-                if (prc == NET_NONFINITE) {
-                    // zj holds the plain cycle's result; the plain cycle takes over for good
-                    kind = ALAMO_PRECOND_MG;
-                    fallbacks++;
-                    if (o->verbose)
-                        fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
-                } else if (prc != ALAMO_OK) {
-                    return prc;
-                }
-                launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
+            int prc;
+            if constexpr (F32) prc = precond_f32(h, kind, vj, zj);
+            else prc = precond_apply(h, kind, vj, zj);
+            if (prc == NET_NONFINITE) {
+                // zj holds the plain cycle's result; the plain cycle takes over for good
+                kind = ALAMO_PRECOND_MG;
+                fallbacks++;
+                if (o->verbose) fprintf(stderr, "alamo fgmres: learned stage not finite, falling back to the cycle\n");
+            } else if (prc != ALAMO_OK) {
+                return prc;
             }
+            if constexpr (F32) launch_node<float>(h->sym, APPLY, l0.g, h->bc, l0.Cf, zj, nullptr, w, 0.f, h->stream);
+            else launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, zj, nullptr, w, 0.0, h->stream);
             // classical Gram-Schmidt on the stored vectors (w' = A z'_j, z'_j = M(stored v_j)); w
             // is vector j + 1 of the basis block, so the same multi-dot pass returns |w'|^2 in
             // d1[j + 1]; the true Arnoldi quantities follow from the scales sc
@@ -1043,7 +1052,7 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
     return ALAMO_OK;
 }
 
-// inner iterations in FP32 for mixed-precision handles with the cycle preconditioner
+// inner iterations in FP32 for mixed-precision handles with the cycle or learned preconditioner
 // (ALAMO_KRY_F32=0 keeps them FP64, for comparisons)
 static const bool g_kry_f32 = [] {
     const char* e = getenv("ALAMO_KRY_F32");
@@ -1052,7 +1061,8 @@ static const bool g_kry_f32 = [] {
 
 static int fgmres_impl(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        const alamo_krylov_opts* o, alamo_solve_stats* st) {
-    if (g_kry_f32 && h->d.precision == ALAMO_PREC_MIXED && o->precond == ALAMO_PRECOND_MG)
+    if (g_kry_f32 && h->d.precision == ALAMO_PREC_MIXED &&
+        (o->precond == ALAMO_PRECOND_MG || o->precond == ALAMO_PRECOND_LEARNED))
         return fgmres_core<float>(h, sol, rhs, tol_rel, tol_abs, o, st);
     return fgmres_core<double>(h, sol, rhs, tol_rel, tol_abs, o, st);
 }

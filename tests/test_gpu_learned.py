@@ -195,31 +195,15 @@ def test_learned_fallback_on_non_finite_network(cuda, ora):
     assert np.all(np.isfinite(to_host(sol)))
 
 
-def test_learned_stage_gain(cuda, ora):
-    """What the trained network adds to one multigrid cycle on a 1:1000 two-phase field it was
-    not trained on: the residual after cycle + network stage is never above the residual after
-    the cycle alone (the stage's step minimises |q - alpha A y|_2), and the measured reduction
-    is printed.  Training uses a different random field of the same size."""
+def _stage_gains(op, net, n, cuda, nrhs=3):
+    """|q| after cycle + network stage / |q| after the cycle alone, q = r - A z, on random
+    correction right-hand sides (zero on the Dirichlet rows)"""
     import torch
-    n = 32
-    pr_train, bc_train, _ = two_phase_rhs(ora, n, seed=101)
-    pr, bc, rhs = two_phase_rhs(ora, n, seed=7)
-    op_train = Elastic(pr_train["ncell"], pr_train["dx"], sym=P.ISO, bc=bc_train, precision=PREC_MIXED,
-                       cycle=CYCLE_V, pre_smooth=1, post_smooth=1)
-    op_train.SetModel(pr_train["C"])
-    net = Net(seed=4)
-    print("train", flush=True)
-    st = net.train(op_train, npairs=8, steps=300, lr=1e-3, seed=5)
-    print("train stats", st)
-    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=PREC_MIXED, cycle=CYCLE_V, pre_smooth=1,
-                 post_smooth=1)
-    op.SetModel(pr["C"])
     net.attach(op)
     N = (n + 1) ** 3
     gains = []
-    for seed in range(3):
+    for seed in range(nrhs):
         r = torch.from_numpy(np.random.default_rng(seed).uniform(-1, 1, 3 * N)).to(cuda)
-        # Dirichlet rows of a correction right-hand side are zero
         r[torch.from_numpy(np.tile(_boundary_mask(n), 3)).to(cuda)] = 0.0
         res = {}
         for kind in (PRECOND_MG, PRECOND_LEARNED):
@@ -229,15 +213,54 @@ def test_learned_stage_gain(cuda, ora):
             op.correctionResidual(0, 0, q, z, r)
             res[kind] = float(torch.linalg.norm(q))
         gains.append(res[PRECOND_LEARNED] / res[PRECOND_MG])
-        assert res[PRECOND_LEARNED] <= res[PRECOND_MG] * (1 + 1e-10)
-    print("residual after cycle + network / after cycle:", gains)
+    return gains
+
+
+def _trained_net_and_field(ora):
+    """a network trained on one 33^3 1:1000 two-phase field, and a held-out field of the same kind"""
+    n = 32
+    pr_train, bc_train, _ = two_phase_rhs(ora, n, seed=101)
+    pr, bc, rhs = two_phase_rhs(ora, n, seed=7)
+    op_train = Elastic(pr_train["ncell"], pr_train["dx"], sym=P.ISO, bc=bc_train, precision=PREC_MIXED,
+                       cycle=CYCLE_V, pre_smooth=1, post_smooth=1)
+    op_train.SetModel(pr_train["C"])
+    net = Net(seed=4)
+    print("train", flush=True)
+    st = net.train(op_train, npairs=8, steps=300, lr=1e-3, seed=5)
+    print("train stats", st, flush=True)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=PREC_MIXED, cycle=CYCLE_V, pre_smooth=1,
+                 post_smooth=1)
+    op.SetModel(pr["C"])
+    return net, op, pr, rhs, n
+
+
+def test_training_makes_the_stage_remove_residual(cuda, ora):
+    """On a held-out 1:1000 field the trained network's stage removes a real part of the residual
+    the cycle leaves, where the same architecture untrained removes almost none (the stage's
+    residual-minimising step alone cannot tell them apart: it never raises the residual)."""
+    net, op, pr, rhs, n = _trained_net_and_field(ora)
+    trained = _stage_gains(op, net, n, cuda)
+    untrained = _stage_gains(op, Net(seed=4), n, cuda)
+    print("residual after cycle + stage / after cycle: trained", trained, "untrained", untrained, flush=True)
+    assert max(trained) <= 1.0 + 1e-10 and max(untrained) <= 1.0 + 1e-10
+    assert np.mean(trained) <= 0.95
+    assert np.mean(trained) <= np.mean(untrained) - 0.03
+
+
+def test_learned_fgmres_iterations_vs_plain(cuda, ora):
+    """FGMRES with the trained stage against the same FGMRES without it on the held-out field:
+    both converge; the stage may not cost iterations (at most 10 % more).  Iterations and seconds
+    are printed; the stage's wall-time cost is the subject of the benchmark, not of this test."""
+    net, op, pr, rhs, n = _trained_net_and_field(ora)
+    net.attach(op)
     sol = {}
     for kind in (PRECOND_MG, PRECOND_LEARNED):
         x = to_dev(np.zeros_like(rhs), cuda)
         sol[kind] = Krylov(op, restart=16, precond=kind, max_iter=400).solve(x, to_dev(rhs, cuda), 1e-10, 0.0)
         print("precond", kind, "iterations", sol[kind]["iterations"], "seconds", sol[kind]["seconds"],
-              "fallbacks", sol[kind]["fallback_events"])
+              "fallbacks", sol[kind]["fallback_events"], flush=True)
         assert sol[kind]["converged"] == 1
+    assert sol[PRECOND_LEARNED]["iterations"] <= 1.1 * sol[PRECOND_MG]["iterations"] + 1
 
 
 def _boundary_mask(n):
