@@ -1,20 +1,24 @@
-"""The two learned-preconditioner configurations at their stated sizes, on one GPU.
+"""The learned-preconditioner configurations and the polycrystal at their stated sizes, on one GPU.
 
-* training: 2000 (residual, error) pairs from the plain solver on 128^3-cell (129^3-node)
-  two-phase fields (Gaussian random field, correlation length 8 cells, thresholded at 0,
-  lambda = mu per phase, contrast 1:100), 200 pairs on each of 10 fields, one Adam step per pair,
-  by the library's own BF16 forward and backward kernels (alamo_net_train);
+* training: (residual, error) pairs from the plain solver on 128^3-cell (129^3-node) two-phase
+  fields (Gaussian random field, correlation length 8 cells, erf interface w = 0.7 as in the
+  benchmark, lambda = mu per phase, contrast 1:100), --pairs pairs on each of --fields fields, one
+  Adam step per pair, by the library's own BF16 forward and backward kernels (alamo_net_train).
+  Thresholded fields (w = 0) are not used: at contrasts of 1:100 and above their strong-form
+  operator has eigenvalues with negative real part, and no multigrid cycle converges on them;
 * evaluation: a 512^3-cell (513^3-node) two-phase field (correlation length 16, contrast 1:1000,
-  seed 5): plain FP64 multigrid (V-cycles), mixed-precision FGMRES, and the learned-preconditioned
-  mixed FGMRES with the network above, all to |b - A u|_inf / |b|_inf <= 1e-10 in FP64; iterations,
-  wall time and the relative L2 difference to the FP64 FGMRES solution.
+  seed 5, w = 0.7): plain FP64 multigrid (V-cycles; the reference's damped point Jacobi and the l1
+  smoother), FP64 and mixed FGMRES and the learned-preconditioned mixed FGMRES with the network
+  above (l1 V-cycle, the benchmark's settings), all to |b - A u|_inf / |b|_inf <= 1e-10 in FP64;
+  iterations, wall time and the relative L2 difference to the FP64 FGMRES solution.
 
     python -m alamo_amd.config_runs [--fields 10 --pairs 200] [--eval-n 512]
     python -m alamo_amd.config_runs --polycrystal [--poly-n 256]
 
-The second form runs the mixed-precision polycrystal configuration instead: 256^3 cells, 64 Voronoi
-grains with rotated cubic stiffness (C11 1.68, C12 1.21, C44 0.75), u_z = 1e-3 on top, bottom fixed,
-Neumann sides; FP64 and mixed FGMRES to 1e-10, their relative L2 difference.
+The second form runs the polycrystal configuration instead: 256^3 cells, 64 Voronoi grains with
+rotated cubic stiffness (C11 1.68, C12 1.21, C44 0.75), u_z = 1e-3 on top, bottom fixed,
+traction-free sides (--lateral neumann for grad(u).n = 0); FP64 and mixed FGMRES to 1e-10, their
+relative L2 difference.
 
 prints progress on stderr and one JSON line on stdout.  It takes minutes of GPU time, so it is
 separate from bench.py.
@@ -33,18 +37,22 @@ def main():
     p.add_argument("--contrast-train", type=float, default=100.0)
     p.add_argument("--eval-n", type=int, default=512)
     p.add_argument("--corr-eval", type=float, default=16.0)
-    p.add_argument("--restart", type=int, default=8)
-    p.add_argument("--sweeps", type=int, default=3, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--width", type=float, default=0.7, help="erf interface parameter of the two-phase fields")
+    p.add_argument("--restart", type=int, default=16)
+    p.add_argument("--sweeps", type=int, default=4, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
+    p.add_argument("--smoother", default="l1", choices=["jacobi", "l1"], help="smoother of the FGMRES V-cycle")
+    p.add_argument("--lateral", default="traction", choices=["traction", "neumann"],
+                   help="lateral faces of the polycrystal")
     p.add_argument("--sweeps-finest", type=int, default=1,
                    help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --sweeps)")
     p.add_argument("--bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES V-cycle (plain multigrid keeps bicgstab)")
     p.add_argument("--polycrystal", action="store_true")
     p.add_argument("--poly-n", type=int, default=256)
-    # the polycrystal (MajorMinor, Neumann sides) wants longer restarts than the two-phase field:
-    # restart 16 with 2 finest-level sweeps (one sweep at restart 8 does not reach 1e-10 in 400
-    # iterations)
+    # the polycrystal (MajorMinor, point Jacobi) takes restart 16 with 3 sweeps per call, 2 on the
+    # finest level (one finest-level sweep at restart 8 does not reach 1e-10 in 400 iterations)
     p.add_argument("--poly-restart", type=int, default=16)
+    p.add_argument("--poly-sweeps", type=int, default=3)
     p.add_argument("--poly-sweeps-finest", type=int, default=2)
     a = p.parse_args()
     if a.polycrystal:
@@ -63,9 +71,10 @@ def main():
     net = Net(seed=2024)
     losses = []
     for f in range(a.fields):
-        pr = P.two_phase_problem(n=a.train_n, corr_cells=8, contrast=a.contrast_train, seed=1 + f)
+        pr = P.two_phase_problem(n=a.train_n, corr_cells=8, contrast=a.contrast_train, seed=1 + f, width=a.width)
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=PREC_MIXED,
-                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest)
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest,
+                     smoother=a.smoother, bottom_solver=a.bottom)
         op.SetModel(pr["C"])
         st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f, solve_max_iter=100, solve_tol=1e-6)
         losses.append((st["first_loss"], st["last_loss"]))
@@ -80,19 +89,21 @@ def main():
     # ---- evaluation -------------------------------------------------------------------------
     n = a.eval_n
     log("generating the %d^3-cell evaluation field" % n)
-    pr = P.two_phase_problem(n=n, corr_cells=a.corr_eval, contrast=1000.0, seed=5)
+    pr = P.two_phase_problem(n=n, corr_cells=a.corr_eval, contrast=1000.0, seed=5, width=a.width)
     N = (n + 1) ** 3
     C = torch.from_numpy(pr["C"]).to(dev)
     F0 = torch.from_numpy(pr["F0"]).to(dev)
     del pr["C"], pr["F0"], pr["eta"]
     rhs, xs = None, {}
-    methods = [("fp64-mg", PREC_FP64, None), ("fp64-fgmres", PREC_FP64, PRECOND_MG),
-               ("mixed-fgmres", PREC_MIXED, PRECOND_MG), ("learned-mixed-fgmres", PREC_MIXED, PRECOND_LEARNED)]
-    for name, prec, kind in methods:
+    methods = [("fp64-mg-jacobi", PREC_FP64, None, "jacobi"), ("fp64-mg-l1", PREC_FP64, None, "l1"),
+               ("fp64-fgmres", PREC_FP64, PRECOND_MG, a.smoother), ("mixed-fgmres", PREC_MIXED, PRECOND_MG, a.smoother),
+               ("learned-mixed-fgmres", PREC_MIXED, PRECOND_LEARNED, a.smoother)]
+    for name, prec, kind, sm in methods:
         krylov = kind is not None
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_V,
-                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=400,
-                     bottom_solver=a.bottom if krylov else "bicgstab", sweeps_finest=a.sweeps_finest if krylov else 0)
+                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=1000,
+                     bottom_solver=a.bottom if krylov else "bicgstab", sweeps_finest=a.sweeps_finest if krylov else 0,
+                     smoother=sm)
         op.SetModel(C, layout=0)
         if rhs is None:
             b = torch.zeros(3 * N, dtype=torch.float64, device=dev)
@@ -106,7 +117,7 @@ def main():
         torch.cuda.synchronize()
         ts = time.perf_counter()
         op.prepareForSolve()
-        s = Krylov(op, restart=a.restart, precond=kind, max_iter=400) if krylov else Linear(op)
+        s = Krylov(op, restart=a.restart, precond=kind, max_iter=1000) if krylov else Linear(op)
         st = s.solve(sol, rhs, 1e-10, 0.0, raise_on_fail=False)
         torch.cuda.synchronize()
         sec = time.perf_counter() - ts
@@ -118,7 +129,9 @@ def main():
         del op, s
         torch.cuda.empty_cache()
     ref = xs["fp64-fgmres"]
-    for name in xs:
+    for name in list(xs):
+        if not out["evaluation"][name]["converged"]:
+            continue
         out["evaluation"][name]["rel_l2_vs_fp64_fgmres"] = float(torch.linalg.norm(xs[name] - ref) /
                                                                  torch.linalg.norm(ref))
     out["evaluation"]["workload"] = ("two-phase %d^3 cells, corr %g, contrast 1:1000, seed 5, Dirichlet box, "
@@ -132,12 +145,13 @@ def polycrystal(a):
     from .operator import Elastic, Krylov, PREC_MIXED, PREC_FP64, CYCLE_V, PRECOND_MG
     dev = torch.device("cuda:0")
     n = a.poly_n
-    pr = P.polycrystal_problem(n=n, ngrains=64, seed=64)
+    lateral = P.TRACTION if a.lateral == "traction" else P.NEUMANN
+    pr = P.polycrystal_problem(n=n, ngrains=64, seed=64, lateral=lateral)
     N = (n + 1) ** 3
     out, xs, rhs = {}, {}, None
     for name, prec in (("fp64-fgmres", PREC_FP64), ("mixed-fgmres", PREC_MIXED)):
         op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=prec,
-                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, max_iter=400,
+                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.poly_sweeps, max_iter=400,
                      bottom_solver=a.bottom, sweeps_finest=a.poly_sweeps_finest)
         op.SetModelGrains(pr["gid"], pr["table"])
         if rhs is None:
@@ -161,8 +175,8 @@ def polycrystal(a):
         del op
     out["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(xs["mixed-fgmres"] - xs["fp64-fgmres"]) /
                                         torch.linalg.norm(xs["fp64-fgmres"]))
-    out["workload"] = ("polycrystal %d^3 cells, 64 grains (seed 64), cubic 1.68/1.21/0.75, u_z 1e-3, 1e-10, %s bottom"
-                       % (n, a.bottom))
+    out["workload"] = ("polycrystal %d^3 cells, 64 grains (seed 64), cubic 1.68/1.21/0.75, u_z 1e-3, %s sides, "
+                       "1e-10, %s bottom" % (n, a.lateral, a.bottom))
     print(json.dumps({"polycrystal": out}))
 
 

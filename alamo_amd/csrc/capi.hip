@@ -495,7 +495,9 @@ static bool dense_invert(const std::vector<double>& A, int64_t n, std::vector<do
 
 // solver.bottom_solver "dense": assemble the coarsest FP64 operator (with its psi and boundary
 // rows) column by column on the device, invert it on the host and keep the inverse on the device.
-static int dense_bottom_factor(alamo_mg* h) {
+// *singular: the operator has no inverse (nothing on the device is changed then).
+static int dense_bottom_factor(alamo_mg* h, bool* singular) {
+    *singular = false;
     LevelBuf& lb = h->L.back();
     const int64_t n = 3 * lb.g.tot;
     if (n > ALAMO_DENSE_BOTTOM_MAX)
@@ -538,9 +540,10 @@ static int dense_bottom_factor(alamo_mg* h) {
         HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
-    if (!dense_invert(A, n, inv))
-        FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest-level operator is singular (no displacement "
-                            "condition pins it); use bottom_solver bicgstab or smoother");
+    if (!dense_invert(A, n, inv)) {
+        *singular = true;
+        return ALAMO_OK;
+    }
     HIPCHK(hipMemcpyAsync(h->dense_inv, inv.data(), n * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return ALAMO_OK;
@@ -628,7 +631,32 @@ static int prepare(alamo_mg* h) {
         }
     }
     HIPCHK(hipGetLastError());
-    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) TRY(dense_bottom_factor(h));
+    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) {
+        bool singular = false;
+        TRY(dense_bottom_factor(h, &singular));
+        // A coarsest level whose own boundary rows leave it singular (2 cells per direction with
+        // traction-free sides: the one-sided traction rows of the two faces meet) ends the hierarchy
+        // one level earlier, as long as that level still fits the dense bottom.
+        while (singular && h->L.size() > 1 && 3 * h->L[h->L.size() - 2].g.tot <= ALAMO_DENSE_BOTTOM_MAX) {
+            if (h->d.verbose)
+                fprintf(stderr, "alamo: coarsest level (%d cells) singular; the hierarchy ends at %d cells\n",
+                        h->L.back().g.n[0], h->L[h->L.size() - 2].g.n[0]);
+            h->L.pop_back();
+            // the new bottom level needs the FP64 cycle buffers of the FP64 bottom solve, which
+            // mixed-precision handles allocate on level 0 and the original bottom only
+            LevelBuf& nb = h->L.back();
+            const int64_t v = 3 * nb.g.tot;
+            if (!nb.cor) TRY(dalloc(h, &nb.cor, v));
+            if (!nb.res) TRY(dalloc(h, &nb.res, v));
+            if (!nb.tmp) TRY(dalloc(h, &nb.tmp, v));
+            // and a BiCGStab workspace of its size, should the bottom solver be switched later
+            if (nb.g.tot <= 17 * 17 * 17) TRY(dalloc(h, &h->bottom_work, 8 * v));
+            TRY(dense_bottom_factor(h, &singular));
+        }
+        if (singular)
+            FAIL(ALAMO_ERR_ARG, "bottom_solver dense: the coarsest-level operator is singular (no displacement "
+                                "condition pins it); use bottom_solver bicgstab or smoother");
+    }
     h->prepared = true;
     return ALAMO_OK;
 }

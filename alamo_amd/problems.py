@@ -136,12 +136,12 @@ def bc_tension_test(kind="uniaxial_stress", disp=None, trac=None):
 def bc_top_pull(uz, lateral=NEUMANN):
     """Boundary table of the polycrystal workload: z=0 clamped (u=0), z=1 gripped with
     u = (0, 0, uz); the four lateral faces and the lateral edges use `lateral`
-    (Neumann grad(u).n = 0 by default).  Regions touching a z face follow the z-face rule.
+    (Neumann grad(u).n = 0 by default, TRACTION for traction-free sides).  Regions touching a
+    z face follow the z-face rule.
 
-    Traction-free lateral faces are a valid choice of `lateral`, but with rotated cubic
-    grains the reference's point-Jacobi smoother diverges on such rows (the traction row
-    of one component couples to the other components at the same node with weights of
-    the size of its own diagonal), so the workload uses Neumann."""
+    With traction-free sides the re-discretised operator of a 2-cell multigrid level is
+    singular (the one-sided traction rows of opposite faces meet); the dense bottom solver
+    then ends the hierarchy at 4 cells, where the operator is regular."""
     t, v = bc_constant()
     t[:, :] = lateral
     for i, name in enumerate(FACES):
@@ -286,11 +286,11 @@ def grain_table(ngrains, seed, C11=1.68, C12=1.21, C44=0.75):
     return tab
 
 
-def polycrystal_problem(n=256, ngrains=64, seed=64, uz=1e-3):
+def polycrystal_problem(n=256, ngrains=64, seed=64, uz=1e-3, lateral=NEUMANN):
     ncell = (n, n, n)
     gid, _ = voronoi_grains(ncell, ngrains, seed)
     tab = grain_table(ngrains, seed)
-    bct, bcv = bc_top_pull(uz)
+    bct, bcv = bc_top_pull(uz, lateral)
     return dict(ncell=ncell, dx=(1.0 / n,) * 3, gid=gid, table=tab, bct=bct, bcv=bcv)
 
 

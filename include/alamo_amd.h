@@ -38,7 +38,9 @@ enum { ALAMO_CYCLE_V = 0, ALAMO_CYCLE_F = 1 };
 enum { ALAMO_BOTTOM_BICGSTAB = 0, /* Jacobi-preconditioned BiCGStab on the coarsest level (the default) */
        ALAMO_BOTTOM_SMOOTHER = 1, /* 8 Fsmooth calls from zero (amrex::MLMG's final_smooth default) */
        ALAMO_BOTTOM_DENSE = 2     /* direct: the coarsest operator assembled and inverted at prepare
-                                     (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle */ };
+                                     (at most ALAMO_DENSE_BOTTOM_MAX unknowns), one product per cycle; a
+                                     singular coarsest level (2 cells with traction-free sides) is dropped
+                                     while the next one fits (alamo_mg_num_levels reports the result) */ };
 #define ALAMO_DENSE_BOTTOM_MAX 1200
 #define ALAMO_NOALLOC_MAX_DOUBLES 8388608 /* 64 MiB: the device scratch of alamo_mg_fsmooth_noalloc */
 enum { ALAMO_SMOOTHER_JACOBI = 0, ALAMO_SMOOTHER_L1 = 1 };

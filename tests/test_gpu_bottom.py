@@ -16,7 +16,7 @@ import pytest
 
 from alamo_amd import problems as P
 from alamo_amd._lib import AlamoError
-from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, PRECOND_MG
+from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, PRECOND_MG, CYCLE_V
 from helpers import random_iso, random_mm, random_diag, to_dev, to_host, rng
 
 pytestmark = pytest.mark.gpu
@@ -119,3 +119,34 @@ def test_dense_bottom_error_paths(cuda):
     # the same operator solves with the default bottom solver
     big.UpdateOpts(bottom_solver="bicgstab")
     big.prepareForSolve()
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_singular_two_cell_level_is_dropped(cuda, ora, precision):
+    """Traction-free sides make the 2-cell level singular: the dense bottom ends the hierarchy at
+    4 cells (FP64 bottom buffers and a BiCGStab workspace of the new bottom level's size), the
+    solve matches the oracle, and switching to the BiCGStab bottom afterwards still solves."""
+    import torch
+    pr = P.polycrystal_problem(n=16, ngrains=8, seed=3, lateral=P.TRACTION)
+    N = 17 ** 3
+    C = pr["table"][pr["gid"].ravel()].ravel()
+    b = ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"])
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=precision,
+                 cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=3, bottom_solver="dense",
+                 max_iter=400)
+    op.SetModel(C, LAYOUT_AOS)
+    full = op.NMGLevels()
+    op.prepareForSolve()
+    print("levels", full, "->", op.NMGLevels(), flush=True)
+    assert op.NMGLevels() == full - 1 and op.LevelNCell(op.NMGLevels() - 1) == (4, 4, 4)
+    xs = []
+    for bottom in ("dense", "bicgstab"):
+        op.UpdateOpts(bottom_solver=bottom)
+        sol = to_dev(np.zeros(3 * N), cuda)
+        st = Krylov(op, restart=16, max_iter=400).solve(sol, to_dev(b, cuda), 1e-11, 0.0)
+        print(bottom, "iterations", st["iterations"], flush=True)
+        assert st["converged"] == 1
+        xs.append(to_host(sol))
+    r = ora.residual(pr["ncell"], pr["dx"], P.MAJORMINOR, C, pr["bct"], xs[1], b)
+    assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
+    assert np.linalg.norm(xs[0] - xs[1]) <= 1e-8 * np.linalg.norm(xs[0])
