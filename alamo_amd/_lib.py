@@ -122,6 +122,8 @@ SIGNATURES = {
     "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_net_mfma_probe32": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_net_wgrad_host": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_stream_probe": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p]),
     "alamo_amr_create": (c_int, [P(MGDesc), c_int, c_void_p, c_void_p, c_void_p, P(c_void_p)]),
     "alamo_amr_destroy": (c_int, [c_void_p]),
@@ -129,6 +131,7 @@ SIGNATURES = {
     "alamo_amr_masks": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, P(SolveStats)]),
     "alamo_net_layer_times": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "alamo_net_layer_times_bwd": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -1948,6 +1948,35 @@ int alamo_net_forward_host(alamo_net* n, const int* nnodes, const float* x, floa
     return rc;
 }
 
+// One layer's weight gradient on host data (a check of the kernel alone): dW[27][cout][ci] +=
+// sum_v dout[v][co] in[v + o][ci]; in [V][cin], dout [V][cout] bf16 bit patterns, channel-last.
+int alamo_net_wgrad_host(alamo_net* n, const int* nnodes, int cin, int cout, const uint16_t* in, const uint16_t* dout,
+                         float* dW) {
+    if (!n || !nnodes || !in || !dout || !dW) FAIL(ALAMO_ERR_ARG, "null argument");
+    const bool ok = (cin == 16 && (cout == 16 || cout == 32)) || (cin == 32 && (cout == 16 || cout == 64)) ||
+                    (cin == 64 && cout == 32);
+    if (!ok) FAIL(ALAMO_ERR_ARG, "channel pair not in the network (16-16, 16-32, 32-16, 32-64, 64-32)");
+    TRY(net_bufs(n, nnodes));
+    const net::Grid3 g = n->B.g[0];
+    const int64_t nw = (int64_t)27 * cin * cout;
+    uint16_t *din = nullptr, *ddo = nullptr;
+    float* ddw = nullptr;
+    HIPCHK(hipMalloc(&din, g.V * cin * 2));
+    HIPCHK(hipMalloc(&ddo, g.V * cout * 2));
+    HIPCHK(hipMalloc(&ddw, nw * 4));
+    HIPCHK(hipMemcpy(din, in, g.V * cin * 2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddo, dout, g.V * cout * 2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddw, dW, nw * 4, hipMemcpyHostToDevice));
+    wgrad_launch(n, g, cin, cout, din, ddo, ddw, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(n->s));
+    HIPCHK(hipMemcpy(dW, ddw, nw * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(din);
+    (void)hipFree(ddo);
+    (void)hipFree(ddw);
+    return ALAMO_OK;
+}
+
 // Forward + backward on host data; gradients accumulate into the network; dx (may be NULL)
 // receives d(loss)/dx [V][16] for the given dy [V][16].
 int alamo_net_backward_host(alamo_net* n, const int* nnodes, const float* x, const float* dy, float* dx) {
@@ -2164,6 +2193,66 @@ int alamo_net_layer_times(alamo_net* n, alamo_mg* h, int reps, double* ms) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return ALAMO_OK;
+}
+
+// mean backward-kernel times of each layer (backward-data convolution, weight gradient with its
+// reduction, bias gradient excluded), HIP events on the network's stream
+int alamo_net_layer_times_bwd(alamo_net* n, alamo_mg* h, int reps, double* ms_data, double* ms_wgrad) {
+    if (!n || !h || !ms_data || !ms_wgrad || reps < 1) FAIL(ALAMO_ERR_ARG, "bad argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    TRY(net_level0(n, h));
+    TRY(net_sync_weights(n));
+    NetBufs& B = n->B;
+    // per layer: grid, cin, cout, forward input, a gradient-of-output buffer (bf16, cout channels)
+    // and an output buffer of cin f32 channels for the backward-data convolution
+    struct L { const net::Grid3* g; int cin, cout; const uint16_t* in; float* dx; };
+    const L layers[6] = {{&B.g[0], 16, 16, B.x, B.dx},   {&B.g[1], 16, 32, B.d1, B.dd1},
+                         {&B.g[2], 32, 64, B.d2, B.dd2}, {&B.g[2], 64, 32, B.a2, B.da2},
+                         {&B.g[1], 32, 16, B.u1, B.du1}, {&B.g[0], 16, 16, B.u0, B.du0}};
+    float* gscratch = nullptr;  // dW target of the timed weight gradients, discarded
+    HIPCHK(hipMalloc(&gscratch, (size_t)27 * 64 * 32 * 4));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int l = 0; l < 6; l++) {
+        const L& y = layers[l];
+        for (int pass = 0; pass < 2; pass++) {
+            auto run = [&] {
+                if (pass == 0) conv_launch(n, *y.g, y.cout, y.cin, B.gb, n->wb[l], nullptr, y.dx, true, 0);
+                else wgrad_launch(n, *y.g, y.cin, y.cout, y.in, B.gb, gscratch, nullptr);
+            };
+            run();  // warm
+            HIPCHK(hipEventRecord(e0, n->s));
+            for (int r = 0; r < reps; r++) run();
+            HIPCHK(hipEventRecord(e1, n->s));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            (pass == 0 ? ms_data : ms_wgrad)[l] = t / reps;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(gscratch);
+    return ALAMO_OK;
+}
+
+// MFMA lane-layout probe (one 16x16x32 bf16 product of host matrices, A 16 x 32, B 32 x 16)
+int alamo_net_mfma_probe32(const uint16_t* A, const uint16_t* B, float* D) {
+    uint16_t *a = nullptr, *b = nullptr;
+    float* d = nullptr;
+    HIPCHK(hipMalloc(&a, 1024));
+    HIPCHK(hipMalloc(&b, 1024));
+    HIPCHK(hipMalloc(&d, 1024));
+    HIPCHK(hipMemcpy(a, A, 1024, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b, B, 1024, hipMemcpyHostToDevice));
+    net::k_mfma_probe32<<<1, 64>>>(a, b, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(D, d, 1024, hipMemcpyDeviceToHost));
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(d);
     return ALAMO_OK;
 }
 

@@ -230,59 +230,148 @@ __global__ void __launch_bounds__(64 * CL_TY) k_conv3_fwd_lds(Grid3 g, int xb, c
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[o][co][ci] += sum_v dout[v][co] in[v + o][ci].  GEMM with
-// M = co, N = ci, K = voxels; one wavefront owns one (o, co tile, ci tile) and a chunk
-// of voxels, and adds its 16x16 result into dW with float atomics.
+// Weight gradient as an LDS-staged implicit GEMM on v_mfma_f32_16x16x32_bf16 (gfx950), two
+// passes, no float atomics:
+//   pass 1 (k_conv3_wgrad_lds): workgroup b accumulates part[b][o][co][ci] = sum over the voxels of
+//     its tiles of dout[v][co] in[v + o][ci];
+//   pass 2 (k_wgrad_reduce): dW[i] += sum_b part[b][i] in a fixed order (deterministic).
+// A tile is 32 voxels in x (one MFMA K chunk) x WG_TY rows x WG_TZ planes; the workgroup stages
+// dout of the tile (all Cout channels) and the 16-channel input slice ci0 .. ci0 + 15 of the tile
+// plus its one-voxel halo in LDS, channel-major (voxels of a row contiguous), so that a lane's 8
+// consecutive voxels of one channel -- its MFMA operand -- are one 16-byte LDS read; the x shift
+// of an offset (dx = -1, +1) reads two aligned 8-voxel groups and funnels them by one element.
+// Operand layout of the 16x16x32 product (checked by tests/test_gpu_learned.py):
+//   A (16 x 32, M x K): lane l holds A[l % 16][8 (l / 16) + 0..7]
+//   B (32 x 16, K x N): lane l holds B[8 (l / 16) + 0..7][l % 16]
+//   D (16 x 16):        lane l holds D[4 (l / 16) + i][l % 16], i = 0..3
+// Here M = co (A = dout^T of a 32-voxel row), N = ci (B = the row's input shifted by o), K = voxels.
+// Wavefront w of the 4 owns the offsets o = w, w + 4, ... (7 or 6 of the 27) for every co tile.
 // ---------------------------------------------------------------------------
-constexpr int WGRAD_CHUNK = 2048;  // voxels per wavefront
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+constexpr int WG_TY = 4, WG_TZ = 2;
+constexpr int WG_ROWS = WG_TY * WG_TZ;                  // K chunks (rows of 32 voxels) per tile
+constexpr int WG_SD = WG_ROWS * 32 + 8;                 // dout: elements per channel (+8: bank spread)
+constexpr int WG_HR = (WG_TY + 2) * (WG_TZ + 2);        // input rows of a tile with its halo
+constexpr int WG_SI = WG_HR * 48 + 8;                   // input: elements per channel; x at 8 + (x - x0)
+constexpr int WG_NWG = 256;                              // pass-1 workgroups at most
 
-__global__ void __launch_bounds__(256) k_conv3_wgrad(Grid3 g, int Cin, int Cout, const uint16_t* __restrict__ in,
-                                                     const uint16_t* __restrict__ dout, float* __restrict__ dW) {
-    const int lane = threadIdx.x % 64;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-    const int NCO = Cout / 16, NCI = Cin / 16;
-    const int64_t nchunk = (g.V + WGRAD_CHUNK - 1) / WGRAD_CHUNK;
-    const int64_t njob = (int64_t)27 * NCO * NCI * nchunk;
-    if (wid >= njob) return;
-    const int64_t chunk = wid % nchunk;
-    int64_t r = wid / nchunk;
-    const int cit = (int)(r % NCI);
-    r /= NCI;
-    const int cot = (int)(r % NCO);
-    const int o = (int)(r / NCO);
-    int dx, dy, dz;
-    off3(o, dx, dy, dz);
-    const int m = lane % 16, kq = 4 * (lane / 16);
-    f4 acc = f4{0.f, 0.f, 0.f, 0.f};
-    const int64_t v0 = chunk * WGRAD_CHUNK;
-    const int64_t v1 = v0 + WGRAD_CHUNK < g.V ? v0 + WGRAD_CHUNK : g.V;
-    for (int64_t vb = v0; vb < v1; vb += 16) {
-        s4 a, b;
+__device__ __forceinline__ uint4 funnel1(uint4 lo, uint4 hi, bool up) {
+    // the 8 elements starting one element after (up) or before (!up) group `up ? lo : hi`
+    if (up)
+        return make_uint4(__builtin_amdgcn_alignbyte(lo.y, lo.x, 2), __builtin_amdgcn_alignbyte(lo.z, lo.y, 2),
+                          __builtin_amdgcn_alignbyte(lo.w, lo.z, 2), __builtin_amdgcn_alignbyte(hi.x, lo.w, 2));
+    return make_uint4(__builtin_amdgcn_alignbyte(hi.x, lo.w, 2), __builtin_amdgcn_alignbyte(hi.y, hi.x, 2),
+                      __builtin_amdgcn_alignbyte(hi.z, hi.y, 2), __builtin_amdgcn_alignbyte(hi.w, hi.z, 2));
+}
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv3_wgrad_lds(Grid3 g, const uint16_t* __restrict__ in,
+                                                         const uint16_t* __restrict__ dout, float* __restrict__ part) {
+    constexpr int NCO = COUT / 16, NCI = CIN / 16, NJ = 7;
+    __shared__ __align__(16) uint16_t sd[COUT * WG_SD];
+    __shared__ __align__(16) uint16_t si[16 * WG_SI];
+    const int t = threadIdx.x, lane = t % 64, w = t / 64;
+    const int m = lane % 16, q = lane / 16;
+    const int tx = (g.N[0] + 31) / 32, ty = (g.N[1] + WG_TY - 1) / WG_TY, tz = (g.N[2] + WG_TZ - 1) / WG_TZ;
+    const int64_t ntile = (int64_t)tx * ty * tz;
+    float* pb = part + (int64_t)blockIdx.x * 27 * CIN * COUT;
+    for (int cit = 0; cit < NCI; cit++) {
+        f4 acc[NJ][NCO];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int64_t v = vb + kq + q;
-            short av = 0, bv = 0;
-            if (v < v1) {
-                av = (short)dout[v * Cout + cot * 16 + m];
-                const int x = (int)(v % g.N[0]);
-                const int y = (int)((v / g.N[0]) % g.N[1]);
-                const int z = (int)(v / ((int64_t)g.N[0] * g.N[1]));
-                const int xs = x + dx, ys = y + dy, zs = z + dz;
-                if (xs >= 0 && xs < g.N[0] && ys >= 0 && ys < g.N[1] && zs >= 0 && zs < g.N[2])
-                    bv = (short)in[(xs + (int64_t)g.N[0] * (ys + (int64_t)g.N[1] * zs)) * Cin + cit * 16 + m];
+        for (int j = 0; j < NJ; j++)
+#pragma unroll
+            for (int c = 0; c < NCO; c++) acc[j][c] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+            const int x0 = 32 * (int)(tile % tx);
+            const int y0 = WG_TY * (int)((tile / tx) % ty);
+            const int z0 = WG_TZ * (int)(tile / ((int64_t)tx * ty));
+            __syncthreads();  // the previous tile's reads of sd / si are done
+            // dout of the tile: item = (voxel of the 32 x WG_ROWS tile, 8-channel chunk)
+            for (int it = t; it < 32 * WG_ROWS * (COUT / 8); it += 256) {
+                const int ch = it % (COUT / 8), vox = it / (COUT / 8);
+                const int xx = vox % 32, r = vox / 32;
+                const int x = x0 + xx, y = y0 + r % WG_TY, z = z0 + r / WG_TY;
+                uint4 d = make_uint4(0, 0, 0, 0);
+                if (x < g.N[0] && y < g.N[1] && z < g.N[2])
+                    d = *(const uint4*)(dout + (x + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z)) * COUT + 8 * ch);
+                const uint32_t dd[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    sd[(8 * ch + e) * WG_SD + r * 32 + xx] = (uint16_t)(dd[e / 2] >> (16 * (e % 2)));
             }
-            a[q] = av;
-            b[q] = bv;
-        }
-        // A[co = m][k = voxel kq+q], B[k = voxel kq+q][ci = m]
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    // D[co = kq + i][ci = m]
+            // input slice of the tile with its halo: x0 - 1 .. x0 + 32 at LDS x = 7 .. 40
+            for (int it = t; it < 34 * WG_HR * 2; it += 256) {
+                const int ch = it % 2, vox = it / 2;
+                const int xx = vox % 34, hr = vox / 34;
+                const int x = x0 - 1 + xx, y = y0 - 1 + hr % (WG_TY + 2), z = z0 - 1 + hr / (WG_TY + 2);
+                uint4 d = make_uint4(0, 0, 0, 0);
+                if (x >= 0 && x < g.N[0] && y >= 0 && y < g.N[1] && z >= 0 && z < g.N[2])
+                    d = *(const uint4*)(in + (x + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z)) * CIN + 16 * cit +
+                                        8 * ch);
+                const uint32_t dd[4] = {d.x, d.y, d.z, d.w};
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int co = cot * 16 + kq + i, ci = cit * 16 + m;
-        atomicAdd(&dW[((int64_t)o * Cout + co) * Cin + ci], acc[i]);
+                for (int e = 0; e < 8; e++)
+                    si[(8 * ch + e) * WG_SI + hr * 48 + 7 + xx] = (uint16_t)(dd[e / 2] >> (16 * (e % 2)));
+            }
+            __syncthreads();
+            for (int r = 0; r < WG_ROWS; r++) {
+                const int ry = r % WG_TY, rz = r / WG_TY;
+                bf8 a[NCO];
+#pragma unroll
+                for (int c = 0; c < NCO; c++)
+                    a[c] = __builtin_bit_cast(bf8, *(const uint4*)&sd[(16 * c + m) * WG_SD + r * 32 + 8 * q]);
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int o = w + 4 * j;
+                    if (o >= 27) break;  // uniform per wavefront
+                    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+                    const int hr = (ry + 1 + dy) + (WG_TY + 2) * (rz + 1 + dz);
+                    const uint16_t* sb = &si[m * WG_SI + hr * 48 + 8 * q];
+                    uint4 bv;
+                    if (dx == 0) bv = *(const uint4*)(sb + 8);
+                    else if (dx > 0) bv = funnel1(*(const uint4*)(sb + 8), *(const uint4*)(sb + 16), true);
+                    else bv = funnel1(*(const uint4*)sb, *(const uint4*)(sb + 8), false);
+                    const bf8 b = __builtin_bit_cast(bf8, bv);
+#pragma unroll
+                    for (int c = 0; c < NCO; c++) acc[j][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], b, acc[j][c], 0, 0, 0);
+                }
+            }
+        }
+        // D[co = 16 c + 4 q + i][ci = 16 cit + m] of offset o
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int o = w + 4 * j;
+            if (o >= 27) break;
+#pragma unroll
+            for (int c = 0; c < NCO; c++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    pb[((int64_t)o * COUT + 16 * c + 4 * q + i) * CIN + 16 * cit + m] = acc[j][c][i];
+        }
     }
+}
+
+// dW[i] += sum over the nwg partials (fixed order)
+__global__ void __launch_bounds__(256) k_wgrad_reduce(int nwg, int64_t n, const float* __restrict__ part,
+                                                      float* __restrict__ dW) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int b = 0; b < nwg; b++) s += part[(int64_t)b * n + i];
+    dW[i] += s;
+}
+
+// Probe of the 16x16x32 operand layout: D = A B for row-major bf16 A (16 x 32) and B (32 x 16).
+__global__ void k_mfma_probe32(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, float* __restrict__ D) {
+    const int l = threadIdx.x;
+    uint16_t a[8], b[8];
+    for (int i = 0; i < 8; i++) {
+        a[i] = A[(l % 16) * 32 + 8 * (l / 16) + i];
+        b[i] = B[(8 * (l / 16) + i) * 16 + l % 16];
+    }
+    f4 c = f4{0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+    for (int i = 0; i < 4; i++) D[(4 * (l / 16) + i) * 16 + l % 16] = c[i];
 }
 
 // dB[co] += sum_v dout[v][co]; Cout divides 256: thread t reads channel t % Cout of voxel

@@ -37,6 +37,7 @@ struct NetBufs {
     uint16_t* gb = nullptr;  // bf16 staging of a gradient fed to a convolution (largest: 32 ch at level 0 size)
     float* feat = nullptr;   // coefficient features at level 0
     float* loss = nullptr;
+    float* wpart = nullptr;  // per-workgroup weight-gradient partials (net::WG_NWG x 27 x 64 x 32)
     std::vector<void*> allocs;
 };
 
@@ -118,6 +119,7 @@ static int net_bufs(alamo_net* n, const int* N0) {
     NALLOC(A, B.gb, std::max(V0 * 16, V1 * 64) * 2);
     NALLOC(A, B.feat, V0 * 2 * 4);
     NALLOC(A, B.loss, 16);
+    NALLOC(A, B.wpart, (int64_t)net::WG_NWG * 27 * 64 * 32 * 4);
     return ALAMO_OK;
 }
 
@@ -167,12 +169,28 @@ static void conv_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, co
     else conv_launch_nt<4>(n, g, cin, in, W, bias, out, out_f32, relu);
 }
 
+template <int CIN, int COUT>
+static void wgrad_pass(alamo_net* n, const net::Grid3& g, const uint16_t* in, const uint16_t* dout, float* dW) {
+    const int64_t ntile = (int64_t)((g.N[0] + 31) / 32) * ((g.N[1] + net::WG_TY - 1) / net::WG_TY) *
+                          ((g.N[2] + net::WG_TZ - 1) / net::WG_TZ);
+    const int nwg = (int)std::min<int64_t>(ntile, net::WG_NWG);
+    net::k_conv3_wgrad_lds<CIN, COUT><<<nwg, 256, 0, n->s>>>(g, in, dout, n->B.wpart);
+    const int64_t nw = (int64_t)27 * CIN * COUT;
+    net::k_wgrad_reduce<<<(unsigned)((nw + 255) / 256), 256, 0, n->s>>>(nwg, nw, n->B.wpart, dW);
+}
+
+// dW += weight gradient, dB += bias gradient of one convolution layer (in: its forward input,
+// dout: the gradient of its output, both bf16 channel-last)
 static void wgrad_launch(alamo_net* n, const net::Grid3& g, int cin, int cout, const uint16_t* in,
                          const uint16_t* dout, float* dW, float* dB) {
-    const int64_t nchunk = (g.V + net::WGRAD_CHUNK - 1) / net::WGRAD_CHUNK;
-    const int64_t jobs = (int64_t)27 * (cout / 16) * (cin / 16) * nchunk;
-    net::k_conv3_wgrad<<<(unsigned)((jobs + 3) / 4), 256, 0, n->s>>>(g, cin, cout, in, dout, dW);
-    net::k_bias_grad<<<(unsigned)std::min<int64_t>(2048, (g.V * cout + 255) / 256), 256, 0, n->s>>>(g.V, cout, dout, dB);
+    if (cin == 16 && cout == 16) wgrad_pass<16, 16>(n, g, in, dout, dW);
+    else if (cin == 16 && cout == 32) wgrad_pass<16, 32>(n, g, in, dout, dW);
+    else if (cin == 32 && cout == 16) wgrad_pass<32, 16>(n, g, in, dout, dW);
+    else if (cin == 32 && cout == 64) wgrad_pass<32, 64>(n, g, in, dout, dW);
+    else wgrad_pass<64, 32>(n, g, in, dout, dW);
+    if (dB)
+        net::k_bias_grad<<<(unsigned)std::min<int64_t>(2048, (g.V * cout + 255) / 256), 256, 0, n->s>>>(g.V, cout, dout,
+                                                                                                         dB);
 }
 
 static int net_sync_weights(alamo_net* n) {

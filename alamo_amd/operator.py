@@ -697,6 +697,14 @@ class Net:
         check(lib().alamo_net_layer_times(self.h, op.h, int(reps), ms.ctypes.data_as(ctypes.c_void_p)))
         return ms
 
+    def layer_times_bwd(self, op, reps=5):
+        """mean milliseconds of each conv layer's backward-data convolution and weight gradient at op's
+        level-0 size: (ms_data[6], ms_wgrad[6])"""
+        md, mw = np.zeros(6), np.zeros(6)
+        check(lib().alamo_net_layer_times_bwd(self.h, op.h, int(reps), md.ctypes.data_as(ctypes.c_void_p),
+                                              mw.ctypes.data_as(ctypes.c_void_p)))
+        return md, mw
+
     def save(self, path):
         check(lib().alamo_net_save(self.h, str(path).encode()))
 

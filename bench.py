@@ -258,7 +258,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": val * 1e3,
+        "ms_per_step": val * 1e3 if val is not None else None,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

@@ -333,6 +333,10 @@ int alamo_net_zero_grads(alamo_net* net);
 /* x, y, dy, dx: host [V][16] float arrays on a grid of nnodes[3] nodes (inputs rounded to bf16) */
 int alamo_net_forward_host(alamo_net* net, const int* nnodes, const float* x, float* y);
 int alamo_net_backward_host(alamo_net* net, const int* nnodes, const float* x, const float* dy, float* dx);
+/* one layer's weight gradient alone: dW[27][cout][cin] += sum_v dout[v][co] in[v + o][ci] (zero
+ * outside the grid); in [V][cin], dout [V][cout] bf16 bit patterns, host arrays */
+int alamo_net_wgrad_host(alamo_net* net, const int* nnodes, int cin, int cout, const uint16_t* in,
+                         const uint16_t* dout, float* dW);
 int alamo_net_adam_step(alamo_net* net, double lr);
 int alamo_net_train(alamo_net* net, alamo_mg* h, const alamo_train_opts* opts, alamo_train_stats* stats);
 int alamo_net_save(alamo_net* net, const char* path);
@@ -345,8 +349,13 @@ int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z);
 /* mean forward-kernel time of each of the 6 convolution layers at h's level-0 size, HIP events on
  * h's stream, reps launches each: ms[6] (for throughput and roofline reports) */
 int alamo_net_layer_times(alamo_net* net, alamo_mg* h, int reps, double* ms);
+/* mean time of each layer's backward-data convolution and weight gradient (the two-pass reduction
+ * included, the bias gradient not) at h's level-0 size, HIP events, reps launches each: ms[6] each */
+int alamo_net_layer_times_bwd(alamo_net* net, alamo_mg* h, int reps, double* ms_data, double* ms_wgrad);
 /* one 16x16x16 bf16 matrix-core product of row-major host matrices (lane-layout check) */
 int alamo_net_mfma_probe(const uint16_t* A, const uint16_t* B, float* D);
+/* one 16x16x32 bf16 product (gfx950): A 16 x 32, B 32 x 16 row-major host matrices, D 16 x 16 */
+int alamo_net_mfma_probe32(const uint16_t* A, const uint16_t* B, float* D);
 
 /* Attainable-bandwidth calibration for roofline reports (no counterpart in the reference): o[v n + q]
  * = a[v n + q] a[3 n + q] + a[4 n + q], v = 0..2, for q < n -- the isotropic FP64 Fapply's five

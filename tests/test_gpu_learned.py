@@ -54,6 +54,25 @@ def test_mfma_lane_layout(cuda):
     np.testing.assert_allclose(_probe(Ab, Bb), ref, rtol=0, atol=1e-5 * np.abs(ref).max())
 
 
+def _probe32(A, B):
+    from alamo_amd._lib import lib, check
+    import ctypes
+    D = np.zeros((16, 16), dtype=np.float32)
+    check(lib().alamo_net_mfma_probe32(A.ctypes.data_as(ctypes.c_void_p), B.ctypes.data_as(ctypes.c_void_p),
+                                        D.ctypes.data_as(ctypes.c_void_p)))
+    return D
+
+
+def test_mfma32_lane_layout(cuda):
+    """The operand/result lane layout the weight-gradient kernel assumes for gfx950's
+    v_mfma_f32_16x16x32_bf16 (A 16 x 32, B 32 x 16): integer-valued products are exact."""
+    r = np.random.default_rng(1)
+    A = r.integers(-8, 9, (16, 32)).astype(np.float32)
+    B = r.integers(-8, 9, (32, 16)).astype(np.float32)
+    D = _probe32(np.ascontiguousarray(_bf16_bits(A)), np.ascontiguousarray(_bf16_bits(B)))
+    np.testing.assert_array_equal(D, (A.astype(np.float64) @ B.astype(np.float64)).astype(np.float32))
+
+
 def _random_params(ora, seed):
     r = np.random.default_rng(seed)
     p = np.zeros(ora.net_nparam(), dtype=np.float32)
@@ -106,6 +125,31 @@ def test_net_backward_parity(cuda, ora, N0):
         print("layer", l, "dW rel err", ew, "dB rel err", eb)
         assert ew <= 5e-3 and eb <= 5e-3, l
     assert _rel(dx, dxref) <= 5e-3
+
+
+@pytest.mark.parametrize("N0", [(9, 13, 5), (65, 33, 33)])
+@pytest.mark.parametrize("cin,cout", [(16, 16), (16, 32), (32, 16), (32, 64), (64, 32)])
+def test_weight_gradient_kernel(cuda, ora, N0, cin, cout):
+    """The weight-gradient kernel alone on exact bf16 inputs against the oracle's FP64 sum, added
+    onto a nonzero dW; (65, 33, 33) has more 32 x 4 x 2-voxel tiles (459) than the kernel has
+    workgroups, so workgroups sum several tiles before the two-pass reduction."""
+    import ctypes
+    from alamo_amd._lib import lib, check
+    V = int(np.prod(N0))
+    r = np.random.default_rng(cin + 7 * cout + V)
+    xb = _bf16_bits(r.standard_normal((V, cin)))
+    db = _bf16_bits(r.standard_normal((V, cout)))
+    dW0 = r.standard_normal((27, cout, cin)).astype(np.float32)
+    dW = dW0.copy()
+    net = Net(seed=1)
+    print("wgrad kernel", N0, cin, cout, flush=True)
+    check(lib().alamo_net_wgrad_host(net.h, np.asarray(N0, np.int32).ctypes.data_as(ctypes.c_void_p), cin, cout,
+                                     xb.ctypes.data_as(ctypes.c_void_p), db.ctypes.data_as(ctypes.c_void_p),
+                                     dW.ctypes.data_as(ctypes.c_void_p)))
+    ref, _ = ora.conv3_wgrad(N0, cin, cout, _bf16_val(xb), _bf16_val(db))
+    err = _rel(dW.astype(np.float64) - dW0, ref)
+    print("rel L2 err", err)
+    assert err <= 1e-5
 
 
 def test_adam_matches_torch(cuda, ora):
