@@ -123,6 +123,10 @@ SIGNATURES = {
     "alamo_amr_fixup_residual_mask": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe": (c_int, [c_void_p, c_void_p, c_void_p]),
     "alamo_net_mfma_probe32": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "alamo_write_plotfile": (c_int, [ctypes.c_char_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                     c_void_p, c_void_p, c_double, c_void_p]),
+    "alamo_write_plotfile_host": (c_int, [ctypes.c_char_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p]),
     "alamo_net_wgrad_host": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "alamo_stream_probe": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p]),
     "alamo_amr_create": (c_int, [P(MGDesc), c_int, c_void_p, c_void_p, c_void_p, P(c_void_p)]),

@@ -136,3 +136,41 @@ class Composite:
         if raise_on_fail:
             check(rc)
         return st.final_res_inf, st
+
+
+def write_plotfile(path, ncell0, prob_lo, prob_hi, boxes, names, fields, time=0.0, steps=None):
+    """AMReX plotfile of nodal fields (Integrator::WritePlotFile's "<prefix>node" output,
+    Integrator.cpp:808-1054): ``fields[l]`` holds ``len(names)`` SoA planes of level l's
+    ``(ncell0 2^l + 1)^3`` nodes, either all numpy arrays (host) or all device tensors;
+    ``boxes[l - 1]`` are level l's inclusive cell boxes ``(lo, hi)`` (level 0: the whole domain)."""
+    nl = len(fields)
+    if len(boxes) != nl - 1:
+        raise ValueError("boxes must list the boxes of every level >= 1")
+    nbox = np.array([0] + [len(bl) for bl in boxes], dtype=np.int32)
+    lo = np.array([b[0] for bl in boxes for b in bl] or [(0, 0, 0)], dtype=np.int32).reshape(-1)
+    hi = np.array([b[1] for bl in boxes for b in bl] or [(0, 0, 0)], dtype=np.int32).reshape(-1)
+    nm = [s.encode() for s in names]
+    cnames = (ctypes.c_char_p * len(nm))(*nm)
+    host = all(isinstance(f, np.ndarray) for f in fields)
+    keep = []
+    if host:
+        for f in fields:
+            keep.append(np.ascontiguousarray(f, dtype=np.float64))
+        ptrs = (ctypes.c_void_p * nl)(*[k.ctypes.data for k in keep])
+    else:
+        for f in fields:
+            if not (f.is_cuda and f.is_contiguous() and str(f.dtype) == "torch.float64"):
+                raise ValueError("device fields must be contiguous float64 CUDA tensors")
+        ptrs = (ctypes.c_void_p * nl)(*[f.data_ptr() for f in fields])
+    for l, f in enumerate(keep if host else fields):
+        need = len(names) * int(np.prod([(int(n) << l) + 1 for n in ncell0]))
+        if (f.size if host else f.numel()) < need:
+            raise ValueError("field of level %d holds fewer than %d values" % (l, need))
+    st = np.asarray(steps if steps is not None else [0] * nl, dtype=np.int32)
+    fn = lib().alamo_write_plotfile_host if host else lib().alamo_write_plotfile
+    c = lambda a, t: np.ascontiguousarray(a, dtype=t)
+    nc, pl, ph = c(ncell0, np.int32), c(prob_lo, np.float64), c(prob_hi, np.float64)
+    check(fn(str(path).encode(), nl, nc.ctypes.data_as(ctypes.c_void_p), pl.ctypes.data_as(ctypes.c_void_p),
+             ph.ctypes.data_as(ctypes.c_void_p), nbox.ctypes.data_as(ctypes.c_void_p),
+             lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p), len(names), cnames, ptrs,
+             float(time), st.ctypes.data_as(ctypes.c_void_p)))

@@ -7,9 +7,9 @@
   Thresholded fields (w = 0) are not used: at contrasts of 1:100 and above their strong-form
   operator has eigenvalues with negative real part, and no multigrid cycle converges on them;
 * evaluation: a 512^3-cell (513^3-node) two-phase field (correlation length 16, contrast 1:1000,
-  seed 5, w = 0.7): plain FP64 multigrid (V-cycles; the reference's damped point Jacobi and the l1
-  smoother), FP64 and mixed FGMRES and the learned-preconditioned mixed FGMRES with the network
-  above (l1 V-cycle, the benchmark's settings), all to |b - A u|_inf / |b|_inf <= 1e-10 in FP64;
+  seed 5, w = 0.7): plain FP64 multigrid (the reference's damped point Jacobi V-cycle, and the l1
+  smoother in F-cycles), FP64 and mixed FGMRES and the learned-preconditioned mixed FGMRES with the
+  network above (l1 F-cycle, the benchmark's settings), all to |b - A u|_inf / |b|_inf <= 1e-10 in FP64;
   iterations, wall time and the relative L2 difference to the FP64 FGMRES solution.
 
     python -m alamo_amd.config_runs [--fields 10 --pairs 200] [--eval-n 512]
@@ -39,14 +39,16 @@ def main():
     p.add_argument("--corr-eval", type=float, default=16.0)
     p.add_argument("--width", type=float, default=0.7, help="erf interface parameter of the two-phase fields")
     p.add_argument("--restart", type=int, default=16)
-    p.add_argument("--sweeps", type=int, default=4, help="Jacobi sweeps per Fsmooth call in the FGMRES V-cycle")
-    p.add_argument("--smoother", default="l1", choices=["jacobi", "l1"], help="smoother of the FGMRES V-cycle")
+    p.add_argument("--cycle", default="F", choices=["F", "V"], help="cycle of the FGMRES preconditioner and of the "
+                   "plain l1 multigrid")
+    p.add_argument("--sweeps", type=int, default=3, help="smoothing sweeps per Fsmooth call in the FGMRES cycle")
+    p.add_argument("--smoother", default="l1", choices=["jacobi", "l1"], help="smoother of the FGMRES cycle")
     p.add_argument("--lateral", default="traction", choices=["traction", "neumann"],
                    help="lateral faces of the polycrystal")
-    p.add_argument("--sweeps-finest", type=int, default=1,
-                   help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --sweeps)")
+    p.add_argument("--sweeps-finest", type=int, default=2,
+                   help="sweeps per Fsmooth call on the finest level of the FGMRES cycle (0: --sweeps)")
     p.add_argument("--bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
-                   help="bottom solver of the FGMRES V-cycle (plain multigrid keeps bicgstab)")
+                   help="bottom solver of the FGMRES cycle (plain multigrid keeps bicgstab)")
     p.add_argument("--polycrystal", action="store_true")
     p.add_argument("--poly-n", type=int, default=256)
     # the polycrystal (MajorMinor, point Jacobi) takes restart 16 with 3 sweeps per call, 2 on the
@@ -60,7 +62,9 @@ def main():
 
     import torch
     from . import problems as P
-    from .operator import Elastic, Linear, Krylov, Net, PREC_MIXED, PREC_FP64, CYCLE_V, PRECOND_MG, PRECOND_LEARNED
+    from .operator import (Elastic, Linear, Krylov, Net, PREC_MIXED, PREC_FP64, CYCLE_F, CYCLE_V, PRECOND_MG,
+                           PRECOND_LEARNED)
+    cyc = CYCLE_F if a.cycle == "F" else CYCLE_V
 
     dev = torch.device("cuda:0")
     log = lambda *x: print(*x, file=sys.stderr, flush=True)
@@ -73,7 +77,7 @@ def main():
     for f in range(a.fields):
         pr = P.two_phase_problem(n=a.train_n, corr_cells=8, contrast=a.contrast_train, seed=1 + f, width=a.width)
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=PREC_MIXED,
-                     cycle=CYCLE_V, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest,
+                     cycle=cyc, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest,
                      smoother=a.smoother, bottom_solver=a.bottom)
         op.SetModel(pr["C"])
         st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f, solve_max_iter=100, solve_tol=1e-6)
@@ -100,7 +104,10 @@ def main():
                ("learned-mixed-fgmres", PREC_MIXED, PRECOND_LEARNED, a.smoother)]
     for name, prec, kind, sm in methods:
         krylov = kind is not None
-        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_V,
+        # plain multigrid: the reference's damped point Jacobi V-cycle (2 sweeps per Fsmooth), and the
+        # l1 smoother in the FGMRES methods' cycle
+        mcyc = cyc if (krylov or sm == "l1") else CYCLE_V
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=mcyc,
                      pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=1000,
                      bottom_solver=a.bottom if krylov else "bicgstab", sweeps_finest=a.sweeps_finest if krylov else 0,
                      smoother=sm)

@@ -28,6 +28,7 @@
 #include "post.h"
 #include "galerkin.h"
 #include "calib.h"
+#include "plotfile.h"
 
 using namespace alamo;
 
@@ -2736,6 +2737,88 @@ int alamo_amr_solve(alamo_amr* a, double* const* sol, const double* const* rhs, 
     if (!std::isfinite(rn)) FAIL(ALAMO_ERR_NAN, "alamo_amr_solve: residual is not finite (diverged)");
     if (!conv) FAIL(ALAMO_ERR_NOCONV, "alamo_amr_solve: max_iter reached before the tolerance");
     return ALAMO_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// AMReX plotfile of nodal fields (Integrator::WritePlotFile, Integrator.cpp:808-1054)
+// ===========================================================================
+extern "C" {
+
+static int write_plotfile_impl(const char* dir, int nlevels, const int* n_cell0, const double* prob_lo,
+                               const double* prob_hi, const int* nbox, const int* box_lo, const int* box_hi, int ncomp,
+                               const char* const* names, const double* const* fields, double time,
+                               const int* level_steps, bool device) {
+    if (!dir || !n_cell0 || !prob_lo || !prob_hi || !nbox || !names || !fields || !level_steps)
+        FAIL(ALAMO_ERR_ARG, "null argument");
+    if (nlevels < 1 || nlevels > 16) FAIL(ALAMO_ERR_ARG, "nlevels must be in [1, 16]");
+    if (ncomp < 1) FAIL(ALAMO_ERR_ARG, "ncomp must be positive");
+    for (int d = 0; d < 3; d++) {
+        if (n_cell0[d] < 1) FAIL(ALAMO_ERR_ARG, "n_cell0 must be positive");
+        if (!(prob_hi[d] > prob_lo[d])) FAIL(ALAMO_ERR_ARG, "prob_hi must exceed prob_lo");
+    }
+    std::vector<plot::Level> L(nlevels);
+    std::vector<std::vector<double>> host(nlevels);
+    int64_t bo = 0;
+    for (int l = 0; l < nlevels; l++) {
+        plot::Level& lv = L[l];
+        int64_t tot = 1;
+        for (int d = 0; d < 3; d++) {
+            lv.n_cell[d] = n_cell0[d] << l;
+            lv.dx[d] = (prob_hi[d] - prob_lo[d]) / lv.n_cell[d];
+            tot *= lv.n_cell[d] + 1;
+        }
+        if (nbox[l] < 0) FAIL(ALAMO_ERR_ARG, "nbox must be >= 0");
+        if (nbox[l] == 0) {
+            if (l > 0) FAIL(ALAMO_ERR_ARG, "levels >= 1 need at least one box");
+            lv.boxes.push_back({{0, 0, 0}, {lv.n_cell[0] - 1, lv.n_cell[1] - 1, lv.n_cell[2] - 1}});
+        }
+        for (int b = 0; b < nbox[l]; b++, bo++) {
+            if (!box_lo || !box_hi) FAIL(ALAMO_ERR_ARG, "null box arrays");
+            plot::CellBox cb;
+            for (int d = 0; d < 3; d++) {
+                cb.lo[d] = box_lo[3 * bo + d];
+                cb.hi[d] = box_hi[3 * bo + d];
+                if (cb.lo[d] < 0 || cb.hi[d] < cb.lo[d] || cb.hi[d] >= lv.n_cell[d])
+                    FAIL(ALAMO_ERR_ARG, "box " + std::to_string(b) + " of level " + std::to_string(l) +
+                                            " is empty or leaves the level's domain");
+            }
+            lv.boxes.push_back(cb);
+        }
+        if (!fields[l]) FAIL(ALAMO_ERR_ARG, "null field of level " + std::to_string(l));
+        if (device) {
+            host[l].resize((size_t)(ncomp * tot));
+            HIPCHK(hipMemcpy(host[l].data(), fields[l], host[l].size() * sizeof(double), hipMemcpyDeviceToHost));
+            lv.data = host[l].data();
+        } else {
+            lv.data = fields[l];
+        }
+    }
+    std::vector<std::string> nm;
+    for (int c = 0; c < ncomp; c++) {
+        if (!names[c] || !names[c][0]) FAIL(ALAMO_ERR_ARG, "empty component name");
+        nm.push_back(names[c]);
+    }
+    std::vector<int> steps(level_steps, level_steps + nlevels);
+    const std::string err = plot::write_node_plotfile(dir, L, ncomp, nm, prob_lo, prob_hi, time, steps);
+    if (!err.empty()) FAIL(ALAMO_ERR_ARG, "alamo_write_plotfile: " + err);
+    return ALAMO_OK;
+}
+
+int alamo_write_plotfile(const char* dir, int nlevels, const int* n_cell0, const double* prob_lo,
+                         const double* prob_hi, const int* nbox, const int* box_lo, const int* box_hi, int ncomp,
+                         const char* const* names, const double* const* fields, double time, const int* level_steps) {
+    return write_plotfile_impl(dir, nlevels, n_cell0, prob_lo, prob_hi, nbox, box_lo, box_hi, ncomp, names, fields,
+                               time, level_steps, true);
+}
+
+int alamo_write_plotfile_host(const char* dir, int nlevels, const int* n_cell0, const double* prob_lo,
+                              const double* prob_hi, const int* nbox, const int* box_lo, const int* box_hi, int ncomp,
+                              const char* const* names, const double* const* fields, double time,
+                              const int* level_steps) {
+    return write_plotfile_impl(dir, nlevels, n_cell0, prob_lo, prob_hi, nbox, box_lo, box_hi, ncomp, names, fields,
+                               time, level_steps, false);
 }
 
 }  // extern "C"

@@ -37,28 +37,30 @@ def parse():
     p.add_argument("--width", type=float, default=0.7,
                    help="interface parameter w of eta = (1 + erf(g / w)) / 2 (0: thresholded)")
     p.add_argument("--krylov-smoother", default="l1", choices=["jacobi", "l1"],
-                   help="smoother of the FGMRES V-cycle (plain multigrid methods: 'jacobi' unless the method "
+                   help="smoother of the FGMRES cycle (plain multigrid methods: 'jacobi' unless the method "
                         "name ends in -l1)")
     p.add_argument("--mg-max-iter", type=int, default=1500, help="cycle / iteration limit of the solves")
     p.add_argument("--fapply-reps", type=int, default=100)
     p.add_argument("--restart", type=int, default=16, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
-                   help="Fsmooth calls before and after the coarse correction in the FGMRES V-cycle")
-    p.add_argument("--krylov-sweeps", type=int, default=4, help="smoothing sweeps per Fsmooth call in the FGMRES V-cycle")
-    p.add_argument("--krylov-sweeps-finest", type=int, default=1,
-                   help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES V-cycle (0: --krylov-sweeps)")
+                   help="Fsmooth calls before and after the coarse correction in the FGMRES cycle")
+    p.add_argument("--krylov-sweeps", type=int, default=3, help="smoothing sweeps per Fsmooth call in the FGMRES cycle")
+    p.add_argument("--krylov-sweeps-finest", type=int, default=2,
+                   help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES cycle (0: --krylov-sweeps)")
     p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
-                   help="bottom solver of the FGMRES V-cycle (the plain multigrid methods keep bicgstab)")
+                   help="bottom solver of the FGMRES cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
     p.add_argument("--coarse-op", default="restrict", choices=["restrict", "galerkin"],
-                   help="coarse-level operators of the FGMRES V-cycle (methods ending in -gal: galerkin)")
+                   help="coarse-level operators of the FGMRES cycle (methods ending in -gal: galerkin)")
     p.add_argument("--methods",
-                   default="fp64v,fp64v-l1,mixedv-l1,fgmres-fp64v,fgmres-mixedv,fgmres-mixedv-gal,fgmres-learned-mixedv",
+                   default="fp64,fp64v,fp64-l1,fgmres-fp64,fgmres-mixed,fgmres-mixedv,fgmres-mixed-gal,"
+                           "fgmres-learned-mixed",
                    help="fp64/mixed: F-cycles with 2+2 Fsmooth calls (the reference's MLMG settings); "
                         "fp64v/mixedv: V-cycles with 1+1 Fsmooth calls (2 pre / 2 post Jacobi sweeps), the "
                         "reference's damped point Jacobi, or the l1 smoother with the suffix -l1; "
-                        "fgmres-*: flexible GMRES(--restart) preconditioned by one V-cycle with --krylov-smooth "
-                        "Fsmooth calls of --krylov-sweeps Jacobi sweeps each side; "
+                        "fgmres-*: flexible GMRES(--restart) preconditioned by one cycle (V with the suffix v, "
+                        "else F) with --krylov-smooth Fsmooth calls of --krylov-sweeps sweeps each side "
+                        "(--krylov-sweeps-finest on the finest level); "
                         "fgmres-learned-*: the same with the learned correction stage after each cycle; "
                         "the suffix -gal selects Galerkin coarse operators")
     p.add_argument("--train-n", type=int, default=128, help="cells per direction of the training fields")
@@ -133,7 +135,7 @@ def main():
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
 
     def make_op(prec, vcyc, krylov, smoother, coarse_op="restrict"):
-        nsm = (a.krylov_smooth if krylov else 1) if vcyc else 2
+        nsm = a.krylov_smooth if krylov else (1 if vcyc else 2)
         return Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
                        pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
                        sweeps_finest=a.krylov_sweeps_finest if krylov else 0, smoother=smoother,
@@ -217,7 +219,8 @@ def main():
             sols[meth] = sol.detach().clone()
         if learned:
             rec["fallback_events"] = st["fallback_events"]
-            rec["layers"] = layer_report(net.layer_times(op, reps=5), n)
+            md, mw = net.layer_times_bwd(op, reps=5)
+            rec["layers"] = layer_report(net.layer_times(op, reps=5), n, md, mw)
         print("%s: %s s/solve, %d iterations, final %.2e%s" % (meth, "%.4f" % rec["seconds"] if rec["seconds"] else "-",
                                                                rec["iterations"], rec["final_rel_res"],
                                                                "" if rec["converged"] else " (not converged)"),
@@ -272,7 +275,7 @@ def main():
                            100 * pr["interface"]["pure_phase_frac"], n, n + 1),
             "interface": pr["interface"],
             "method": best,
-            "fgmres": {"restart": a.restart, "vcycle_smooth_calls": a.krylov_smooth,
+            "fgmres": {"restart": a.restart, "cycle_smooth_calls": a.krylov_smooth,
                        "sweeps_per_call": a.krylov_sweeps,
                        "sweeps_per_call_finest": a.krylov_sweeps_finest or a.krylov_sweeps,
                        "smoother": a.krylov_smoother, "bottom_solver": a.krylov_bottom},
@@ -393,19 +396,25 @@ def oracle_residual(pr, x, b):
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of MI355X
 
 
-def layer_report(ms, n):
-    """per-layer throughput of the network's forward convolutions at n^3 cells: FLOPs
-    2 27 Cin Cout per voxel, bytes (Cin + Cout) 2 per voxel, bound min(MFMA peak, HBM x AI)"""
+def layer_report(ms, n, ms_data=None, ms_wgrad=None):
+    """per-layer throughput of the network's convolutions at n^3 cells: forward, backward-data and
+    weight gradient each 2 27 Cin Cout FLOPs per voxel; bytes (Cin + Cout) 2 per voxel (weight
+    gradient: the two inputs), bound min(MFMA peak, HBM x AI)"""
     spec = [(16, 16, 0), (16, 32, 1), (32, 64, 2), (64, 32, 2), (32, 16, 1), (16, 16, 0)]
     out = []
-    for (cin, cout, lev), t in zip(spec, ms):
+    for l, ((cin, cout, lev), t) in enumerate(zip(spec, ms)):
         V = (n // (2 ** lev) + 1) ** 3
         flops = 2.0 * 27 * cin * cout * V
         byts = (cin + cout) * 2.0 * V
         bound = min(MFMA_BF16_PEAK_TFLOPS, HBM_PEAK_GBS * flops / byts / 1e3)
-        tf = flops / (t * 1e-3) / 1e12
-        out.append({"cin": cin, "cout": cout, "level": lev, "ms": t, "tflops": tf, "bound_tflops": bound,
-                    "frac": tf / bound})
+        rec = {"cin": cin, "cout": cout, "level": lev, "bound_tflops": bound}
+        for key, tt in (("fwd", t), ("bwd_data", None if ms_data is None else ms_data[l]),
+                        ("wgrad", None if ms_wgrad is None else ms_wgrad[l])):
+            if tt is None:
+                continue
+            tf = flops / (tt * 1e-3) / 1e12
+            rec[key] = {"ms": tt, "tflops": tf, "frac": tf / bound}
+        out.append(rec)
     return out
 
 
@@ -418,8 +427,7 @@ def train_network(a, P, Elastic, Net, precision, cycle, smoother, coarse_op):
     for f in range(a.train_fields):
         pr = P.two_phase_problem(n=a.train_n, corr_cells=a.corr, contrast=a.contrast, seed=a.seed + 1 + f,
                                  width=a.width)
-        vc = cycle == 0
-        nsm = a.krylov_smooth if vc else 2
+        nsm = a.krylov_smooth
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=precision, cycle=cycle,
                      pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps,
                      sweeps_finest=a.krylov_sweeps_finest, smoother=smoother, bottom_solver=a.krylov_bottom,
@@ -446,7 +454,7 @@ def cpu_model():
 
 
 def cpu_baseline(pr, rhs, a, results):
-    """The value method's algorithm (FGMRES(restart) right-preconditioned by the same V-cycle, l1
+    """The value method's algorithm (FGMRES(restart) right-preconditioned by the same cycle, l1
     smoother, BiCGStab bottom to 1e-10) on the oracle, FP64, on the host cores:
 
     * at the full 257^3 size: setup (hierarchy, diagonals) and one iteration timed, scaled to the
@@ -462,8 +470,11 @@ def cpu_baseline(pr, rhs, a, results):
     cores = int(gomp.omp_get_max_threads())
     best = min((k for k in results if results[k]["converged"]), key=lambda k: results[k]["seconds"])
     gpu_its = results[best]["iterations"]
+    fcyc = not best.endswith("v")
     kw = dict(restart=a.restart, nu1=a.krylov_smooth, nu2=a.krylov_smooth, sweeps=a.krylov_sweeps,
-              sweeps_finest=a.krylov_sweeps_finest, smoother=1 if a.krylov_smoother == "l1" else 0)
+              sweeps_finest=a.krylov_sweeps_finest, smoother=1 if a.krylov_smoother == "l1" else 0,
+              cycle=1 if fcyc else 0)
+    cyc_name = "%s %s-cycle" % (a.krylov_smoother, "F" if fcyc else "V")
 
     def timed(p, b, max_iter):
         t0 = time.perf_counter()
@@ -478,9 +489,9 @@ def cpu_baseline(pr, rhs, a, results):
         per_it = max(t_one - t_setup, 1e-9)
         print("cpu baseline: %d thread(s), one iteration %.1fs" % (nthr, per_it), file=sys.stderr, flush=True)
         return {"value": t_setup + gpu_its * per_it, "unit": "s", "cores": nthr,
-                "sample": "oracle FP64 FGMRES(%d) + l1 V-cycle on the full %d^3-node field, %d thread(s): setup+"
+                "sample": "oracle FP64 FGMRES(%d) + %s on the full %d^3-node field, %d thread(s): setup+"
                           "residual (%.2fs) and one iteration (%.2fs) timed, scaled to the GPU method's %d "
-                          "iterations" % (a.restart, a.n + 1, nthr, t_setup, per_it, gpu_its)}
+                          "iterations" % (a.restart, cyc_name, a.n + 1, nthr, t_setup, per_it, gpu_its)}
 
     out = extrapolated(cores)
     out.update(kind="port", cpu_model=cpu_model(), method=best,

@@ -391,6 +391,24 @@ int alamo_amr_masks(alamo_amr* amr, int amrlev, int* region, int* fine_mask, voi
 int alamo_amr_solve(alamo_amr* amr, double* const* sol, const double* const* rhs, double tol_rel, double tol_abs,
                     alamo_solve_stats* stats);
 
+/* AMReX plotfile of nodal fields: Integrator::WritePlotFile's "<prefix>node" output
+ * (src/Integrator/Integrator.cpp:808-1054, through amrex::WriteMultiLevelPlotfile), in the
+ * HyperCLaw-V1.1 layout: <dir>/Header, <dir>/Level_<l>/Cell_H and Cell_D_00000 (one I/O rank), and
+ * the reference's <dir>/Checkpoint list of the level box arrays.  Level l has n_cell0 2^l cells
+ * per direction on [prob_lo, prob_hi]; nbox[l] inclusive cell boxes of level l in its own index
+ * space, concatenated in box_lo / box_hi (3 ints each; nbox[0] = 0: the whole domain), written as
+ * nodal boxes.  fields[l]: device array of ncomp SoA planes of the level's (n + 1)^3 nodes (the
+ * full-domain levels of alamo_amr_level); names[ncomp] the component names; level_steps[l] the
+ * step numbers recorded per level.  Creates `dir` and its Level_<l> subdirectories. */
+int alamo_write_plotfile(const char* dir, int nlevels, const int* n_cell0, const double* prob_lo,
+                         const double* prob_hi, const int* nbox, const int* box_lo, const int* box_hi, int ncomp,
+                         const char* const* names, const double* const* fields, double time, const int* level_steps);
+/* the same with host arrays fields[l] */
+int alamo_write_plotfile_host(const char* dir, int nlevels, const int* n_cell0, const double* prob_lo,
+                              const double* prob_hi, const int* nbox, const int* box_lo, const int* box_hi, int ncomp,
+                              const char* const* names, const double* const* fields, double time,
+                              const int* level_steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -902,7 +902,8 @@ static int fgmres(Hier& H, const MGOpts& o, int m, double* x, const double* b, d
         int jend = 0;
         for (int j = 0; j < m; j++) {
             W.res[0] = V[j];
-            vcycle(H, o, W, 0);
+            if (o.cycle == 1) fcycle(H, o, W);
+            else vcycle(H, o, W, 0);
             Z[j] = W.cor[0];
             apply_l(H, 0, Z[j].data(), V[j + 1].data());
             for (int i = 0; i <= j; i++) {
@@ -1479,7 +1480,8 @@ static int composite_solve(Composite& A, double* const* u, const double* const* 
 // right preconditioner.  Returns the iterations, negative when max_iter ended the solve.
 int ora_krylov_solve(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* x,
                      const double* b, double tol_rel, double tol_abs, int restart, int max_iter, int nu1, int nu2,
-                     int sweeps, int sweeps_finest, int smoother, double omega, double bottom_tol, double* final_res) {
+                     int sweeps, int sweeps_finest, int smoother, double omega, double bottom_tol, double* final_res,
+                     int cycle) {
     Hier H;
     build_hier(H, ncell, dx, sym, bct, C, -1);
     if (smoother == 1) {
@@ -1489,7 +1491,7 @@ int ora_krylov_solve(const int* ncell, const double* dx, int sym, const double* 
             l1_diag_rows(H.L[l], sym, H.C[l].data(), bct, H.Dl1[l].data());
         }
     }
-    MGOpts o{0, nu1, nu2, sweeps, omega, max_iter, 200, bottom_tol};
+    MGOpts o{cycle, nu1, nu2, sweeps, omega, max_iter, 200, bottom_tol};
     o.smoother = smoother;
     o.sweeps_finest = sweeps_finest;
     return fgmres(H, o, restart, x, b, tol_rel, tol_abs, final_res);

@@ -58,15 +58,17 @@ def test_l1_rows_equal_probing(ora, sym, ncell):
     assert np.allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
 
 
-def test_oracle_fgmres_matches_multigrid(ora):
-    """the oracle's FGMRES with the l1 V-cycle reaches the true residual 1e-12 |b| (checked with the
-    oracle's residual) and the solution of the reference multigrid on a 1:1000 two-phase field"""
+@pytest.mark.parametrize("cycle", [0, 1])
+def test_oracle_fgmres_matches_multigrid(ora, cycle):
+    """the oracle's FGMRES with the l1 V-cycle (cycle 0) or F-cycle (cycle 1) reaches the true residual
+    1e-12 |b| (checked with the oracle's residual) and the solution of the reference multigrid on a
+    1:1000 two-phase field"""
     n = 16
     pr = P.two_phase_problem(n=n, corr_cells=8, contrast=1000.0, seed=3, width=1.0)
     N = (n + 1) ** 3
     rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["F0"], pr["bct"], np.zeros(3 * N),
                          ora.bc_init(pr["ncell"], pr["bct"], pr["bcv"]))
-    x, it, fr = ora.krylov_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-12)
+    x, it, fr = ora.krylov_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-12, cycle=cycle)
     r = ora.residual(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], x, rhs)
     assert it > 0 and np.abs(r).max() <= 1e-12 * np.abs(rhs).max()
     xm, itm, _, _, _ = ora.mg_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], rhs, tol_rel=1e-12, cycle=0,
