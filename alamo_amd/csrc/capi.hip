@@ -266,6 +266,27 @@ static void vfill(T* x, int64_t n, T v, hipStream_t s) {
     k_fill<T><<<gs_blocks(n), 256, 0, s>>>(x, n, v);
 }
 
+// dst = src over n elements on the handle's stream: 16-byte loads and stores when both are
+// 16-byte aligned (every level buffer and Krylov vector here is), one work-item per 16 bytes;
+// the staging copies of the F-cycle preconditioner use it instead of the runtime's copy.
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy16(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+    constexpr int E = 16 / sizeof(T);
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = n / E;
+    if (q < nv) ((uint4*)dst)[q] = ((const uint4*)src)[q];
+    if (q < n - nv * E) dst[nv * E + q] = src[nv * E + q];  // the tail of fewer than E elements
+}
+template <typename T>
+static void vcopy(T* dst, const T* src, int64_t n, hipStream_t s) {
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) != 0) {
+        (void)hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s);
+        return;
+    }
+    const int64_t nv = std::max<int64_t>(n / (16 / (int64_t)sizeof(T)), 1);
+    k_copy16<T><<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(src, dst, n);
+}
+
 template <typename T>
 static int norms_dev(alamo_mg* h, const T* x, int64_t n, double* ninf, double* nl2) {
     const int nb = (int)std::min<int64_t>(h->nblk_red, (n + RED_BLOCK - 1) / RED_BLOCK);
@@ -791,9 +812,9 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
         if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
             vcycle_into<double, double>(h, v, z);
         } else {
-            HIPCHK(hipMemcpyAsync(l0.res, v, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            vcopy<double>(l0.res, v, n, h->stream);
             one_cycle<double>(h);
-            HIPCHK(hipMemcpyAsync(z, l0.cor, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            vcopy<double>(z, l0.cor, n, h->stream);
         }
     }
     return ALAMO_OK;
@@ -881,9 +902,9 @@ static int precond_f32(alamo_mg* h, int kind, const float* v, float* z) {
     if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
         vcycle_into<float, float>(h, v, z);
     } else {
-        HIPCHK(hipMemcpyAsync(l0.resf, v, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+        vcopy<float>(l0.resf, v, n, h->stream);
         one_cycle<float>(h);
-        HIPCHK(hipMemcpyAsync(z, l0.corf, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+        vcopy<float>(z, l0.corf, n, h->stream);
     }
     if (kind != ALAMO_PRECOND_LEARNED) return ALAMO_OK;
     if (!h->lq) TRY(dalloc(h, &h->lq, 5 * n));

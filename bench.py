@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--krylov-sweeps", type=int, default=3, help="smoothing sweeps per Fsmooth call in the FGMRES cycle")
     p.add_argument("--krylov-sweeps-finest", type=int, default=2,
                    help="Jacobi sweeps per Fsmooth call on the finest level of the FGMRES cycle (0: --krylov-sweeps)")
+    p.add_argument("--krylov-coarsest", type=int, default=4,
+                   help="cells per direction of the coarsest level of the FGMRES cycle (0: coarsen to 2 cells)")
     p.add_argument("--krylov-bottom", default="dense", choices=["bicgstab", "dense", "smoother"],
                    help="bottom solver of the FGMRES cycle (the plain multigrid methods keep bicgstab)")
     p.add_argument("--skip-cpu", action="store_true")
@@ -134,13 +136,18 @@ def main():
     bc = (pr["bct"], pr["bcv"])
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
 
+    # the FGMRES cycle stops coarsening at --krylov-coarsest cells (its dense bottom); the plain
+    # multigrid methods keep the reference's full depth
+    mcl = int(round(math.log2(n / a.krylov_coarsest))) if a.krylov_coarsest > 0 else -1
+
     def make_op(prec, vcyc, krylov, smoother, coarse_op="restrict"):
         nsm = a.krylov_smooth if krylov else (1 if vcyc else 2)
         return Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=prec, cycle=CYCLE_V if vcyc else CYCLE_F,
                        pre_smooth=nsm, post_smooth=nsm, sweeps_per_smooth=a.krylov_sweeps if krylov else 2,
                        sweeps_finest=a.krylov_sweeps_finest if krylov else 0, smoother=smoother,
                        max_iter=a.mg_max_iter, verbose=a.verbose, coarse_op=coarse_op,
-                       bottom_solver=a.krylov_bottom if krylov or coarse_op == "galerkin" else "bicgstab")
+                       bottom_solver=a.krylov_bottom if krylov or coarse_op == "galerkin" else "bicgstab",
+                       max_coarsening_level=mcl if krylov else -1)
 
     results = {}
     sols = {}
@@ -477,9 +484,11 @@ def cpu_baseline(pr, rhs, a, results):
     cyc_name = "%s %s-cycle" % (a.krylov_smoother, "F" if fcyc else "V")
 
     def timed(p, b, max_iter):
+        nc = p["ncell"][0]
+        mcl = int(round(math.log2(nc / a.krylov_coarsest))) if a.krylov_coarsest > 0 else -1
         t0 = time.perf_counter()
         x, it, fr = ora.krylov_solve(p["ncell"], p["dx"], P.ISO, p["C"], p["bct"], b, tol_rel=1e-10,
-                                     max_iter=max_iter, **kw)
+                                     max_iter=max_iter, max_level=mcl, **kw)
         return time.perf_counter() - t0, it, fr
 
     def extrapolated(nthr):

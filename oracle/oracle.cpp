@@ -1481,9 +1481,9 @@ static int composite_solve(Composite& A, double* const* u, const double* const* 
 int ora_krylov_solve(const int* ncell, const double* dx, int sym, const double* C, const int* bct, double* x,
                      const double* b, double tol_rel, double tol_abs, int restart, int max_iter, int nu1, int nu2,
                      int sweeps, int sweeps_finest, int smoother, double omega, double bottom_tol, double* final_res,
-                     int cycle) {
+                     int cycle, int max_level) {
     Hier H;
-    build_hier(H, ncell, dx, sym, bct, C, -1);
+    build_hier(H, ncell, dx, sym, bct, C, max_level);
     if (smoother == 1) {
         H.Dl1.resize(H.L.size());
         for (size_t l = 0; l < H.L.size(); l++) {

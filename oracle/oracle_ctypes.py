@@ -110,15 +110,17 @@ def l1_diag_rows(ncell, dx, sym, C, bct):
 
 
 def krylov_solve(ncell, dx, sym, C, bct, b, x0=None, tol_rel=1e-10, tol_abs=0.0, restart=16, max_iter=400, nu1=1,
-                 nu2=1, sweeps=3, sweeps_finest=1, smoother=1, omega=2.0 / 3.0, bottom_tol=1e-10, cycle=0):
-    """FGMRES(restart) with one V-cycle (cycle 0) or F-cycle (cycle 1) as right preconditioner; returns
-    (x, iterations, final |r|_inf)"""
+                 nu2=1, sweeps=3, sweeps_finest=1, smoother=1, omega=2.0 / 3.0, bottom_tol=1e-10, cycle=0,
+                 max_level=-1):
+    """FGMRES(restart) with one V-cycle (cycle 0) or F-cycle (cycle 1) as right preconditioner, over
+    levels 0 .. max_level (-1: coarsen to 2 cells); returns (x, iterations, final |r|_inf)"""
     ncell = _c(ncell, np.int32); dx = _c(dx); C = _c(C); bct = _c(bct, np.int32); b = _c(b)
     x = np.zeros_like(b) if x0 is None else _c(x0).copy()
     fr = ctypes.c_double(0)
     it = lib().ora_krylov_solve(_i(ncell), _d(dx), sym, _d(C), _i(bct), _d(x), _d(b), ctypes.c_double(tol_rel),
                                 ctypes.c_double(tol_abs), restart, max_iter, nu1, nu2, sweeps, sweeps_finest, smoother,
-                                ctypes.c_double(omega), ctypes.c_double(bottom_tol), ctypes.byref(fr), int(cycle))
+                                ctypes.c_double(omega), ctypes.c_double(bottom_tol), ctypes.byref(fr), int(cycle),
+                                int(max_level))
     return x, it, fr.value
 
 
