@@ -463,6 +463,43 @@ static void fcycle(alamo_mg* h) {
     }
 }
 
+// z = F-cycle(v) from a zero guess for a preconditioner, v and z in the cycle precision: the same
+// steps as fcycle on a copy of v, without the copies.  The first restriction reads v in place; on
+// level 0 the initial guess P cor_1 goes straight into the hold buffer (no clearing pass, and
+// 0 + x = x exactly), the level's right-hand side is v - A hold, and the last step writes
+// z = cor + hold.  Bit-identical to copy, fcycle, copy.
+template <typename T>
+static void fcycle_into(alamo_mg* h, const T* v, T* z) {
+    const int bot = (int)h->L.size() - 1;
+    LevelBuf& l0 = h->L[0];
+    const int64_t n = 3 * l0.g.tot;
+    LevelBuf& l1 = h->L[1];
+    k_restrict<T><<<nblocks(l1.g.tot), 256, 0, h->stream>>>(l1.g, l0.g, 3, v, LB<T>::res(l1));
+    for (int l = 2; l <= bot; l++) {
+        LevelBuf& lf = h->L[l - 1];
+        LevelBuf& lc = h->L[l];
+        k_restrict<T><<<nblocks(lc.g.tot), 256, 0, h->stream>>>(lc.g, lf.g, 3, LB<T>::res(lf), LB<T>::res(lc));
+    }
+    bottom<T>(h);
+    for (int l = bot - 1; l >= 1; l--) {
+        LevelBuf& lb = h->L[l];
+        LevelBuf& lc = h->L[l + 1];
+        const int64_t nl = 3 * lb.g.tot;
+        vfill<T>(LB<T>::cor(lb), nl, T(0), h->stream);
+        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
+        std::swap(LB<T>::res(lb), LB<T>::rescor(lb));
+        std::swap(LB<T>::hold(lb), LB<T>::cor(lb));
+        vcycle<T>(h, l);
+        k_add<T><<<gs_blocks(nl), 256, 0, h->stream>>>(LB<T>::cor(lb), LB<T>::cor(lb), LB<T>::hold(lb), nl);
+    }
+    k_interp_add<T, true><<<interp_blocks(l1.g, l0.g), 256, 0, h->stream>>>(l1.g, l0.g, LB<T>::cor(l1),
+                                                                             LB<T>::hold(l0));
+    level_op<T>(h, 0, RESID, LB<T>::hold(l0), v, LB<T>::res(l0), T(0), h->stream);
+    vcycle<T>(h, 0);
+    k_add<T><<<gs_blocks(n), 256, 0, h->stream>>>(z, LB<T>::cor(l0), LB<T>::hold(l0), n);
+}
+
 // amrex::MLMG::oneIter: an F-cycle while iter < max_fmg_iter, then V-cycles (the reference sets
 // max_fmg_iter to 1e8, Linear.H:261, so every iteration is an F-cycle)
 template <typename T>
@@ -811,6 +848,8 @@ static int precond_apply(alamo_mg* h, int kind, const double* v, double* z) {
     } else {
         if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
             vcycle_into<double, double>(h, v, z);
+        } else if (h->d.cycle == ALAMO_CYCLE_F && h->d.max_fmg_iter > 0 && h->L.size() > 1) {
+            fcycle_into<double>(h, v, z);
         } else {
             vcopy<double>(l0.res, v, n, h->stream);
             one_cycle<double>(h);
@@ -901,6 +940,8 @@ static int precond_f32(alamo_mg* h, int kind, const float* v, float* z) {
     const int64_t n = 3 * l0.g.tot;
     if (h->d.cycle == ALAMO_CYCLE_V && h->d.pre_smooth * h->d.sweeps_per_smooth > 0 && h->L.size() > 1) {
         vcycle_into<float, float>(h, v, z);
+    } else if (h->d.cycle == ALAMO_CYCLE_F && h->d.max_fmg_iter > 0 && h->L.size() > 1) {
+        fcycle_into<float>(h, v, z);
     } else {
         vcopy<float>(l0.resf, v, n, h->stream);
         one_cycle<float>(h);

@@ -314,7 +314,8 @@ __global__ void __launch_bounds__(256) k_restrict_psi(Geom gc, Geom gf, const do
 // lane, so every lane evaluates the even and the odd formula of its row case and keeps its own.  The odd formula's
 // x + 1 coarse neighbour is read at offset `odd` (0 on even lanes), so no read leaves the level.
 // Each node keeps its case's formula and term order.
-template <typename T>
+// SET: fine = I(crse) instead (the F-cycle's initial guess of a level, without clearing it first)
+template <typename T, bool SET = false>
 __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* __restrict__ crse,
                                                     T* __restrict__ fine) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,7 +359,8 @@ __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* _
             vo = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
                              at(1, 1, 0) + at(1, 1, 1));
         }
-        fine[p * gf.tot + mf] += o ? vo : ve;
+        if (SET) fine[p * gf.tot + mf] = o ? vo : ve;
+        else fine[p * gf.tot + mf] += o ? vo : ve;
     }
 }
 

@@ -190,7 +190,8 @@ def test_linear_solve_eshelby(cuda, ora, precision, cycle):
     assert err <= 1e-8
 
 
-@pytest.mark.parametrize("precision,cycle,precond", [(0, 1, 1), (PREC_MIXED, 0, 1), (0, 0, PRECOND_NONE)])
+@pytest.mark.parametrize("precision,cycle,precond", [(0, 1, 1), (PREC_MIXED, 0, 1), (PREC_MIXED, 1, 1),
+                                                     (0, 0, PRECOND_NONE)])
 def test_krylov_solve_eshelby(cuda, ora, precision, cycle, precond):
     """Flexible GMRES (multigrid-cycle or no preconditioner) converges to the oracle's solution."""
     n = 16
@@ -205,6 +206,28 @@ def test_krylov_solve_eshelby(cuda, ora, precision, cycle, precond):
     err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
     print("krylov iterations", st["iterations"], "rel L2 err", err)
     assert st["converged"] == 1 and err <= 1e-8
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(smoother="l1", sweeps_per_smooth=3, sweeps_finest=2,
+                                               pre_smooth=1, post_smooth=1, bottom_solver="dense")])
+def test_fcycle_preconditioner_is_one_fcycle(cuda, ora, opts):
+    """The F-cycle preconditioner (alamo_mg_precond; reads r in place, writes z = cor + hold without
+    staging copies) equals one F-cycle of the multigrid solve from a zero guess bit for bit."""
+    from alamo_amd.operator import precond_apply, PRECOND_MG
+    pr, bc, rhs = eshelby_rhs(ora, 32)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, cycle=1, max_iter=10, **opts)
+    op.SetModel(pr["C"])
+    op.prepareForSolve()
+    r = to_dev(rhs, cuda)
+    z = to_dev(np.zeros_like(rhs), cuda)
+    precond_apply(op, PRECOND_MG, r, z)
+    lin = Linear(op)
+    lin.setFixedIter(1)
+    x = to_dev(np.zeros_like(rhs), cuda)
+    st = lin.solve(x, r, 1e-14, 0.0, raise_on_fail=False)
+    assert st["iterations"] == 1
+    np.testing.assert_array_equal(to_host(z), to_host(x))
+    np.testing.assert_array_equal(to_host(r), rhs)  # the right-hand side is only read
 
 
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
