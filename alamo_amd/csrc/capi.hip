@@ -871,11 +871,11 @@ static int krylov_scratch(alamo_mg* h) {
 }
 
 template <typename T>
-static int multidot_t(alamo_mg* h, const T* w, const T* V, int64_t n, int nv, double* out) {
+static int multidot_t(alamo_mg* h, const T* w, const T* V, int64_t n, int nv, double* out, int64_t ldv = -1) {
     if (nv < 1 || nv > KRY_MAXV) FAIL(ALAMO_ERR_ARG, "multidot: vector count out of range");
     TRY(krylov_scratch(h));
     const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
-    launch_multidot(w, V, n, n, nv, nb, h->kpart, h->stream);
+    launch_multidot(w, V, n, ldv < 0 ? n : ldv, nv, nb, h->kpart, h->stream);
     k_sum_rows<<<nv, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
     HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, nv * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -888,9 +888,9 @@ static int multidot(alamo_mg* h, const double* w, const double* V, int64_t n, in
 
 // w -= sum_{i < nv} c_i V_i with c already uploaded (upload_coef); *nrm = |w|_2 afterwards
 template <typename T>
-static int multiaxpy_norm(alamo_mg* h, T* w, const T* V, int64_t n, int nv, double* nrm) {
+static int multiaxpy_norm(alamo_mg* h, T* w, const T* V, int64_t n, int nv, double* nrm, int64_t ldv = -1) {
     const int nb = (int)std::min<int64_t>(KRY_DOT_BLOCKS, (n + RED_BLOCK - 1) / RED_BLOCK);
-    k_multiaxpy_nrm<T><<<nb, RED_BLOCK, 0, h->stream>>>(w, V, n, n, nv, h->kcoef, h->kpart);
+    launch_multiaxpy_nrm<T>(w, V, n, ldv < 0 ? n : ldv, nv, h->kcoef, nb, h->kpart, h->stream);
     k_sum_rows<<<1, RED_BLOCK, 0, h->stream>>>(h->kpart, nb, h->kdots);
     HIPCHK(hipMemcpyAsync(h->red_host, h->kdots, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -976,20 +976,23 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
     if (!h->prepared) TRY(prepare(h));
     LevelBuf& l0 = h->L[0];
     const int64_t n = 3 * l0.g.tot;
+    // the basis blocks' leading dimension: n rounded up to 64 elements, so every vector starts
+    // 256-byte aligned and the Gram-Schmidt kernels take their 16-byte form
+    const int64_t ld = (n + 63) / 64 * 64;
     TRY(krylov_scratch(h));
     TB *V, *Z;
     if constexpr (F32) {
         if (h->kry_mf < m) {
-            TRY(dalloc(h, &h->kVf, (int64_t)(m + 1) * n));
-            TRY(dalloc(h, &h->kZf, (int64_t)m * n));
+            TRY(dalloc(h, &h->kVf, (int64_t)(m + 1) * ld));
+            TRY(dalloc(h, &h->kZf, (int64_t)m * ld));
             h->kry_mf = m;
         }
         V = h->kVf;
         Z = h->kZf;
     } else {
         if (h->kry_m < m) {
-            TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * n));
-            TRY(dalloc(h, &h->kZ, (int64_t)m * n));
+            TRY(dalloc(h, &h->kV, (int64_t)(m + 1) * ld));
+            TRY(dalloc(h, &h->kZ, (int64_t)m * ld));
             h->kry_m = m;
         }
         V = h->kV;
@@ -1020,9 +1023,9 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
         gv[0] = beta;
         int jend = 0;
         for (int j = 0; j < m; j++) {
-            TB* vj = V + (int64_t)j * n;
-            TB* zj = Z + (int64_t)j * n;
-            TB* w = V + (int64_t)(j + 1) * n;
+            TB* vj = V + (int64_t)j * ld;
+            TB* zj = Z + (int64_t)j * ld;
+            TB* w = V + (int64_t)(j + 1) * ld;
             int prc;
             if constexpr (F32) prc = precond_f32(h, kind, vj, zj);
             else prc = precond_apply(h, kind, vj, zj);
@@ -1041,7 +1044,7 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
             // d1[j + 1]; the true Arnoldi quantities follow from the scales sc
             const bool lazy = g_kry_lazy && kind != ALAMO_PRECOND_LEARNED;
             const double sj = sc[j];
-            TRY(multidot_t<TB>(h, w, V, n, j + 2, d1));
+            TRY(multidot_t<TB>(h, w, V, n, j + 2, d1, ld));
             const double wn = std::sqrt(std::max(d1[j + 1], 0.0));
             for (int i = 0; i <= j; i++) {
                 H[i][j] = sj * sc[i] * d1[i];
@@ -1049,16 +1052,16 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
             }
             TRY(upload_coef(h, cf, j + 1));
             double g = 0;  // |w''| of the stored vector
-            TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g));
+            TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g, ld));
             if (g < g_reorth_eta * wn) {
                 // the first pass cancelled most of w: re-orthogonalise (CGS2)
-                TRY(multidot_t<TB>(h, w, V, n, j + 1, d2));
+                TRY(multidot_t<TB>(h, w, V, n, j + 1, d2, ld));
                 for (int i = 0; i <= j; i++) {
                     H[i][j] += sj * sc[i] * d2[i];
                     cf[i] = sc[i] * sc[i] * d2[i];
                 }
                 TRY(upload_coef(h, cf, j + 1));
-                TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g));
+                TRY(multiaxpy_norm<TB>(h, w, V, n, j + 1, &g, ld));
                 reorths++;
             }
             const double hn = sj * g;  // the true |w''|
@@ -1108,7 +1111,7 @@ static int fgmres_core(alamo_mg* h, double* sol, const double* rhs, double tol_r
         }
         for (int i = 0; i < jend; i++) cf[i] = y[i] * sc[i];  // z_i = sc_i x stored z_i
         TRY(upload_coef(h, cf, jend));
-        k_multiaxpy_add<TB><<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, n, jend, h->kcoef);
+        k_multiaxpy_add<TB><<<gs_blocks(n), 256, 0, h->stream>>>(sol, Z, n, ld, jend, h->kcoef);
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, sol, rhs, r, 0.0, h->stream);
         TRY(norms_dev<double>(h, r, n, &rinf, &beta));
         if (o->verbose)
