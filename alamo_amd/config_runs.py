@@ -38,7 +38,8 @@ def main():
     p.add_argument("--eval-n", type=int, default=512)
     p.add_argument("--corr-eval", type=float, default=16.0)
     p.add_argument("--width", type=float, default=0.7, help="erf interface parameter of the two-phase fields")
-    p.add_argument("--restart", type=int, default=16)
+    p.add_argument("--restart", type=int, default=24)
+    p.add_argument("--coarsest", type=int, default=4, help="cells per direction of the FGMRES cycle's coarsest level")
     p.add_argument("--cycle", default="F", choices=["F", "V"], help="cycle of the FGMRES preconditioner and of the "
                    "plain l1 multigrid")
     p.add_argument("--sweeps", type=int, default=3, help="smoothing sweeps per Fsmooth call in the FGMRES cycle")
@@ -107,7 +108,10 @@ def main():
         # plain multigrid: the reference's damped point Jacobi V-cycle (2 sweeps per Fsmooth), and the
         # l1 smoother in the FGMRES methods' cycle
         mcyc = cyc if (krylov or sm == "l1") else CYCLE_V
+        import math
+        mcl = int(round(math.log2(n / a.coarsest))) if (krylov and a.coarsest > 0) else -1
         op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=mcyc,
+                     max_coarsening_level=mcl,
                      pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps if krylov else 2, max_iter=1000,
                      bottom_solver=a.bottom if krylov else "bicgstab", sweeps_finest=a.sweeps_finest if krylov else 0,
                      smoother=sm)

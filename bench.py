@@ -41,7 +41,7 @@ def parse():
                         "name ends in -l1)")
     p.add_argument("--mg-max-iter", type=int, default=1500, help="cycle / iteration limit of the solves")
     p.add_argument("--fapply-reps", type=int, default=100)
-    p.add_argument("--restart", type=int, default=16, help="FGMRES restart length")
+    p.add_argument("--restart", type=int, default=24, help="FGMRES restart length")
     p.add_argument("--krylov-smooth", type=int, default=1,
                    help="Fsmooth calls before and after the coarse correction in the FGMRES cycle")
     p.add_argument("--krylov-sweeps", type=int, default=3, help="smoothing sweeps per Fsmooth call in the FGMRES cycle")
