@@ -60,6 +60,11 @@ constexpr int ROWS64_BIG = 192;
 template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 128; };
 template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 128; };
 template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 64; };
+// The FP32 Jacobi sweep with the l1 smoother needs 141 VGPRs (3 waves per SIMD), where an 8-row
+// workgroup (8 waves) leaves a third of the wave slots of a CU empty; 4-row workgroups fill all 12.
+// At 257^3 inside the F-cycle FGMRES solve: 8 x 64 0.460 s, 4 x 32 0.447 s, 4 x 16 0.448 s, 2 x 32
+// 0.449 s, 4 x 128 0.454 s, 4 x 64 0.462 s, 2 x 64 0.461 s.
+template <> struct Rows64Big<float, 2> { static constexpr int RW = 4, KCH = 32; };
 // levels of < ROWS64_MID cells in y or z (65^3 nodes and the like): 2 x 2 tiles keep about
 // 1000 workgroups in flight.  At 65^3 in FP32: Jacobi 6.3 us, residual 6.0 us (4 x 8: 13.1 and
 // 12.1; the node-parallel rows took 18 and 15)
