@@ -21,6 +21,7 @@ namespace net {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 struct Grid3 {
     int N[3];       // nodes per direction
@@ -190,22 +191,30 @@ __global__ void __launch_bounds__(64 * CL_TY) k_conv3_fwd_lds(Grid3 g, int xb, c
         for (int a = 0; a < MT; a++)
 #pragma unroll
             for (int t = 0; t < NT; t++) acc[a][t] = f4{0.f, 0.f, 0.f, 0.f};
-        for (int o = 0; o < 27; o++) {
+        // gfx950's 16x16x32 products (operand layout at k_conv3_wgrad_lds): K = 32 is two offsets of
+        // the 16 channels (lanes 0-31 the even offset of a pair, lanes 32-63 the odd one; the 27th
+        // offset pairs with zero weights), or the 32 channels of one offset
+        constexpr int NP = CIN == 16 ? 14 : 27;
+        const int half = lane >> 5;
+        const int c0 = CIN == 16 ? 8 * ((lane >> 4) & 1) : 8 * (lane >> 4);
+        for (int pp = 0; pp < NP; pp++) {
+            const int o = CIN == 16 ? 2 * pp + half : pp;
+            const bool live = o < 27;
             int dx, dy, dz;
-            off3(o, dx, dy, dz);
+            off3(live ? o : 26, dx, dy, dz);
             const uint16_t(*pl)[CW][CIN] = tile[slot(z + dz)];
+            bf8 bw[NT];
 #pragma unroll
-            for (int ci0 = 0; ci0 < CIN; ci0 += 16) {
-                s4 bw[NT];
+            for (int t = 0; t < NT; t++) {
+                uint4 wv = make_uint4(0, 0, 0, 0);
+                if (live) wv = *(const uint4*)(W + ((int64_t)o * Cout + t * 16 + m) * CIN + c0);
+                bw[t] = __builtin_bit_cast(bf8, wv);
+            }
 #pragma unroll
-                for (int t = 0; t < NT; t++) bw[t] = *(const s4*)(W + ((int64_t)o * Cout + t * 16 + m) * CIN + ci0 + kq);
+            for (int a = 0; a < MT; a++) {
+                const bf8 av = __builtin_bit_cast(bf8, *(const uint4*)&pl[w + 1 + dy][16 * a + m + dx + 1][c0]);
 #pragma unroll
-                for (int a = 0; a < MT; a++) {
-                    const s4 av = *(const s4*)&pl[w + 1 + dy][16 * a + m + dx + 1][ci0 + kq];
-#pragma unroll
-                    for (int t = 0; t < NT; t++)
-                        acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc[a][t], 0, 0, 0);
-                }
+                for (int t = 0; t < NT; t++) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[t], acc[a][t], 0, 0, 0);
             }
         }
         const int64_t rowo = (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
@@ -247,7 +256,6 @@ __global__ void __launch_bounds__(64 * CL_TY) k_conv3_fwd_lds(Grid3 g, int xb, c
 // Here M = co (A = dout^T of a 32-voxel row), N = ci (B = the row's input shifted by o), K = voxels.
 // Wavefront w of the 4 owns the offsets o = w, w + 4, ... (7 or 6 of the 27) for every co tile.
 // ---------------------------------------------------------------------------
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 constexpr int WG_TY = 4, WG_TZ = 2;
 constexpr int WG_ROWS = WG_TY * WG_TZ;                  // K chunks (rows of 32 voxels) per tile
 constexpr int WG_SD = WG_ROWS * 32 + 8;                 // dout: elements per channel (+8: bank spread)
