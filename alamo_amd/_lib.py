@@ -42,7 +42,7 @@ class KrylovOpts(ctypes.Structure):
 # name -> (restype, argtypes)
 class TrainOpts(ctypes.Structure):
     _fields_ = [("npairs", c_int), ("steps", c_int), ("lr", c_double), ("seed", ctypes.c_uint64),
-                ("solve_max_iter", c_int), ("solve_tol", c_double), ("verbose", c_int)]
+                ("solve_max_iter", c_int), ("solve_tol", c_double), ("verbose", c_int), ("loss", c_int)]
 
 
 class TrainStats(ctypes.Structure):
@@ -117,6 +117,7 @@ SIGNATURES = {
     "alamo_net_load": (c_int, [c_void_p, ctypes.c_char_p]),
     "alamo_mg_set_net": (c_int, [c_void_p, c_void_p]),
     "alamo_mg_precond": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "alamo_mg_apply_transpose": (c_int, [c_void_p, c_void_p, c_void_p]),
     "alamo_amr_fine_mask": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_reflux": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "alamo_amr_average_down": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

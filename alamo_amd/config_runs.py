@@ -1,9 +1,11 @@
 """The learned-preconditioner configurations and the polycrystal at their stated sizes, on one GPU.
 
-* training: (residual, error) pairs from the plain solver on 128^3-cell (129^3-node) two-phase
+* training: residuals left by one cycle of the plain solver on 128^3-cell (129^3-node) two-phase
   fields (Gaussian random field, correlation length 8 cells, erf interface w = 0.7 as in the
   benchmark, lambda = mu per phase, contrast 1:100), --pairs pairs on each of --fields fields, one
-  Adam step per pair, by the library's own BF16 forward and backward kernels (alamo_net_train).
+  Adam step per pair, by the library's own BF16 forward and backward kernels (alamo_net_train), on
+  --loss residual (the stage's squared residual ratio, the default) or error (MSE against the
+  cycle's remaining error, solved per pair).
   Thresholded fields (w = 0) are not used: at contrasts of 1:100 and above their strong-form
   operator has eigenvalues with negative real part, and no multigrid cycle converges on them;
 * evaluation: a 512^3-cell (513^3-node) two-phase field (correlation length 16, contrast 1:1000,
@@ -34,6 +36,7 @@ def main():
     p.add_argument("--train-n", type=int, default=128)
     p.add_argument("--fields", type=int, default=10)
     p.add_argument("--pairs", type=int, default=200, help="pairs per field")
+    p.add_argument("--loss", default="residual", choices=["residual", "error"], help="network training loss")
     p.add_argument("--contrast-train", type=float, default=100.0)
     p.add_argument("--eval-n", type=int, default=512)
     p.add_argument("--corr-eval", type=float, default=16.0)
@@ -81,13 +84,14 @@ def main():
                      cycle=cyc, pre_smooth=1, post_smooth=1, sweeps_per_smooth=a.sweeps, sweeps_finest=a.sweeps_finest,
                      smoother=a.smoother, bottom_solver=a.bottom)
         op.SetModel(pr["C"])
-        st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f, solve_max_iter=100, solve_tol=1e-6)
+        st = net.train(op, npairs=a.pairs, steps=a.pairs, lr=1e-3, seed=100 + f,
+                       solve_max_iter=0 if a.loss == "residual" else 100, solve_tol=1e-6, loss=a.loss)
         losses.append((st["first_loss"], st["last_loss"]))
         log("field %d: pairs %d (converged %d), loss %.4f -> %.4f, data %.1fs, train %.1fs"
             % (f, st["pairs"], st["pairs_converged"], st["first_loss"], st["last_loss"], st["seconds_data"],
                st["seconds_train"]))
         del op
-    out["training"] = {"fields": a.fields, "pairs": a.fields * a.pairs, "cells": a.train_n,
+    out["training"] = {"loss": a.loss, "fields": a.fields, "pairs": a.fields * a.pairs, "cells": a.train_n,
                        "contrast": a.contrast_train, "first_loss": losses[0][0], "last_loss": losses[-1][1],
                        "seconds": time.time() - t0}
 

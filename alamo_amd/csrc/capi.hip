@@ -151,6 +151,7 @@ struct alamo_mg {
     double* kdots = nullptr;  // multi-dot results (device)
     alamo_net* net = nullptr; // learned correction (non-owning)
     double* lq = nullptr;     // FP64 scratch of the learned stage
+    double* rowtab = nullptr; // explicit level-0 rows, 243 doubles per node (transpose applies)
     std::vector<void*> allocs;
 };
 
@@ -1764,6 +1765,31 @@ int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z) {
     return rc;
 }
 
+// (Re)build the explicit level-0 row table from the current coefficients and boundary table.
+static int row_table(alamo_mg* h) {
+    if (h->psi_set) FAIL(ALAMO_ERR_ARG, "the transposed operator has no psi terms");
+    const Geom& g = h->L[0].g;
+    if (!h->rowtab) TRY(dalloc(h, &h->rowtab, 243 * g.tot));
+    const unsigned nb = nblocks(g.tot);
+    if (h->sym == ISO) k_row_table<ISO><<<nb, 256, 0, h->stream>>>(g, h->bc, h->L[0].C, h->rowtab);
+    else if (h->sym == MAJORMINOR) k_row_table<MAJORMINOR><<<nb, 256, 0, h->stream>>>(g, h->bc, h->L[0].C, h->rowtab);
+    else k_row_table<DIAGONAL><<<nb, 256, 0, h->stream>>>(g, h->bc, h->L[0].C, h->rowtab);
+    HIPCHK(hipGetLastError());
+    return ALAMO_OK;
+}
+
+int alamo_mg_apply_transpose(alamo_mg* h, const double* r, double* out) {
+    if (!h || !r || !out) FAIL(ALAMO_ERR_ARG, "null argument");
+    if (!h->model_set) FAIL(ALAMO_ERR_STATE, "SetModel has not been called");
+    if (r == out) FAIL(ALAMO_ERR_ARG, "r and out must not alias");
+    TRY(row_table(h));
+    const Geom& g = h->L[0].g;
+    k_row_apply_t<<<nblocks(g.tot), 256, 0, h->stream>>>(g, h->rowtab, r, out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ALAMO_OK;
+}
+
 int alamo_linear_solve(alamo_mg* h, double* sol, const double* rhs, double tol_rel, double tol_abs,
                        alamo_solve_stats* stats) {
     if (!h || !sol || !rhs) FAIL(ALAMO_ERR_ARG, "null argument");
@@ -1937,6 +1963,27 @@ __global__ void k_pack_target(Geom g, int64_t V, const double* __restrict__ e, d
     if (v >= V) return;
     const int64_t m = node_offset(g, v);
     for (int c = 0; c < 3; c++) t[v * 3 + c] = (float)(e[c * g.tot + m] * inv_s);
+}
+// z (FP64 planes) = t[v][0..2]
+__global__ void k_unpack3(Geom g, int64_t V, const float* __restrict__ t, double* __restrict__ z) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int64_t m = node_offset(g, v);
+    for (int c = 0; c < 3; c++) z[c * g.tot + m] = (double)t[v * 3 + c];
+}
+// dy[v][c] = gy_c(node v) for c < 3, 0 for the other channels
+__global__ void k_pack_grad(Geom g, int64_t V, const double* __restrict__ gy, float* __restrict__ dy) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int64_t m = node_offset(g, v);
+#pragma unroll
+    for (int c = 0; c < 16; c++) dy[v * 16 + c] = c < 3 ? (float)gy[c * g.tot + m] : 0.f;
+}
+// out = a x + b y
+__global__ void k_lincomb(int64_t n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                          double* __restrict__ out) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        out[q] = a * x[q] + b * y[q];
 }
 __global__ void k_f32_to_bf16_host_layout(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
@@ -2127,7 +2174,10 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     if (o->npairs < 1 || o->steps < 1) FAIL(ALAMO_ERR_ARG, "npairs and steps must be positive");
     if (!h->prepared) TRY(prepare(h));
     if (h->L.size() < 2) FAIL(ALAMO_ERR_ARG, "training needs at least two multigrid levels");
+    if (o->loss != ALAMO_TRAIN_LOSS_ERROR && o->loss != ALAMO_TRAIN_LOSS_RESIDUAL) FAIL(ALAMO_ERR_ARG, "unknown loss");
+    const bool resloss = o->loss == ALAMO_TRAIN_LOSS_RESIDUAL;
     TRY(net_level0(n, h));
+    if (resloss) TRY(row_table(h));
     n->train_seed = o->seed;
     LevelBuf& l0 = h->L[0];
     // V network voxels; n3 elements of a level-0 vector
@@ -2136,7 +2186,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     std::vector<void*> tmp;
     auto talloc = [&](void** p, size_t b) -> int { return net_alloc(tmp, p, b); };
     int rc = ALAMO_OK;
-    double *r = nullptr, *z = nullptr, *q = nullptr, *e = nullptr;
+    double *r = nullptr, *z = nullptr, *q = nullptr, *e = nullptr, *qa = nullptr;
     std::vector<double*> lv(nlev, nullptr);
     uint16_t* X = nullptr;
     float* T = nullptr;
@@ -2144,6 +2194,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
     rc |= talloc((void**)&z, n3 * 8);
     rc |= talloc((void**)&q, n3 * 8);
     rc |= talloc((void**)&e, n3 * 8);
+    if (resloss) rc |= talloc((void**)&qa, 2 * n3 * 8);  // q, then A y: one block for the two dots
     for (int l = 1; l < nlev; l++) rc |= talloc((void**)&lv[l], 3 * h->L[l].g.tot * 8);
     rc |= talloc((void**)&X, (size_t)o->npairs * V * 16 * 2);
     rc |= talloc((void**)&T, (size_t)o->npairs * V * 3 * 4);
@@ -2167,25 +2218,46 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
             k_interp_add<double><<<interp_blocks(h->L[l + 1].g, h->L[l].g), 256, 0, h->stream>>>(h->L[l + 1].g, h->L[l].g, cur, fine);
             cur = fine;
         }
+        if (resloss && o->solve_max_iter > 0) {
+            // the residual loss trains on what the Krylov solve leaves: r becomes the residual after
+            // k in [0, solve_max_iter] plain FGMRES iterations on it (its slowest modes)
+            const int k = (int)((o->seed * 31ull + 7919ull * p) % (uint64_t)(o->solve_max_iter + 1));
+            if (k > 0) {
+                vfill<double>(e, n3, 0.0, h->stream);
+                alamo_krylov_opts kk = ko;
+                kk.max_iter = k;
+                alamo_solve_stats ss{};
+                const int src = fgmres_impl(h, e, r, 1e-30, 0.0, &kk, &ss);
+                if (src != ALAMO_OK && src != ALAMO_ERR_NOCONV) { rc = src; break; }
+                launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, e, r, q, 0.0, h->stream);
+                std::swap(r, q);
+            }
+        }
         // z = V(r), q = r - A z, e = A^-1 q
         rc = precond_apply(h, ALAMO_PRECOND_MG, r, z);
         if (rc) break;
         launch_node<double>(h->sym, RESID, l0.g, h->bc, l0.C, z, r, q, 0.0, h->stream);
+        double ql2 = 0;
+        rc = norms_dev<double>(h, q, n3, nullptr, &ql2);
+        if (rc) break;
+        const double s = std::max(ql2 / std::sqrt(3.0 * (double)V), 1e-300);
+        net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, q, 1.0 / s, n->B.feat,
+                                                           X + (int64_t)p * V * 16);
+        if (resloss) {
+            // the residual loss needs q only
+            k_pack_target<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, q, 1.0 / s, T + (int64_t)p * V * 3);
+            if (o->verbose) fprintf(stderr, "alamo train: pair %d generated\n", p);
+            continue;
+        }
         vfill<double>(e, n3, 0.0, h->stream);
         alamo_solve_stats ss{};
         const int src = fgmres_impl(h, e, q, o->solve_tol, 0.0, &ko, &ss);
         if (src != ALAMO_OK && src != ALAMO_ERR_NOCONV) { rc = src; break; }
         solved += ss.converged;
-        double ql2 = 0;
-        rc = norms_dev<double>(h, q, n3, nullptr, &ql2);
-        if (rc) break;
         double el2 = 0;
         rc = norms_dev<double>(h, e, n3, nullptr, &el2);
         if (rc) break;
-        const double s = std::max(ql2 / std::sqrt(3.0 * (double)V), 1e-300);
         const double se = std::max(el2 / std::sqrt(3.0 * (double)V), 1e-300);
-        net::k_pack_input<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, q, 1.0 / s, n->B.feat,
-                                                           X + (int64_t)p * V * 16);
         k_pack_target<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, e, 1.0 / se, T + (int64_t)p * V * 3);
         if (o->verbose) fprintf(stderr, "alamo train: pair %d generated (solve %d its)\n", p, ss.iterations);
     }
@@ -2199,16 +2271,45 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
         HIPCHK(hipMemcpyAsync(n->B.x, X + (int64_t)p * V * 16, V * 16 * 2, hipMemcpyDeviceToDevice, n->s));
         rc = net_forward(n);
         if (rc) break;
-        HIPCHK(hipMemsetAsync(n->B.loss, 0, 4, n->s));
-        net::k_mse<<<(unsigned)std::min<int64_t>(1024, (V + 255) / 256), 256, 0, n->s>>>(
-            V, n->B.y, T + (int64_t)p * V * 3, n->B.dy, n->B.loss);
+        const bool rep = step == 0 || step == o->steps - 1 || (o->verbose && step % 50 == 0);
+        double lv_ = 0;
+        if (resloss) {
+            // L = 1 - c <q, A y> / <q, q> with c = <q, A y> / <A y, A y>;
+            // dL/d(A y) = -(2 c / <q, q>) (q - c A y), and dL/dy = A^T dL/d(A y)
+            double *qv = qa, *Ay = qa + n3, *yv = z, *gu = r, *gy = e;
+            net::k_unpack_output<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, n->B.y, 1.0, yv, 0);
+            k_unpack3<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, T + (int64_t)p * V * 3, qv);
+            launch_node<double>(h->sym, APPLY, l0.g, h->bc, l0.C, yv, nullptr, Ay, 0.0, h->stream);
+            double d[2], qq = 0;
+            rc = multidot(h, Ay, qv, n3, 2, d);  // <A y, q>, <A y, A y>
+            if (rc) break;
+            rc = multidot(h, qv, qv, n3, 1, &qq);
+            if (rc) break;
+            if (!std::isfinite(d[0]) || !std::isfinite(d[1])) { rc = ALAMO_ERR_NAN; break; }
+            if (d[1] > 0.0 && qq > 0.0) {
+                const double c = d[0] / d[1];
+                lv_ = 1.0 - c * d[0] / qq;
+                k_lincomb<<<gs_blocks(n3), 256, 0, h->stream>>>(n3, -2.0 * c / qq, qv, 2.0 * c * c / qq, Ay, gu);
+                k_row_apply_t<<<nblocks(l0.g.tot), 256, 0, h->stream>>>(l0.g, h->rowtab, gu, gy);
+                k_pack_grad<<<nb256(V), 256, 0, h->stream>>>(l0.g, V, gy, n->B.dy);
+            } else {
+                lv_ = 1.0;
+                HIPCHK(hipMemsetAsync(n->B.dy, 0, V * 16 * 4, n->s));
+            }
+        } else {
+            HIPCHK(hipMemsetAsync(n->B.loss, 0, 4, n->s));
+            net::k_mse<<<(unsigned)std::min<int64_t>(1024, (V + 255) / 256), 256, 0, n->s>>>(
+                V, n->B.y, T + (int64_t)p * V * 3, n->B.dy, n->B.loss);
+        }
         rc = net_backward(n, false);
         if (rc) break;
-        const bool rep = step == 0 || step == o->steps - 1 || (o->verbose && step % 50 == 0);
         if (rep) {
-            float lv_ = 0;
-            HIPCHK(hipMemcpyAsync(&lv_, n->B.loss, 4, hipMemcpyDeviceToHost, n->s));
-            HIPCHK(hipStreamSynchronize(n->s));
+            if (!resloss) {
+                float lf = 0;
+                HIPCHK(hipMemcpyAsync(&lf, n->B.loss, 4, hipMemcpyDeviceToHost, n->s));
+                HIPCHK(hipStreamSynchronize(n->s));
+                lv_ = lf;
+            }
             if (step == 0) first = lv_;
             last = lv_;
             if (o->verbose) fprintf(stderr, "alamo train: step %d loss %.6e\n", step, lv_);

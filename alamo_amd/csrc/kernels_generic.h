@@ -112,6 +112,51 @@ __global__ void __launch_bounds__(256) k_l1_diag(Geom g, BCTable bc, const T* __
     for (int p = 0; p < 3; p++) out[p * g.tot + m] = D[p];
 }
 
+// Explicit rows of the level's operator (no psi terms): tab[(3 m + q) 81 + 3 o + c] is the weight of
+// u_c at offset slot o (13 + dx + 3 dy + 9 dz) in row component q of node m.
+template <int SYM>
+__global__ void __launch_bounds__(256) k_row_table(Geom g, BCTable bc, const double* __restrict__ C,
+                                                   double* __restrict__ tab) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
+    row_coeffs<double, SYM>(g, bc, C, i, j, k, [&](int q, const double(&acc)[27][3]) {
+        double* t = tab + (3 * m + q) * 81;
+        for (int o = 0; o < 27; o++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) t[3 * o + c] = acc[o][c];
+    });
+}
+
+// out = A^T r from the row table: out_d(n) = sum over slots o and components q of the weight of
+// u_d(n) in row q of node n - offset(o), times r_q(n - offset(o)).  The stencil reaches one node,
+// so these 27 rows are all that hold a weight of u(n).
+__global__ void __launch_bounds__(256) k_row_apply_t(Geom g, const double* __restrict__ tab,
+                                                     const double* __restrict__ r, double* __restrict__ out) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.tot) return;
+    int i, j, k;
+    node_ijk(g, m, i, j, k);
+    if (i >= g.N[0]) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int o = 0; o < 27; o++) {
+        const int a = i - (o % 3 - 1), b = j - ((o / 3) % 3 - 1), c = k - (o / 9 - 1);
+        if (a < 0 || b < 0 || c < 0 || a > g.n[0] || b > g.n[1] || c > g.n[2]) continue;
+        const int64_t mr = a + g.sy * b + g.sz * c;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double rq = r[q * g.tot + mr];
+            const double* t = tab + (3 * mr + q) * 81 + 3 * o;
+#pragma unroll
+            for (int d = 0; d < 3; d++) s[d] += t[d] * rq;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) out[d * g.tot + m] = s[d];
+}
+
 // ---------------------------------------------------------------------------
 // Reductions: per-block partial (max |x|, sum x^2) or (sum a*b), wavefront
 // shuffles inside a 64-wide wavefront, LDS across the 4 wavefronts of a block.

@@ -189,8 +189,10 @@ __device__ __forceinline__ T l1_finish(const T acc[27][3], int q) {
     return acc[13][q] < T(0) ? -s : s;
 }
 
-template <typename T, int SYM>
-__device__ void l1_row(const Geom& g, const BCTable& bc, const T* __restrict__ C, int i, int j, int k, T Dl1[3]) {
+// The rows of the operator at node (i, j, k), no psi terms: sink(q, acc) for each row component q,
+// acc[o][c] the weight of u_c at offset slot o.
+template <typename T, int SYM, typename Sink>
+__device__ void row_coeffs(const Geom& g, const BCTable& bc, const T* __restrict__ C, int i, int j, int k, Sink&& sink) {
     constexpr int NC = NCoef<SYM>::value;
     const int64_t m = i + g.sy * j + g.sz * k;
     const int hi[3] = {g.n[0], g.n[1], g.n[2]};
@@ -234,7 +236,7 @@ __device__ void l1_row(const Geom& g, const BCTable& bc, const T* __restrict__ C
             } else if (t == NEUMANN) {
                 for (int l = 0; l < 3; l++) d1w(l, T(nd[l]), q);
             }
-            Dl1[q] = l1_finish(acc, q);
+            sink(q, acc);
         }
         return;
     }
@@ -271,8 +273,13 @@ __device__ void l1_row(const Geom& g, const BCTable& bc, const T* __restrict__ C
                         acc[13 - sl[ll]][kk] -= w;
                     }
                 }
-        Dl1[q] = l1_finish(acc, q);
+        sink(q, acc);
     }
+}
+
+template <typename T, int SYM>
+__device__ void l1_row(const Geom& g, const BCTable& bc, const T* __restrict__ C, int i, int j, int k, T Dl1[3]) {
+    row_coeffs<T, SYM>(g, bc, C, i, j, k, [&](int q, const T(&acc)[27][3]) { Dl1[q] = l1_finish(acc, q); });
 }
 
 // (A u) at an interior node, generic form (any SYM), direct global loads.

@@ -20,6 +20,7 @@ from .problems import ISO, MAJORMINOR, DIAGONAL, NCOEF
 
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 PRECOND_NONE, PRECOND_MG, PRECOND_LEARNED = 0, 1, 2
+LOSS_ERROR, LOSS_RESIDUAL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 CYCLE_V, CYCLE_F = 0, 1
 
@@ -580,6 +581,12 @@ def precond_apply(op, kind, r, z):
     check(lib().alamo_mg_precond(op.h, int(kind), op._vec(r, 0, "precond r"), op._vec(z, 0, "precond z")))
 
 
+def apply_transpose(op, r, out):
+    """out = A^T r for the level-0 operator (no reference counterpart; no psi terms)"""
+    check(lib().alamo_mg_apply_transpose(op.h, op._vec(r, 0, "apply_transpose r"),
+                                         op._vec(out, 0, "apply_transpose out")))
+
+
 class Krylov:
     """Flexible GMRES(restart) around the operator's multigrid cycle (no reference counterpart:
     an outer accelerator for the same MLMG-style cycle, stopping on the reference's max-norm test)."""
@@ -685,8 +692,14 @@ class Net:
     def adam_step(self, lr):
         check(lib().alamo_net_adam_step(self.h, float(lr)))
 
-    def train(self, op, npairs=8, steps=200, lr=1e-3, seed=1, solve_max_iter=60, solve_tol=1e-6, verbose=0):
-        o = _lib.TrainOpts(npairs, steps, lr, seed, solve_max_iter, solve_tol, verbose)
+    def train(self, op, npairs=8, steps=200, lr=1e-3, seed=1, solve_max_iter=60, solve_tol=1e-6, verbose=0,
+              loss="error"):
+        """Adam on pairs generated with op: loss "error" (MSE against the cycle's remaining error, solved
+        per pair) or "residual" (1 - cos^2(q, A y), the stage's squared residual ratio)"""
+        losses = {"error": LOSS_ERROR, "residual": LOSS_RESIDUAL}
+        if loss not in losses:
+            raise _lib.AlamoError(1, "loss must be one of %s" % sorted(losses))
+        o = _lib.TrainOpts(npairs, steps, lr, seed, solve_max_iter, solve_tol, verbose, losses[loss])
         st = _lib.TrainStats()
         check(lib().alamo_net_train(self.h, op.h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()

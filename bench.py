@@ -69,6 +69,9 @@ def parse():
     p.add_argument("--train-fields", type=int, default=2)
     p.add_argument("--train-pairs", type=int, default=8, help="(residual, error) pairs per training field")
     p.add_argument("--train-steps", type=int, default=300, help="Adam steps per training field")
+    p.add_argument("--train-loss", default="residual", choices=["residual", "error"],
+                   help="network training loss: the stage's squared residual ratio 1 - cos^2(q, A y), or the "
+                        "MSE against the cycle's remaining error (one FGMRES solve per pair)")
     p.add_argument("--verbose", type=int, default=0)
     p.add_argument("--fapply-only", action="store_true",
                    help="only time Fapply on the finest level (plus a device copy of the same bytes), "
@@ -441,9 +444,10 @@ def train_network(a, P, Elastic, Net, precision, cycle, smoother, coarse_op):
                      coarse_op=coarse_op)
         op.SetModel(pr["C"])
         stats.append(net.train(op, npairs=a.train_pairs, steps=a.train_steps, lr=1e-3, seed=77 + f,
-                               solve_max_iter=100, solve_tol=1e-6, verbose=a.verbose))
+                               solve_max_iter=0 if a.train_loss == "residual" else 100, solve_tol=1e-6,
+                               verbose=a.verbose, loss=a.train_loss))
         del op
-    info = {"fields": a.train_fields, "cells": a.train_n, "pairs_per_field": a.train_pairs,
+    info = {"loss": a.train_loss, "fields": a.train_fields, "cells": a.train_n, "pairs_per_field": a.train_pairs,
             "steps_per_field": a.train_steps, "first_loss": stats[0]["first_loss"],
             "last_loss": stats[-1]["last_loss"], "seconds": time.time() - t0}
     print("network trained: %s" % json.dumps(info), file=sys.stderr, flush=True)

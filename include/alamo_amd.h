@@ -316,9 +316,15 @@ typedef struct {
     double lr;                 /* Adam learning rate */
     uint64_t seed;             /* training-set seed (stored in checkpoints) */
     int solve_max_iter;        /* FGMRES iterations allowed for each pair's error */
-    double solve_tol;          /* relative tolerance of those solves */
+    double solve_tol;          /* relative tolerance of those solves (ALAMO_TRAIN_LOSS_ERROR) */
     int verbose;
+    int loss;                  /* ALAMO_TRAIN_LOSS_* (0, the error loss, when zero-initialised) */
 } alamo_train_opts;
+/* ALAMO_TRAIN_LOSS_ERROR: mean |y - e / rms(e)|^2 against e = A^-1 q, solved per pair by FGMRES.
+ * ALAMO_TRAIN_LOSS_RESIDUAL: 1 - <q, A y>^2 / (|q|^2 |A y|^2), the squared residual ratio
+ * |q - alpha A y| / |q| the stage reaches with its residual-minimising alpha; needs no solves,
+ * back-propagated through A^T (alamo_mg_apply_transpose's row table; no psi terms). */
+enum { ALAMO_TRAIN_LOSS_ERROR = 0, ALAMO_TRAIN_LOSS_RESIDUAL = 1 };
 typedef struct {
     int pairs, pairs_converged, steps;
     double first_loss, last_loss, seconds_data, seconds_train;
@@ -346,6 +352,11 @@ int alamo_mg_set_net(alamo_mg* h, alamo_net* net);
  * from zero, the operator's MLMG settings; ALAMO_PRECOND_LEARNED: that cycle plus the network
  * stage), device pointers, synchronous */
 int alamo_mg_precond(alamo_mg* h, int kind, const double* r, double* z);
+/* out = A^T r for the level-0 operator (no reference counterpart: the reference applies A only).
+ * Builds an explicit table of A's rows (243 doubles per node, kept with the handle; no psi terms)
+ * from the current coefficients and boundary table on every call, then gathers.  Device vectors
+ * of 3 planes, r and out distinct; synchronous.  Used by ALAMO_TRAIN_LOSS_RESIDUAL. */
+int alamo_mg_apply_transpose(alamo_mg* h, const double* r, double* out);
 /* mean forward-kernel time of each of the 6 convolution layers at h's level-0 size, HIP events on
  * h's stream, reps launches each: ms[6] (for throughput and roofline reports) */
 int alamo_net_layer_times(alamo_net* net, alamo_mg* h, int reps, double* ms);

@@ -260,8 +260,9 @@ def _stage_gains(op, net, n, cuda, nrhs=3):
     return gains
 
 
-def _trained_net_and_field(ora):
-    """a network trained on one 33^3 1:1000 two-phase field, and a held-out field of the same kind"""
+def _trained_net_and_field(ora, loss="error"):
+    """a network trained (on the given loss) on one 33^3 1:1000 two-phase field, and a held-out field of
+    the same kind"""
     n = 32
     pr_train, bc_train, _ = two_phase_rhs(ora, n, seed=101)
     pr, bc, rhs = two_phase_rhs(ora, n, seed=7)
@@ -270,8 +271,11 @@ def _trained_net_and_field(ora):
     op_train.SetModel(pr_train["C"])
     net = Net(seed=4)
     print("train", flush=True)
-    st = net.train(op_train, npairs=8, steps=300, lr=1e-3, seed=5)
-    print("train stats", st, flush=True)
+    st = net.train(op_train, npairs=8, steps=300, lr=1e-3, seed=5, loss=loss)
+    print("train stats", loss, st, flush=True)
+    assert st["pairs"] == 8 and np.isfinite(st["last_loss"])
+    if loss == "residual":
+        assert st["last_loss"] < st["first_loss"]
     op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, precision=PREC_MIXED, cycle=CYCLE_V, pre_smooth=1,
                  post_smooth=1)
     op.SetModel(pr["C"])
@@ -289,6 +293,20 @@ def test_training_makes_the_stage_remove_residual(cuda, ora):
     assert max(trained) <= 1.0 + 1e-10 and max(untrained) <= 1.0 + 1e-10
     assert np.mean(trained) <= 0.95
     assert np.mean(trained) <= np.mean(untrained) - 0.03
+
+
+def test_residual_loss_training(cuda, ora):
+    """Training on the residual loss 1 - cos^2(q, A y) (back-propagated through A^T) lowers it, and on
+    a held-out field the trained stage removes more of the residual the cycle leaves than the same
+    network untrained."""
+    net, op, pr, rhs, n = _trained_net_and_field(ora, loss="residual")
+    trained = _stage_gains(op, net, n, cuda)
+    untrained = _stage_gains(op, Net(seed=4), n, cuda)
+    print("residual loss: after cycle + stage / after cycle: trained", trained, "untrained", untrained, flush=True)
+    assert max(trained) <= 1.0 + 1e-10
+    assert np.mean(trained) <= np.mean(untrained) - 0.03
+    with pytest.raises(AlamoError):
+        net.train(op, loss="mse")
 
 
 def test_learned_fgmres_iterations_vs_plain(cuda, ora):
