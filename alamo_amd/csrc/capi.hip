@@ -416,13 +416,15 @@ static void bottom(alamo_mg* h) {
         k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(lb.cor, lb.corf, n);
 }
 
+// guess: the top level's cor already holds an initial guess (its pre-smoothing sweeps start from it)
 template <typename T>
-static void vcycle(alamo_mg* h, int top, bool first_done = false) {
+static void vcycle(alamo_mg* h, int top, bool first_done = false, bool guess = false) {
     const int bot = (int)h->L.size() - 1;
     bool done = first_done;
     for (int l = top; l < bot; l++) {
         LevelBuf& lb = h->L[l];
-        presmooth_from_zero<T>(h, l, done);
+        if (guess && l == top) sweeps<T>(h, l, h->d.pre_smooth * sps(h, l));
+        else presmooth_from_zero<T>(h, l, done);
         level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
         LevelBuf& lc = h->L[l + 1];
         // the restriction also applies the next level's first pre-smoothing sweep (from zero)
@@ -464,16 +466,18 @@ static void fcycle(alamo_mg* h) {
     }
 }
 
-// z = F-cycle(v) from a zero guess for a preconditioner, v and z in the cycle precision: the same
-// steps as fcycle on a copy of v, without the copies.  The first restriction reads v in place; on
-// level 0 the initial guess P cor_1 goes straight into the hold buffer (no clearing pass, and
-// 0 + x = x exactly), the level's right-hand side is v - A hold, and the last step writes
-// z = cor + hold.  Bit-identical to copy, fcycle, copy.
+// z = F-cycle(v) from a zero guess for a preconditioner, v and z in the cycle precision: the steps
+// of fcycle with each level's V-cycle run on the solution itself instead of on a correction --
+// the level's sweeps start from u = P cor_{l+1} with the level's own right-hand side, where fcycle
+// forms res - A hold, smooths a correction from zero and adds hold back.  The smoother is affine
+// in u, so both are the same cycle up to rounding, without the residual, the first from-zero pass
+// and the final addition on every level.  The first restriction reads v in place; on level 0 the
+// sweeps read v in place and the solution lives in z (cor takes z's place when the level's sweep
+// count is even, its partner's when odd, so the last sweep writes z).
 template <typename T>
 static void fcycle_into(alamo_mg* h, const T* v, T* z) {
     const int bot = (int)h->L.size() - 1;
     LevelBuf& l0 = h->L[0];
-    const int64_t n = 3 * l0.g.tot;
     LevelBuf& l1 = h->L[1];
     k_restrict<T><<<nblocks(l1.g.tot), 256, 0, h->stream>>>(l1.g, l0.g, 3, v, LB<T>::res(l1));
     for (int l = 2; l <= bot; l++) {
@@ -485,20 +489,23 @@ static void fcycle_into(alamo_mg* h, const T* v, T* z) {
     for (int l = bot - 1; l >= 1; l--) {
         LevelBuf& lb = h->L[l];
         LevelBuf& lc = h->L[l + 1];
-        const int64_t nl = 3 * lb.g.tot;
-        vfill<T>(LB<T>::cor(lb), nl, T(0), h->stream);
-        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
-        level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
-        std::swap(LB<T>::res(lb), LB<T>::rescor(lb));
-        std::swap(LB<T>::hold(lb), LB<T>::cor(lb));
-        vcycle<T>(h, l);
-        k_add<T><<<gs_blocks(nl), 256, 0, h->stream>>>(LB<T>::cor(lb), LB<T>::cor(lb), LB<T>::hold(lb), nl);
+        k_interp_add<T, true><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc),
+                                                                                 LB<T>::cor(lb));
+        vcycle<T>(h, l, false, true);
     }
+    T* own_cor = LB<T>::cor(l0);
+    T* own_tmp = LB<T>::tmp(l0);
+    T* own_res = LB<T>::res(l0);
+    const int nsw = (h->d.pre_smooth + h->d.post_smooth) * sps(h, 0);
+    if (nsw % 2) LB<T>::tmp(l0) = z;
+    else LB<T>::cor(l0) = z;
+    LB<T>::res(l0) = const_cast<T*>(v);
     k_interp_add<T, true><<<interp_blocks(l1.g, l0.g), 256, 0, h->stream>>>(l1.g, l0.g, LB<T>::cor(l1),
-                                                                             LB<T>::hold(l0));
-    level_op<T>(h, 0, RESID, LB<T>::hold(l0), v, LB<T>::res(l0), T(0), h->stream);
-    vcycle<T>(h, 0);
-    k_add<T><<<gs_blocks(n), 256, 0, h->stream>>>(z, LB<T>::cor(l0), LB<T>::hold(l0), n);
+                                                                             LB<T>::cor(l0));
+    vcycle<T>(h, 0, false, true);
+    LB<T>::cor(l0) = own_cor;
+    LB<T>::tmp(l0) = own_tmp;
+    LB<T>::res(l0) = own_res;
 }
 
 // amrex::MLMG::oneIter: an F-cycle while iter < max_fmg_iter, then V-cycles (the reference sets

@@ -209,10 +209,13 @@ def test_krylov_solve_eshelby(cuda, ora, precision, cycle, precond):
 
 
 @pytest.mark.parametrize("opts", [dict(), dict(smoother="l1", sweeps_per_smooth=3, sweeps_finest=2,
-                                               pre_smooth=1, post_smooth=1, bottom_solver="dense")])
+                                               pre_smooth=1, post_smooth=1, bottom_solver="dense"),
+                                  dict(pre_smooth=2, post_smooth=1, sweeps_per_smooth=1)])
 def test_fcycle_preconditioner_is_one_fcycle(cuda, ora, opts):
-    """The F-cycle preconditioner (alamo_mg_precond; reads r in place, writes z = cor + hold without
-    staging copies) equals one F-cycle of the multigrid solve from a zero guess bit for bit."""
+    """The F-cycle preconditioner (alamo_mg_precond: each level's V-cycle runs on the solution from
+    the interpolated coarser solution, r is read in place and the last level-0 sweep writes z, for
+    even and odd level-0 sweep counts) equals one F-cycle of the multigrid solve from a zero guess,
+    which smooths corrections, to FP64 rounding."""
     from alamo_amd.operator import precond_apply, PRECOND_MG
     pr, bc, rhs = eshelby_rhs(ora, 32)
     op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, cycle=1, max_iter=10, **opts)
@@ -226,7 +229,10 @@ def test_fcycle_preconditioner_is_one_fcycle(cuda, ora, opts):
     x = to_dev(np.zeros_like(rhs), cuda)
     st = lin.solve(x, r, 1e-14, 0.0, raise_on_fail=False)
     assert st["iterations"] == 1
-    np.testing.assert_array_equal(to_host(z), to_host(x))
+    zx, xx = to_host(z), to_host(x)
+    err = np.abs(zx - xx).max()
+    print("preconditioner vs one F-cycle: max diff %.3e of %.3e" % (err, np.abs(xx).max()))
+    assert err <= 1e-12 * np.abs(xx).max()
     np.testing.assert_array_equal(to_host(r), rhs)  # the right-hand side is only read
 
 
