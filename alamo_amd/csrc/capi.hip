@@ -437,7 +437,7 @@ static void vcycle(alamo_mg* h, int top, bool first_done = false, bool guess = f
     for (int l = bot - 1; l >= top; l--) {
         LevelBuf& lb = h->L[l];
         LevelBuf& lc = h->L[l + 1];
-        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        launch_interp<T>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb), h->stream);
         smooth<T>(h, l, h->d.post_smooth);
     }
 }
@@ -456,7 +456,7 @@ static void fcycle(alamo_mg* h) {
         LevelBuf& lc = h->L[l + 1];
         const int64_t n = 3 * lb.g.tot;
         vfill<T>(LB<T>::cor(lb), n, T(0), h->stream);
-        k_interp_add<T><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb));
+        launch_interp<T>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb), h->stream);
         // res = res - L(cor)
         level_op<T>(h, l, RESID, LB<T>::cor(lb), LB<T>::res(lb), LB<T>::rescor(lb), T(0), h->stream);
         std::swap(LB<T>::res(lb), LB<T>::rescor(lb));
@@ -489,8 +489,7 @@ static void fcycle_into(alamo_mg* h, const T* v, T* z) {
     for (int l = bot - 1; l >= 1; l--) {
         LevelBuf& lb = h->L[l];
         LevelBuf& lc = h->L[l + 1];
-        k_interp_add<T, true><<<interp_blocks(lc.g, lb.g), 256, 0, h->stream>>>(lc.g, lb.g, LB<T>::cor(lc),
-                                                                                 LB<T>::cor(lb));
+        launch_interp<T, true>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb), h->stream);
         vcycle<T>(h, l, false, true);
     }
     T* own_cor = LB<T>::cor(l0);
@@ -500,8 +499,7 @@ static void fcycle_into(alamo_mg* h, const T* v, T* z) {
     if (nsw % 2) LB<T>::tmp(l0) = z;
     else LB<T>::cor(l0) = z;
     LB<T>::res(l0) = const_cast<T*>(v);
-    k_interp_add<T, true><<<interp_blocks(l1.g, l0.g), 256, 0, h->stream>>>(l1.g, l0.g, LB<T>::cor(l1),
-                                                                             LB<T>::cor(l0));
+    launch_interp<T, true>(l1.g, l0.g, LB<T>::cor(l1), LB<T>::cor(l0), h->stream);
     vcycle<T>(h, 0, false, true);
     LB<T>::cor(l0) = own_cor;
     LB<T>::tmp(l0) = own_tmp;
@@ -1702,7 +1700,7 @@ int alamo_mg_interpolate(alamo_mg* h, int amrlev, int fmglev, double* fine, cons
     if (fmglev >= (int)h->L.size() - 1) FAIL(ALAMO_ERR_ARG, "fmglev must have a coarser level");
     LevelBuf& lc = h->L[fmglev + 1];
     LevelBuf& lf = h->L[fmglev];
-    k_interp_add<double><<<interp_blocks(lc.g, lf.g), 256, 0, (hipStream_t)stream>>>(lc.g, lf.g, crse, fine);
+    launch_interp<double>(lc.g, lf.g, crse, fine, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return ALAMO_OK;
 }
@@ -2222,7 +2220,7 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
         for (int l = lc - 1; l >= 0; l--) {
             double* fine = l == 0 ? r : lv[l];
             vfill<double>(fine, 3 * h->L[l].g.tot, 0.0, h->stream);
-            k_interp_add<double><<<interp_blocks(h->L[l + 1].g, h->L[l].g), 256, 0, h->stream>>>(h->L[l + 1].g, h->L[l].g, cur, fine);
+            launch_interp<double>(h->L[l + 1].g, h->L[l].g, cur, fine, h->stream);
             cur = fine;
         }
         if (resloss && o->solve_max_iter > 0) {
