@@ -113,6 +113,8 @@ struct LevelBuf {
     double* gtab = nullptr;
     float* gtabf = nullptr;
     int ngrains_cap = 0;
+    double* mmd = nullptr;  // MajorMinor gradient contraction (18 planes), prepared with the level
+    float* mmdf = nullptr;
 };
 
 }  // namespace
@@ -232,16 +234,21 @@ static void launch_node(int sym, int mode, const Geom& g, const BCTable& bc, con
             // a per-grain model whose table fits the LDS budget reads it from there
             const size_t tabb = (size_t)21 * g.ngrains * sizeof(T);
             const bool grains = g.gid && tabb <= MM_GRAIN_LDS_MAX;
+            // a prepared level reads its node-local gradient contraction instead of differencing C
+            const bool dcon = std::is_same<T, double>::value ? g.mmd != nullptr : g.mmdf != nullptr;
             if (mode == APPLY) {
                 if (grains) k_mm_interior_gid<T, APPLY><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else if (dcon) k_mm_interior_d<T, APPLY><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 else k_mm_interior<T, APPLY><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, APPLY><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             } else if (mode == RESID) {
                 if (grains) k_mm_interior_gid<T, RESID><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else if (dcon) k_mm_interior_d<T, RESID><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 else k_mm_interior<T, RESID><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, RESID><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             } else {
                 if (grains) k_mm_interior_gid<T, JACOBI><<<gi, 256, tabb, s>>>(g, u, b, out, omega);
+                else if (dcon) k_mm_interior_d<T, JACOBI><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 else k_mm_interior<T, JACOBI><<<gi, 256, 0, s>>>(g, C, u, b, out, omega);
                 k_boundary<T, MAJORMINOR, JACOBI><<<gb, 256, 0, s>>>(g, bc, C, u, b, out, omega, nb);
             }
@@ -693,6 +700,22 @@ static int prepare(alamo_mg* h) {
             k_convert<double, float><<<gs_blocks(h->nc * lb.g.tot), 256, 0, h->stream>>>(lb.C, lb.Cf,
                                                                                          h->nc * lb.g.tot);
             k_convert<double, float><<<gs_blocks(3 * lb.g.tot), 256, 0, h->stream>>>(lb.D, lb.Df, 3 * lb.g.tot);
+        }
+        // MajorMinor levels with coefficient planes: the node-local gradient contraction their
+        // interior rows read
+        lb.g.mmd = nullptr;
+        lb.g.mmdf = nullptr;
+        const int64_t ni = (int64_t)(lb.g.N[0] - 2) * (lb.g.N[1] - 2) * (lb.g.N[2] - 2);
+        if (h->sym == MAJORMINOR && !st && ni > 0) {
+            if (!lb.mmd) TRY(dalloc(h, &lb.mmd, 18 * lb.g.tot));
+            k_mm_dcoef<<<nblocks(ni), 256, 0, h->stream>>>(lb.g, lb.C, lb.mmd);
+            lb.g.mmd = lb.mmd;
+            if (h->d.precision == ALAMO_PREC_MIXED) {
+                if (!lb.mmdf) TRY(dalloc(h, &lb.mmdf, 18 * lb.g.tot));
+                k_convert<double, float><<<gs_blocks(18 * lb.g.tot), 256, 0, h->stream>>>(lb.mmd, lb.mmdf,
+                                                                                          18 * lb.g.tot);
+                lb.g.mmdf = lb.mmdf;
+            }
         }
     }
     HIPCHK(hipGetLastError());
@@ -1422,6 +1445,10 @@ static int set_model_impl(alamo_mg* h, int amrlev, const double* C, int layout, 
     h->model_set = true;
     h->prepared = false;
     h->model_version = ++g_model_version;
+    for (LevelBuf& lb : h->L) {  // the gradient contractions follow the model at the next preparation
+        lb.g.mmd = nullptr;
+        lb.g.mmdf = nullptr;
+    }
     l0.g.gid = nullptr;  // a full model replaces a per-grain one
     l0.g.gtab = nullptr;
     l0.g.gtabf = nullptr;

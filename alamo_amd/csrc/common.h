@@ -57,6 +57,11 @@ struct Geom {
     // so a sweep is x += (b - A x) / dsm
     const double* dsm;
     const float* dsmf;
+    // MajorMinor levels after preparation: the node-local contraction of the stiffness gradient,
+    // D_ikl = sum_j d_j C_ijkl (central differences; symmetric in k, l), 18 planes i * 6 + pair(k, l),
+    // FP64 and FP32 copies; null when not prepared (the rows then difference C themselves)
+    const double* mmd;
+    const float* mmdf;
 };
 
 // Boundary types per region and component, packed for kernel arguments.

@@ -520,6 +520,44 @@ __global__ void __launch_bounds__(256) k_mm_interior(Geom g, const T* __restrict
     }
 }
 
+template <typename T> __device__ __forceinline__ const T* mm_contraction(const Geom& g);
+template <> __device__ __forceinline__ const double* mm_contraction<double>(const Geom& g) { return g.mmd; }
+template <> __device__ __forceinline__ const float* mm_contraction<float>(const Geom& g) { return g.mmdf; }
+
+// MajorMinor interior rows from the prepared gradient contraction (node-local coefficients only)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_mm_interior_d(Geom g, const T* __restrict__ C, const T* __restrict__ u,
+                                                       const T* __restrict__ b, T* __restrict__ out, T omega) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t I0 = g.N[0] - 2, I1 = g.N[1] - 2;
+    if (q >= I0 * I1 * (g.N[2] - 2)) return;
+    const int i = 1 + (int)(q % I0), j = 1 + (int)((q / I0) % I1), k = 1 + (int)(q / (I0 * I1));
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+    T f[3], D[3];
+    apply_interior_mm_d<T>(g, C, mm_contraction<T>(g), u, m, f, MODE == JACOBI ? D : nullptr);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const int64_t qq = p * g.tot + m;
+        if (MODE == APPLY) out[qq] = f[p];
+        else if (MODE == RESID) out[qq] = b[qq] - f[p];
+        else out[qq] = jacobi_update(g, qq, u[qq], b[qq], f[p], D[p], omega);
+    }
+}
+
+// the 18 planes of D_ikl = sum_j d_j C_ijkl on a MajorMinor level's interior nodes (boundary
+// nodes keep what the buffer holds; no row reads them there)
+__global__ void __launch_bounds__(256) k_mm_dcoef(Geom g, const double* __restrict__ C, double* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t I0 = g.N[0] - 2, I1 = g.N[1] - 2;
+    if (q >= I0 * I1 * (g.N[2] - 2)) return;
+    const int i = 1 + (int)(q % I0), j = 1 + (int)((q / I0) % I1), k = 1 + (int)(q / (I0 * I1));
+    const int64_t m = i + g.sy * j + g.sz * (int64_t)k;
+    double d[18];
+    mm_grad_contraction<double>(g, C, m, d);
+#pragma unroll
+    for (int e = 0; e < 18; e++) out[(int64_t)e * g.tot + m] = d[e];
+}
+
 // The same rows for a model given per grain: the grain table is staged in LDS once per workgroup
 // and each slot's seven reads come from it by the grain ids of the node and its six neighbours
 // (2-byte loads) instead of from 21 coefficient planes.  Same arithmetic in the same order as

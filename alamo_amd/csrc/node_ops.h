@@ -352,7 +352,7 @@ __host__ __device__ constexpr int mm_pb(int P) { return P == 0 ? 0 : P == 1 ? 1 
 // Terms of slot (pair I, pair J), I <= J: C_ijkl = c for every ordered (i, j) of one pair and
 // ordered (k, l) of the other; each adds c u_k,lj + (d_j C) u_k,l to f_i (and c w_j to D_i on the
 // diagonal slots).  All indices are template constants.
-template <int I, int J, typename T>
+template <int I, int J, typename T, bool DC = true>
 __device__ __forceinline__ void mm_slot_terms(T c, const T dc[3], const T gg[3][3][3], const T G[3][3], T f[3],
                                               T* D, const T w[3]) {
     constexpr int ia = mm_pa(I), ib = mm_pb(I), ka = mm_pa(J), kb = mm_pb(J);
@@ -363,8 +363,13 @@ __device__ __forceinline__ void mm_slot_terms(T c, const T dc[3], const T gg[3][
 #pragma unroll
         for (int t = 0; t < nk; t++) {
             const int k = t == 0 ? ka : kb, l = t == 0 ? kb : ka;
-            f[i] += c * gg[k][l][j] + dc[j] * G[k][l];
-            if (I != J) f[k] += c * gg[i][j][l] + dc[l] * G[i][j];
+            if (DC) {
+                f[i] += c * gg[k][l][j] + dc[j] * G[k][l];
+                if (I != J) f[k] += c * gg[i][j][l] + dc[l] * G[i][j];
+            } else {
+                f[i] += c * gg[k][l][j];
+                if (I != J) f[k] += c * gg[i][j][l];
+            }
         }
         if (I == J && D) D[i] += c * w[j];
     }
@@ -455,6 +460,80 @@ __device__ __forceinline__ void apply_interior_mm_cs(const Geom& g, const CS& cs
         mm_slot_terms<I, J, T>(c, dc, gg, G, f, D, w);
     };
     for_slots<0, 0>(slot);
+}
+
+// The same MajorMinor interior row from node-local data only: the 21 slots at the node for
+// C : grad grad u, and the prepared contraction D_ikl = sum_j d_j C_ijkl (Geom::mmd) for the
+// gradient term sum_kl D_ikl u_k,l -- 39 coefficient loads per node instead of 21 x 7.
+template <typename T>
+__device__ __forceinline__ void apply_interior_mm_d(const Geom& g, const T* __restrict__ C,
+                                                    const T* __restrict__ Dm, const T* __restrict__ u, int64_t m,
+                                                    T f[3], T* D) {
+    const int64_t st[3] = {1, g.sy, g.sz};
+    const T ih[3] = {T(0.5 / g.dx[0]), T(0.5 / g.dx[1]), T(0.5 / g.dx[2])};
+    const T i2[3] = {T(1.0 / (g.dx[0] * g.dx[0])), T(1.0 / (g.dx[1] * g.dx[1])), T(1.0 / (g.dx[2] * g.dx[2]))};
+    const T w[3] = {T(-2) * i2[0], T(-2) * i2[1], T(-2) * i2[2]};
+    T gg[3][3][3], G[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const T* up = u + p * g.tot;
+        const T u0 = up[m];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T up1 = up[m + st[a]], um1 = up[m - st[a]];
+            gg[p][a][a] = (up1 - T(2) * u0 + um1) * i2[a];
+            G[p][a] = (up1 - um1) * ih[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = a + 1; b < 3; b++) {
+                const T v = (up[m + st[a] + st[b]] + up[m - st[a] - st[b]] - up[m + st[a] - st[b]] -
+                             up[m - st[a] + st[b]]) * (ih[a] * ih[b]);
+                gg[p][a][b] = v;
+                gg[p][b][a] = v;
+            }
+    }
+    f[0] = f[1] = f[2] = T(0);
+    if (D) D[0] = D[1] = D[2] = T(0);
+    const T dz[3] = {T(0), T(0), T(0)};
+    auto slot = [&](auto ti, auto tj) {
+        constexpr int I = decltype(ti)::value, J = decltype(tj)::value;
+        constexpr int q = mm_slot(mm_pa(I), mm_pb(I), mm_pa(J), mm_pb(J));
+        mm_slot_terms<I, J, T, false>(C[(int64_t)q * g.tot + m], dz, gg, G, f, D, w);
+    };
+    for_slots<0, 0>(slot);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        T s = T(0);
+#pragma unroll
+        for (int P = 0; P < 6; P++) {
+            const int k = mm_pa(P), l = mm_pb(P);
+            const T gs = k == l ? G[k][k] : G[k][l] + G[l][k];
+            s += Dm[(int64_t)(6 * i + P) * g.tot + m] * gs;
+        }
+        f[i] += s;
+    }
+}
+
+// D_ikl = sum_j d_j C_ijkl at an interior node (central differences of the 21 slot planes, the
+// differences the rows above take): 18 values, out[6 i + pair(k, l)]
+template <typename T>
+__device__ __forceinline__ void mm_grad_contraction(const Geom& g, const T* __restrict__ C, int64_t m, T out[18]) {
+    const int64_t st[3] = {1, g.sy, g.sz};
+    const T ih[3] = {T(0.5 / g.dx[0]), T(0.5 / g.dx[1]), T(0.5 / g.dx[2])};
+#pragma unroll
+    for (int e = 0; e < 18; e++) out[e] = T(0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int P = 0; P < 6; P++) {
+                const int q = mm_slot(i, j, mm_pa(P), mm_pb(P));
+                out[6 * i + P] += (C[(int64_t)q * g.tot + m + st[j]] - C[(int64_t)q * g.tot + m - st[j]]) * ih[j];
+            }
+    }
 }
 
 // psi terms of an interior row (Elastic.cpp:243-260): f = psi_avg f + (C : grad u) . grad psi (1 - s),

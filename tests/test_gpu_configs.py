@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from alamo_amd import problems as P
-from alamo_amd.operator import Elastic, Linear, Krylov, PREC_MIXED, CYCLE_V
+from alamo_amd.operator import Elastic, Linear, Krylov, LAYOUT_AOS, PREC_MIXED, CYCLE_V
 from helpers import eshelby_rhs, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -53,10 +53,15 @@ def _polycrystal(ora, n, lateral=P.NEUMANN):
     return pr, N, b
 
 
-def _poly_op(pr, precision, **kw):
+def _poly_op(pr, precision, records=False, **kw):
+    """the polycrystal handle: a per-grain model, or (records) the same stiffness as full 21-slot
+    records per node, the form an integrator hands over"""
     op = Elastic(pr["ncell"], pr["dx"], sym=P.MAJORMINOR, bc=(pr["bct"], pr["bcv"]), precision=precision,
                  cycle=CYCLE_V, pre_smooth=1, post_smooth=1, max_iter=400, **kw)
-    op.SetModelGrains(pr["gid"], pr["table"])
+    if records:
+        op.SetModel(pr["table"][pr["gid"]].ravel(), LAYOUT_AOS)
+    else:
+        op.SetModelGrains(pr["gid"], pr["table"])
     return op
 
 
@@ -76,20 +81,21 @@ def test_config_polycrystal_64_vs_oracle(cuda, ora):
     assert st["converged"] == 1 and err <= 1e-8
 
 
-@pytest.mark.parametrize("n", [128, 256])
-def test_config_polycrystal_true_residual(cuda, ora, n):
+@pytest.mark.parametrize("n,records", [(128, False), (256, False), (128, True)])
+def test_config_polycrystal_true_residual(cuda, ora, n, records):
     pr, N, b = _polycrystal(ora, n, lateral=P.TRACTION)
     C = pr["table"][pr["gid"]].ravel()
     rhs = ora.newton_rhs(pr["ncell"], pr["dx"], P.MAJORMINOR, C, np.zeros(9 * N), pr["bct"], np.zeros(3 * N), b)
     x = {}
     for prec in (0, PREC_MIXED):
-        op = _poly_op(pr, prec, sweeps_per_smooth=3, sweeps_finest=2, bottom_solver="dense")
+        op = _poly_op(pr, prec, records, sweeps_per_smooth=3, sweeps_finest=2, bottom_solver="dense")
         full = op.NMGLevels()
         op.prepareForSolve()
         # the 2-cell level is singular with traction-free sides: the hierarchy ends at 4 cells
         assert op.NMGLevels() == full - 1 and op.LevelNCell(op.NMGLevels() - 1)[0] == 4
         sol = to_dev(np.zeros(3 * N), cuda)
-        print("traction-free polycrystal %d^3 FGMRES precision" % n, prec, flush=True)
+        print("traction-free polycrystal %d^3 FGMRES precision" % n, prec, "records" if records else "grains",
+              flush=True)
         st = Krylov(op, restart=16, max_iter=400).solve(sol, to_dev(rhs, cuda), 1e-10, 0.0)
         print("iterations", st["iterations"], "seconds", st["seconds"])
         assert st["converged"] == 1

@@ -156,6 +156,32 @@ def test_coefficient_restriction(cuda, ora, sym, ncell):
         assert np.abs(soa - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("ncell", [(12, 10, 8), (16, 16, 16)])
+def test_mm_prepared_rows(cuda, ora, ncell):
+    """MajorMinor rows of a prepared handle read the node-local gradient contraction
+    D_ikl = sum_j d_j C_ijkl instead of differencing the 21 slot planes: Fapply and the residual on
+    every level against the oracle's rows on that level's restricted coefficients (1e-12 of the
+    scale); every operator entry point prepares the handle first."""
+    op, C, (t, v), nodes, dx = build(P.MAJORMINOR, ncell, 61)
+    u0 = rng(62).uniform(-1, 1, 3 * nodes)
+    op.prepareForSolve()
+    for lev in range(op.NMGLevels()):
+        nl = op.LevelNodes(lev)
+        ncl = tuple(n >> lev for n in ncell)
+        dxl = tuple(d * (1 << lev) for d in dx)
+        Cl = C if lev == 0 else ora.coeff_level(ncell, dx, P.MAJORMINOR, C, lev)
+        u = u0 if lev == 0 else rng(63 + lev).uniform(-1, 1, 3 * nl)
+        b = rng(70 + lev).uniform(-1, 1, 3 * nl)
+        print("prepared MajorMinor rows, level", lev, ncl, flush=True)
+        f = to_dev(np.zeros(3 * nl), cuda)
+        op.Fapply(0, lev, f, to_dev(u, cuda))
+        ref = ora.fapply(ncl, dxl, P.MAJORMINOR, Cl, t, u)
+        close(to_host(f), ref)
+        r = to_dev(np.zeros(3 * nl), cuda)
+        op.correctionResidual(0, lev, r, to_dev(u, cuda), to_dev(b, cuda))
+        close(to_host(r), b - ref)
+
+
 def test_bc_init_and_newton_rhs(cuda, ora):
     ncell = (10, 12, 8)
     pr_bc = P.bc_tension_test("uniaxial_stress", disp=0.01)
@@ -310,9 +336,10 @@ def test_fixed_iter_and_update_opts(cuda, ora):
 
 def test_grain_table_rows_match_planes(cuda, ora):
     """A SetModelGrains model reads its interior rows from the grain table (ids + 21-slot records
-    staged in LDS) instead of the 21 coefficient planes; the arithmetic is the same, so Fapply,
-    the residual and Fsmooth agree bit for bit with the same model given as full records, and a
-    later full SetModel replaces the grain path."""
+    staged in LDS) instead of the 21 coefficient planes; the same model given as full records runs
+    rows that read the prepared node-local gradient contraction (another summation order), so
+    Fapply, the residual and Fsmooth agree to FP64 rounding and mixed solves to their tolerance.
+    A later full SetModel replaces the grain path."""
     pr = P.polycrystal_problem(n=40, ngrains=24, seed=7)
     N = 41 ** 3
     C = pr["table"][pr["gid"]].ravel()
@@ -336,7 +363,7 @@ def test_grain_table_rows_match_planes(cuda, ora):
         op.Fsmooth(0, 0, x, to_dev(b, cuda))
         res.append([to_host(out), to_host(r), to_host(x)])
     for a, c in zip(res[0], res[1]):
-        assert np.array_equal(a, c)
+        close(c, a)
     close(res[0][0], ora.fapply(pr["ncell"], pr["dx"], P.MAJORMINOR, C, pr["bct"], u))
     # mixed solves: the FP32 table and the FP32 planes are the same conversions
     sols = []
@@ -351,7 +378,9 @@ def test_grain_table_rows_match_planes(cuda, ora):
         sol = to_dev(np.zeros(3 * N), cuda)
         st = Krylov(op, restart=10, max_iter=200).solve(sol, rhs, 1e-10, 0.0)
         sols.append((st["iterations"], to_host(sol)))
-    assert sols[0][0] == sols[1][0] and np.array_equal(sols[0][1], sols[1][1])
+    print("mixed FGMRES iterations: grains", sols[0][0], "records", sols[1][0])
+    assert abs(sols[0][0] - sols[1][0]) <= 2
+    assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-8 * np.linalg.norm(sols[1][1])
     # a full model set afterwards replaces the per-grain path
     op = ops[0]
     C2 = C * 1.5
