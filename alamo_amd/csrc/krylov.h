@@ -110,14 +110,16 @@ __global__ void __launch_bounds__(RED_BLOCK) k_multidot16(const T* __restrict__ 
     }
 }
 
-// Launch k_multidot over nv <= KRY_MAXV vectors in register-sized groups of at most 8 (the
-// 16-byte form when the vectors are aligned for it).
+// Launch k_multidot over nv <= KRY_MAXV vectors in register-sized groups (the 16-byte form,
+// in groups of up to 16, when the vectors are aligned for it; else groups of up to 8).
 template <typename T>
 inline void launch_multidot(const T* w, const T* V, int64_t n, int64_t ldv, int nv, int nb, double* partial,
                             hipStream_t s) {
     const bool v16 = aligned16(w, V, ldv);
-    for (int i0 = 0; i0 < nv; i0 += 8) {
-        const int k = nv - i0 < 8 ? nv - i0 : 8;
+    // groups of up to 16 vectors in the 16-byte form (each pass re-reads w), 8 in the scalar form
+    const int gmax = v16 ? 16 : 8;
+    for (int i0 = 0; i0 < nv; i0 += gmax) {
+        const int k = nv - i0 < gmax ? nv - i0 : gmax;
         const T* Vi = V + i0 * ldv;
         double* P = partial + (int64_t)i0 * nb;
 #define ALAMO_MD(K)                                                                        \
@@ -132,9 +134,28 @@ inline void launch_multidot(const T* w, const T* V, int64_t n, int64_t ldv, int 
             case 5: ALAMO_MD(5)
             case 6: ALAMO_MD(6)
             case 7: ALAMO_MD(7)
-            default: ALAMO_MD(8)
+            case 8: ALAMO_MD(8)
+            default:
+                break;
         }
 #undef ALAMO_MD
+        // the 16-byte form only: 9 .. 16 vectors in one pass
+#define ALAMO_MD16(K)                                                                      \
+    k_multidot16<K, T><<<nb, RED_BLOCK, 0, s>>>(w, Vi, n, ldv, P);                         \
+    break;
+        switch (k) {
+            case 9: ALAMO_MD16(9)
+            case 10: ALAMO_MD16(10)
+            case 11: ALAMO_MD16(11)
+            case 12: ALAMO_MD16(12)
+            case 13: ALAMO_MD16(13)
+            case 14: ALAMO_MD16(14)
+            case 15: ALAMO_MD16(15)
+            case 16: ALAMO_MD16(16)
+            default:
+                break;
+        }
+#undef ALAMO_MD16
     }
 }
 
