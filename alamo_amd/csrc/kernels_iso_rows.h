@@ -41,12 +41,14 @@ constexpr int ROWS_KCH = ALAMO_ROWS_KCH;  // default z planes per workgroup
 // rows per workgroup and planes per chunk of the aligned-tile kernel below the large-level
 // class, per precision.  Measured at 129^3 nodes in one process (4 x 16 in the hardware order in
 // brackets), both in the XCD-contiguous order: FP32 4 x 8 Jacobi 21.6 us (29.1), residual
-// 19.7 (28.5); FP64 4 x 16 Jacobi 39.7 (47.1), residual 36.4 (46.0)
+// 19.7 (28.5); FP64 4 x 16 Jacobi 39.7 (47.1), residual 36.4 (46.0).  Inside the 257^3 F-cycle
+// FGMRES solve (whose 129^3 level this is) FP32 4 x 4 took 0.4026 s, 4 x 8 0.4048, 2 x 4 0.4032,
+// 4 x 2 0.4086.
 #ifndef ALAMO_ROWS64_RW_F32
 #define ALAMO_ROWS64_RW_F32 4
 #endif
 #ifndef ALAMO_ROWS64_KCH_F32
-#define ALAMO_ROWS64_KCH_F32 8
+#define ALAMO_ROWS64_KCH_F32 4
 #endif
 template <typename T> struct Rows64Cfg { static constexpr int RW = ROWS_RW, KCH = ROWS_KCH; };
 template <> struct Rows64Cfg<float> { static constexpr int RW = ALAMO_ROWS64_RW_F32, KCH = ALAMO_ROWS64_KCH_F32; };
@@ -65,11 +67,14 @@ template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8,
 // At 257^3 inside the F-cycle FGMRES solve: 8 x 64 0.460 s, 4 x 32 0.447 s, 4 x 16 0.448 s, 2 x 32
 // 0.449 s, 4 x 128 0.454 s, 4 x 64 0.462 s, 2 x 64 0.461 s.
 template <> struct Rows64Big<float, 2> { static constexpr int RW = 4, KCH = 32; };
-// levels of < ROWS64_MID cells in y or z (65^3 nodes and the like): 2 x 2 tiles keep about
-// 1000 workgroups in flight.  At 65^3 in FP32: Jacobi 6.3 us, residual 6.0 us (4 x 8: 13.1 and
-// 12.1; the node-parallel rows took 18 and 15)
+// levels of < ROWS64_MID cells in y or z (65^3 nodes and below): 2-row tiles of one output plane.
+// These levels are latency-bound, and a workgroup's planes arrive one after another (the plane
+// ring loads plane p + 1 while computing p - 1), so one output plane (three loads) beats two
+// (four): the 257^3 F-cycle FGMRES solve, which visits them about 110 times per iteration, 0.4155
+// -> 0.4068 s.  At 65^3 in FP32 the 2 x 2 tiles took Jacobi 6.3 us, residual 6.0 us (4 x 8: 13.1
+// and 12.1; the node-parallel rows took 18 and 15)
 constexpr int ROWS64_MID = 96;
-struct Rows64Small { static constexpr int RW = 2, KCH = 2; };
+struct Rows64Small { static constexpr int RW = 2, KCH = 1; };
 
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l receives lane l + 1 (0 past the last lane)
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l receives lane l - 1 (0 before lane 0)
