@@ -112,8 +112,9 @@ __global__ void __launch_bounds__(256) k_l1_diag(Geom g, BCTable bc, const T* __
     for (int p = 0; p < 3; p++) out[p * g.tot + m] = D[p];
 }
 
-// Explicit rows of the level's operator (no psi terms): tab[(3 m + q) 81 + 3 o + c] is the weight of
-// u_c at offset slot o (13 + dx + 3 dy + 9 dz) in row component q of node m.
+// Explicit rows of the level's operator (no psi terms), 243 planes: plane (27 q + o) 3 + c holds the
+// weight of u_c at offset slot o (13 + dx + 3 dy + 9 dz) in row component q, per node (neighbouring
+// work-items of the gather below read neighbouring elements).
 template <int SYM>
 __global__ void __launch_bounds__(256) k_row_table(Geom g, BCTable bc, const double* __restrict__ C,
                                                    double* __restrict__ tab) {
@@ -123,10 +124,9 @@ __global__ void __launch_bounds__(256) k_row_table(Geom g, BCTable bc, const dou
     node_ijk(g, m, i, j, k);
     if (i >= g.N[0]) return;
     row_coeffs<double, SYM>(g, bc, C, i, j, k, [&](int q, const double(&acc)[27][3]) {
-        double* t = tab + (3 * m + q) * 81;
         for (int o = 0; o < 27; o++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) t[3 * o + c] = acc[o][c];
+            for (int c = 0; c < 3; c++) tab[((int64_t)(27 * q + o) * 3 + c) * g.tot + m] = acc[o][c];
     });
 }
 
@@ -148,9 +148,8 @@ __global__ void __launch_bounds__(256) k_row_apply_t(Geom g, const double* __res
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const double rq = r[q * g.tot + mr];
-            const double* t = tab + (3 * mr + q) * 81 + 3 * o;
 #pragma unroll
-            for (int d = 0; d < 3; d++) s[d] += t[d] * rq;
+            for (int d = 0; d < 3; d++) s[d] += tab[((int64_t)(27 * q + o) * 3 + d) * g.tot + mr] * rq;
         }
     }
 #pragma unroll
