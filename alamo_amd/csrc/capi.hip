@@ -524,44 +524,56 @@ static void one_cycle(alamo_mg* h, int iter = 0) {
 // ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
-// Gauss-Jordan elimination with partial pivoting of the column-major n x n matrix A into the
-// row-major inverse; false when a pivot falls below 1e-11 max|A| (a singular coarsest operator,
-// e.g. one that no displacement condition pins)
+// In-place Gauss-Jordan inverse with partial pivoting of the column-major n x n matrix A into the
+// row-major inverse: the row interchanges are recorded and undone as column interchanges at the
+// end (n^3 multiply-adds, half of eliminating into a separate identity), and each row's update
+// runs over the pivot row's nonzeros when that row is at most half full (the identity rows of a
+// Dirichlet boundary then cost one entry per row).  false when a pivot falls below 1e-11 max|A|
+// (a singular coarsest operator, e.g. one that no displacement condition pins).
 static bool dense_invert(const std::vector<double>& A, int64_t n, std::vector<double>& inv) {
-    std::vector<double> M(n * n);
+    std::vector<double>& M = inv;
+    M.assign(n * n, 0.0);
     double amax = 0.0;
     for (int64_t i = 0; i < n; i++)
         for (int64_t j = 0; j < n; j++) {
             M[i * n + j] = A[j * n + i];
             amax = std::max(amax, std::fabs(M[i * n + j]));
         }
-    inv.assign(n * n, 0.0);
-    for (int64_t i = 0; i < n; i++) inv[i * n + i] = 1.0;
+    std::vector<int64_t> piv(n), nz;
+    nz.reserve(n);
     for (int64_t c = 0; c < n; c++) {
         int64_t p = c;
         for (int64_t r = c + 1; r < n; r++)
             if (std::fabs(M[r * n + c]) > std::fabs(M[p * n + c])) p = r;
         if (!(std::fabs(M[p * n + c]) > 1e-11 * amax)) return false;
+        piv[c] = p;
         if (p != c)
-            for (int64_t j = 0; j < n; j++) {
-                std::swap(M[p * n + j], M[c * n + j]);
-                std::swap(inv[p * n + j], inv[c * n + j]);
-            }
-        const double d = 1.0 / M[c * n + c];
+            for (int64_t j = 0; j < n; j++) std::swap(M[p * n + j], M[c * n + j]);
+        double* rc = &M[c * n];
+        const double d = 1.0 / rc[c];
+        rc[c] = 1.0;
+        nz.clear();
         for (int64_t j = 0; j < n; j++) {
-            M[c * n + j] *= d;
-            inv[c * n + j] *= d;
+            rc[j] *= d;
+            if (rc[j] != 0.0) nz.push_back(j);
         }
+        const bool dense = 2 * (int64_t)nz.size() > n;
         for (int64_t r = 0; r < n; r++) {
             if (r == c) continue;
-            const double f = M[r * n + c];
+            double* rr = &M[r * n];
+            const double f = rr[c];
             if (f == 0.0) continue;
-            for (int64_t j = 0; j < n; j++) {
-                M[r * n + j] -= f * M[c * n + j];
-                inv[r * n + j] -= f * inv[c * n + j];
+            rr[c] = 0.0;
+            if (dense) {
+                for (int64_t j = 0; j < n; j++) rr[j] -= f * rc[j];
+            } else {
+                for (int64_t j : nz) rr[j] -= f * rc[j];
             }
         }
     }
+    for (int64_t c = n - 1; c >= 0; c--)
+        if (piv[c] != c)
+            for (int64_t r = 0; r < n; r++) std::swap(M[r * n + c], M[r * n + piv[c]]);
     return true;
 }
 
