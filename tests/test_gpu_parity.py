@@ -234,6 +234,29 @@ def test_krylov_solve_eshelby(cuda, ora, precision, cycle, precond):
     assert st["converged"] == 1 and err <= 1e-8
 
 
+@pytest.mark.parametrize("cycle", [0, 1])
+def test_one_fgmres_step_matches_oracle(cuda, ora, cycle):
+    """One FGMRES(1) step, x = y M(b), with the benchmark's cycle (l1 smoother, 1 + 1 calls of 3 sweeps,
+    2 on the finest level, coarsest level 4 cells, all-Dirichlet box, FP64), against the oracle's
+    FGMRES limited to one iteration: it pins the cycle itself, where converged solutions would hide a
+    wrong weight behind extra iterations (the oracle's BiCGStab bottom is solved to 1e-14, where the
+    GPU inverts the bottom operator; measured 1e-14)."""
+    n = 32
+    pr, bc, rhs = eshelby_rhs(ora, n)
+    op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=bc, cycle=cycle, pre_smooth=1, post_smooth=1,
+                 sweeps_per_smooth=3, sweeps_finest=2, smoother="l1", bottom_solver="dense", max_coarsening_level=3)
+    op.SetModel(pr["C"])
+    sol = to_dev(np.zeros_like(rhs), cuda)
+    st = Krylov(op, restart=1, max_iter=1).solve(sol, to_dev(rhs, cuda), 1e-14, 0.0, raise_on_fail=False)
+    assert st["iterations"] == 1
+    x, it, fr = ora.krylov_solve(pr["ncell"], pr["dx"], P.ISO, pr["C"], bc[0], rhs, tol_rel=1e-14, restart=1,
+                                 max_iter=1, nu1=1, nu2=1, sweeps=3, sweeps_finest=2, smoother=1, bottom_tol=1e-14,
+                                 cycle=cycle, max_level=3)
+    err = np.linalg.norm(to_host(sol) - x) / np.linalg.norm(x)
+    print("one FGMRES step, cycle", cycle, "rel L2 vs oracle", err)
+    assert abs(it) == 1 and err <= 1e-9  # the oracle returns -iterations when not converged
+
+
 @pytest.mark.parametrize("opts", [dict(), dict(smoother="l1", sweeps_per_smooth=3, sweeps_finest=2,
                                                pre_smooth=1, post_smooth=1, bottom_solver="dense"),
                                   dict(pre_smooth=2, post_smooth=1, sweeps_per_smooth=1)])
