@@ -408,10 +408,12 @@ __global__ void __launch_bounds__(256) k_interp_add(Geom gc, Geom gf, const T* _
     }
 }
 
-// The same prolongation by fine x-rows: a wavefront takes 64 coarse nodes I of one fine row (j, k)
-// and writes its fine nodes 2I and 2I + 1, so every coarse value is one coalesced load per needed
-// coarse row (one to four by the parities of j and k; I + 1 by a second, shifted load) instead of up
-// to eight loads per fine node.  Same sums in the same order as k_interp_add.  Needs gf.n = 2 gc.n.
+// The same prolongation by fine x-rows: a wavefront takes 64 coarse nodes I < N0 - 1 of one fine row
+// (j, k) and writes its fine nodes 2I and 2I + 1 (the lane of I = N0 - 2 also the row's last node),
+// so every coarse value is one coalesced load per needed coarse row (one to four by the parities of
+// j and k; I + 1 by a second, shifted load) instead of up to eight loads per fine node, and a row of
+// 2^k + 1 nodes takes 2^(k-7) wavefronts.  Same sums in the same order as k_interp_add.  Needs
+// gf.n = 2 gc.n.
 constexpr int INTERP_ROW_WAVES = 4;
 template <typename T, bool SET = false>
 __global__ void __launch_bounds__(64 * INTERP_ROW_WAVES) k_interp_rows(Geom gc, Geom gf, int nchunk,
@@ -422,40 +424,45 @@ __global__ void __launch_bounds__(64 * INTERP_ROW_WAVES) k_interp_rows(Geom gc, 
     if (row >= (int64_t)gf.N[1] * gf.N[2]) return;
     const int j = (int)(row % gf.N[1]), k = (int)(row / gf.N[1]);
     const int I = (int)(w - row * nchunk) * 64 + (threadIdx.x & 63);
-    if (I >= gc.N[0]) return;
-    const bool odd_live = I + 1 < gc.N[0];  // fine node 2I + 1 exists
+    if (I >= gc.N[0] - 1) return;
+    const bool last = I + 2 == gc.N[0];  // also the row's last fine node 2I + 2, the even node of I + 1
     const int J = j >> 1, K = k >> 1;
     const bool ej = (j & 1) == 0, ek = (k & 1) == 0;
     const int64_t mc = I + gc.sy * J + gc.sz * (int64_t)K;
     const int64_t mf = 2 * I + gf.sy * j + gf.sz * (int64_t)k;
     const int64_t sy = gc.sy, sz = gc.sz;
-    const int o = odd_live ? 1 : 0;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         const T* Cp = crse + p * gc.tot + mc;
-        auto at = [&](int a, int b, int c) { return Cp[a * o + b * sy + c * sz]; };
-        T ve, vo;
+        auto at = [&](int a, int b, int c) { return Cp[a + b * sy + c * sz]; };
+        T ve, vo, vl;
         if (ej && ek) {
             ve = at(0, 0, 0);
             vo = T(0.5) * (at(0, 0, 0) + at(1, 0, 0));
+            vl = at(1, 0, 0);
         } else if (ek) {
             ve = T(0.5) * (at(0, 0, 0) + at(0, 1, 0));
             vo = T(0.25) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(1, 1, 0));
+            vl = T(0.5) * (at(1, 0, 0) + at(1, 1, 0));
         } else if (ej) {
             ve = T(0.5) * (at(0, 0, 0) + at(0, 0, 1));
             vo = T(0.25) * (at(0, 0, 0) + at(0, 0, 1) + at(1, 0, 0) + at(1, 0, 1));
+            vl = T(0.5) * (at(1, 0, 0) + at(1, 0, 1));
         } else {
             ve = T(0.25) * (at(0, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1));
             vo = T(0.125) * (at(0, 0, 0) + at(1, 0, 0) + at(0, 1, 0) + at(0, 0, 1) + at(0, 1, 1) + at(1, 0, 1) +
                              at(1, 1, 0) + at(1, 1, 1));
+            vl = T(0.25) * (at(1, 0, 0) + at(1, 1, 0) + at(1, 0, 1) + at(1, 1, 1));
         }
         T* Fp = fine + p * gf.tot + mf;
         if (SET) {
             Fp[0] = ve;
-            if (odd_live) Fp[1] = vo;
+            Fp[1] = vo;
+            if (last) Fp[2] = vl;
         } else {
             Fp[0] += ve;
-            if (odd_live) Fp[1] += vo;
+            Fp[1] += vo;
+            if (last) Fp[2] += vl;
         }
     }
 }
@@ -471,7 +478,7 @@ inline unsigned interp_blocks(const Geom& gc, const Geom& gf) {
 template <typename T, bool SET = false>
 inline void launch_interp(const Geom& gc, const Geom& gf, const T* crse, T* fine, hipStream_t s) {
     if (gf.n[0] == 2 * gc.n[0] && gf.n[1] == 2 * gc.n[1] && gf.n[2] == 2 * gc.n[2]) {
-        const int nchunk = (gc.N[0] + 63) / 64;
+        const int nchunk = (gc.N[0] - 1 + 63) / 64;
         const int64_t waves = (int64_t)gf.N[1] * gf.N[2] * nchunk;
         k_interp_rows<T, SET><<<(unsigned)((waves + INTERP_ROW_WAVES - 1) / INTERP_ROW_WAVES), 64 * INTERP_ROW_WAVES,
                                 0, s>>>(gc, gf, nchunk, crse, fine);
