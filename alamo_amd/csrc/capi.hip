@@ -578,10 +578,8 @@ static bool dense_invert(const std::vector<double>& A, int64_t n, std::vector<do
 }
 
 // solver.bottom_solver "dense": assemble the coarsest FP64 operator (with its psi and boundary
-// rows) column by column on the device, invert it on the host and keep the inverse on the device.
-// *singular: the operator has no inverse (nothing on the device is changed then).
-static int dense_bottom_factor(alamo_mg* h, bool* singular) {
-    *singular = false;
+// rows) column by column on the device into A (host, column-major; synchronous), ...
+static int dense_bottom_assemble(alamo_mg* h, std::vector<double>& A) {
     LevelBuf& lb = h->L.back();
     const int64_t n = 3 * lb.g.tot;
     if (n > ALAMO_DENSE_BOTTOM_MAX)
@@ -592,7 +590,7 @@ static int dense_bottom_factor(alamo_mg* h, bool* singular) {
         TRY(dalloc(h, &h->dense_inv, n * n));
         h->dense_n = n;
     }
-    std::vector<double> A(n * n, 0.0), inv;
+    A.assign(n * n, 0.0);
     if (h->gal && h->L.size() > 1) {
         // the coarsest Galerkin stencil written out column-major
         const Geom& g = lb.g;
@@ -624,6 +622,15 @@ static int dense_bottom_factor(alamo_mg* h, bool* singular) {
         HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, n * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
+    return ALAMO_OK;
+}
+
+// ... then invert it on the host and keep the inverse on the device.  *singular: the operator has
+// no inverse (nothing on the device is changed then).
+static int dense_bottom_finish(alamo_mg* h, const std::vector<double>& A, bool* singular) {
+    *singular = false;
+    const int64_t n = h->dense_n;
+    std::vector<double> inv;
     if (!dense_invert(A, n, inv)) {
         *singular = true;
         return ALAMO_OK;
@@ -631,6 +638,12 @@ static int dense_bottom_factor(alamo_mg* h, bool* singular) {
     HIPCHK(hipMemcpyAsync(h->dense_inv, inv.data(), n * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return ALAMO_OK;
+}
+
+static int dense_bottom_factor(alamo_mg* h, bool* singular) {
+    std::vector<double> A;
+    TRY(dense_bottom_assemble(h, A));
+    return dense_bottom_finish(h, A, singular);
 }
 
 static int prepare(alamo_mg* h) {
@@ -684,6 +697,11 @@ static int prepare(alamo_mg* h) {
         }
         h->gal = true;
     }
+    // the dense bottom's operator is assembled before the diagonal passes are queued, so that its
+    // inversion on the host (the end of this function) overlaps them
+    std::vector<double> Abot;
+    const bool dense = h->d.bottom_solver == ALAMO_BOTTOM_DENSE;
+    if (dense) TRY(dense_bottom_assemble(h, Abot));
     for (int l = 0; l < nl; l++) {
         LevelBuf& lb = h->L[l];
         const bool st = h->gal && l > 0;
@@ -731,9 +749,9 @@ static int prepare(alamo_mg* h) {
         }
     }
     HIPCHK(hipGetLastError());
-    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) {
+    if (dense) {
         bool singular = false;
-        TRY(dense_bottom_factor(h, &singular));
+        TRY(dense_bottom_finish(h, Abot, &singular));
         // A coarsest level whose own boundary rows leave it singular (2 cells per direction with
         // traction-free sides: the one-sided traction rows of the two faces meet) ends the hierarchy
         // one level earlier, as long as that level still fits the dense bottom.
