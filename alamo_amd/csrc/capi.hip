@@ -2347,7 +2347,11 @@ int alamo_net_train(alamo_net* n, alamo_mg* h, const alamo_train_opts* o, alamo_
             if (rc) break;
             rc = multidot(h, qv, qv, n3, 1, &qq);
             if (rc) break;
-            if (!std::isfinite(d[0]) || !std::isfinite(d[1])) { rc = ALAMO_ERR_NAN; break; }
+            if (!std::isfinite(d[0]) || !std::isfinite(d[1])) {
+                g_err = "alamo_net_train: the network output is not finite";
+                rc = ALAMO_ERR_NAN;
+                break;
+            }
             if (d[1] > 0.0 && qq > 0.0) {
                 const double c = d[0] / d[1];
                 lv_ = 1.0 - c * d[0] / qq;
