@@ -406,12 +406,15 @@ static void bottom(alamo_mg* h) {
     const int bot = (int)h->L.size() - 1;
     LevelBuf& lb = h->L[bot];
     const int64_t n = 3 * lb.g.tot;
-    // the bottom solve always runs in FP64
+    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) {
+        // the FP64 inverse applied to the cycle-precision vectors in place (FP64 products and sum)
+        k_dense_matvec<T, T><<<nblocks(n, 4), 256, 0, h->stream>>>(h->dense_inv, LB<T>::res(lb), LB<T>::cor(lb), n);
+        return;
+    }
+    // the other bottom solves run in FP64
     if constexpr (std::is_same<T, float>::value)
         k_convert<float, double><<<gs_blocks(n), 256, 0, h->stream>>>(lb.resf, lb.res, n);
-    if (h->d.bottom_solver == ALAMO_BOTTOM_DENSE) {
-        k_dense_matvec<<<nblocks(n, 4), 256, 0, h->stream>>>(h->dense_inv, lb.res, lb.cor, n);
-    } else if (h->d.bottom_solver == ALAMO_BOTTOM_SMOOTHER) {
+    if (h->d.bottom_solver == ALAMO_BOTTOM_SMOOTHER) {
         vfill<double>(lb.cor, n, 0.0, h->stream);
         smooth<double>(h, bot, 8);
     } else {

@@ -780,15 +780,18 @@ __global__ void __launch_bounds__(256) k_bottom_columns(Geom g, BCTable bc, cons
     bottom_apply<double, SYM>(g, bc, C, E + (int64_t)blockIdx.x * n, A + (int64_t)blockIdx.x * n);
 }
 
-__global__ void __launch_bounds__(256) k_dense_matvec(const double* __restrict__ Ainv, const double* __restrict__ b,
-                                                      double* __restrict__ x, int64_t n) {
+// (b and x may be FP32 vectors: the products and the sum run in FP64, x is rounded once, as
+// converting b, multiplying in FP64 and converting x would give)
+template <typename Tb, typename Tx>
+__global__ void __launch_bounds__(256) k_dense_matvec(const double* __restrict__ Ainv, const Tb* __restrict__ b,
+                                                      Tx* __restrict__ x, int64_t n) {
     const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (row >= n) return;  // whole wavefronts leave together
     double s = 0.0;
-    for (int64_t j = lane; j < n; j += 64) s += Ainv[row * n + j] * b[j];
+    for (int64_t j = lane; j < n; j += 64) s += Ainv[row * n + j] * (double)b[j];
     s = wave_sum(s);
-    if (lane == 0) x[row] = s;
+    if (lane == 0) x[row] = (Tx)s;
 }
 
 }  // namespace alamo
