@@ -208,6 +208,9 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
     const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
     const bool l1 = MODE == JACOBI && smoother_diag<T>(g) != nullptr;  // l1-Jacobi smoother level
+    // FP32 sweeps read the prepared l1 diagonal (12 more bytes per node, against the VALU work of
+    // forming it here: the 257^3 solve 0.396 -> 0.389 s); FP64 sweeps form it from the row below
+    constexpr bool L1_READ = sizeof(T) == 4;
     const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
     const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
     const int rs = w * R64_W + 1 + lane, rc = (w + 1) * R64_W + 1 + lane, rn = (w + 2) * R64_W + 1 + lane;
@@ -250,11 +253,15 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
         if (p < k1) fetch(p + 1, qx);
         const uint32_t pc = szb * (uint32_t)(p - 1);
         const uint32_t mcb = colb + pc;
-        T bq[3] = {T(0), T(0), T(0)}, bh[3] = {T(0), T(0), T(0)};
+        T bq[3] = {T(0), T(0), T(0)}, bh[3] = {T(0), T(0), T(0)}, dq[3] = {T(1), T(1), T(1)};
         if (MODE != APPLY && p >= k0 + 1) {
             if (out_i || out_f) {
 #pragma unroll
                 for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
+            }
+            if (L1_READ && l1 && out_i) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) dq[v] = ldb(smoother_diag<T>(g) + (int64_t)v * g.tot, mcb);
             }
             if (out_fh) {
 #pragma unroll
@@ -313,7 +320,7 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
             // the parts shared by the three rows are formed once per node.
             const T ih[3] = {idx, idy, idz}, hh[3] = {hx, hy, hz}, w2[3] = {ix2, iy2, iz2};
             T Mf[3], am[3], SM = T(0), SA = T(0), SIH = T(0);
-            if (l1) {
+            if (l1 && !L1_READ) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     am[a] = fabs(dm[a]);
@@ -332,7 +339,9 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
                 else if (MODE == RESID) stb(O[v], mcb, bq[v] - f);
                 else {
                     const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
-                    if (l1) {
+                    if (l1 && L1_READ) {
+                        stb(O[v], mcb, qc.u[v] + (bq[v] - f) / dq[v]);
+                    } else if (l1) {
                         const T io = SIH - ih[v];  // sum of 1/h_a over a != v
                         const T S = T(2) * fmax((lam + T(2) * mu) * w2[v], fabs(dl[v] + T(2) * dm[v]) * hh[v]) +
                                     (SM - Mf[v]) + fabs(dl[v]) * io + ih[v] * (SA - am[v]) +
