@@ -62,10 +62,12 @@ constexpr int ROWS64_BIG = 192;
 template <typename T, int MODE> struct Rows64Big { static constexpr int RW = 4, KCH = 128; };
 template <> struct Rows64Big<double, 0> { static constexpr int RW = 8, KCH = 128; };
 template <int MODE> struct Rows64Big<float, MODE> { static constexpr int RW = 8, KCH = 64; };
-// The FP32 Jacobi sweep with the l1 smoother needs 141 VGPRs (3 waves per SIMD), where an 8-row
-// workgroup (8 waves) leaves a third of the wave slots of a CU empty; 4-row workgroups fill all 12.
-// At 257^3 inside the F-cycle FGMRES solve: 8 x 64 0.460 s, 4 x 32 0.447 s, 4 x 16 0.448 s, 2 x 32
-// 0.449 s, 4 x 128 0.454 s, 4 x 64 0.462 s, 2 x 64 0.461 s.
+// The FP32 Jacobi sweep with the l1 smoother: when it formed the l1 diagonal itself it needed 141
+// VGPRs (3 waves per SIMD), where an 8-row workgroup (8 waves) left a third of the wave slots of a
+// CU empty and 4-row workgroups filled all 12 (at 257^3 inside the F-cycle FGMRES solve: 8 x 64
+// 0.460 s, 4 x 32 0.447 s, 4 x 16 0.448 s, 2 x 32 0.449 s, 4 x 128 0.454 s, 4 x 64 0.462 s, 2 x 64
+// 0.461 s).  Reading the prepared diagonal it needs 128 (4 waves), and the shapes even out: 4 x 32
+// 0.3879 s, 8 x 32 0.3879, 8 x 64 0.3883, 4 x 16 0.3875, 4 x 64 0.3884, 2 x 32 0.3991.
 template <> struct Rows64Big<float, 2> { static constexpr int RW = 4, KCH = 32; };
 // levels of < ROWS64_MID cells in y or z (65^3 nodes and below): 2-row tiles of one output plane.
 // These levels are latency-bound, and a workgroup's planes arrive one after another (the plane
