@@ -218,23 +218,47 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     const int rs = w * R64_W + 1 + lane, rc = (w + 1) * R64_W + 1 + lane, rn = (w + 2) * R64_W + 1 + lane;
     const int rce = (w + 1) * R64_W + ee;
 
+    // One-plane tiles (the small levels, whose launches are latency-bound) load their three planes
+    // and the output plane's right-hand side up front, one memory latency instead of a chain of
+    // three; deeper tiles load plane p + 1 while staging plane p.
+    constexpr bool ONE = KCH == 1;
     T ph[5] = {}, pg = T(0);  // prefetched halo row and edge gather of the next plane
-    auto fetch = [&](int p, RowPlane<T>& d) {
+    T ph1[5] = {}, pg1 = T(0), ph2[5] = {}, pg2 = T(0);  // ONE: the halos of planes k0 and k0 + 1
+    auto fetch_to = [&](int p, RowPlane<T>& d, T (&h)[5], T& gq) {
         const uint32_t pz = szb * (uint32_t)p;
         const uint32_t o = colb + pz;
 #pragma unroll
         for (int v = 0; v < 3; v++) d.u[v] = ldb(U[v], o);
         d.lam = ldb(la, o);
         d.mu = ldb(ma, o);
-        if (gl) pg = ldb(gbase, gcolb + pz);
+        if (gl) gq = ldb(gbase, gcolb + pz);
         if (hrow >= 0) {
             const uint32_t oh = hcolb + pz;
 #pragma unroll
-            for (int v = 0; v < 3; v++) ph[v] = ldb(U[v], oh);
-            ph[3] = ldb(la, oh);
-            ph[4] = ldb(ma, oh);
+            for (int v = 0; v < 3; v++) h[v] = ldb(U[v], oh);
+            h[3] = ldb(la, oh);
+            h[4] = ldb(ma, oh);
         }
     };
+    auto fetch = [&](int p, RowPlane<T>& d) { fetch_to(p, d, ph, pg); };
+    // right-hand side (and prepared l1 diagonal) of output plane p - 1
+    auto load_b = [&](int p, T (&bq)[3], T (&bh)[3], T (&dq)[3]) {
+        const uint32_t pc = szb * (uint32_t)(p - 1);
+        const uint32_t mcb = colb + pc;
+        if (out_i || out_f) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
+        }
+        if (L1_READ && l1 && out_i) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) dq[v] = ldb(smoother_diag<T>(g) + (int64_t)v * g.tot, mcb);
+        }
+        if (out_fh) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) bh[v] = ldb(B[v], ecolb + pc);
+        }
+    };
+    T pbq[3] = {T(0), T(0), T(0)}, pbh[3] = {T(0), T(0), T(0)}, pdq[3] = {T(1), T(1), T(1)};  // ONE
     // x difference of a row value whose edge-lane neighbour is e (DPP in uniform flow)
     auto dx_of = [&](T v, T e) {
         const T r = from_right(v), l = from_left(v);
@@ -252,22 +276,20 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
 #pragma unroll
             for (int v = 0; v < 5; v++) sp[v][hrow * R64_W + 1 + lane] = ph[v];
         }
-        if (p < k1) fetch(p + 1, qx);
+        if (!ONE && p < k1) fetch(p + 1, qx);
         const uint32_t pc = szb * (uint32_t)(p - 1);
         const uint32_t mcb = colb + pc;
         T bq[3] = {T(0), T(0), T(0)}, bh[3] = {T(0), T(0), T(0)}, dq[3] = {T(1), T(1), T(1)};
         if (MODE != APPLY && p >= k0 + 1) {
-            if (out_i || out_f) {
+            if (ONE) {
 #pragma unroll
-                for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
-            }
-            if (L1_READ && l1 && out_i) {
-#pragma unroll
-                for (int v = 0; v < 3; v++) dq[v] = ldb(smoother_diag<T>(g) + (int64_t)v * g.tot, mcb);
-            }
-            if (out_fh) {
-#pragma unroll
-                for (int v = 0; v < 3; v++) bh[v] = ldb(B[v], ecolb + pc);
+                for (int v = 0; v < 3; v++) {
+                    bq[v] = pbq[v];
+                    bh[v] = pbh[v];
+                    dq[v] = pdq[v];
+                }
+            } else {
+                load_b(p, bq, bh, dq);
             }
         }
         __syncthreads();
@@ -375,6 +397,23 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
         }
     };
     RowPlane<T> qa, qb, qc, qd;  // plane p lives in record (p - k0 + 1) % 4
+    if constexpr (ONE) {
+        // k1 = k0 + 1 <= n2: planes k0 - 1, k0, k0 + 1, output plane k0
+        fetch_to(k0 - 1, qa, ph, pg);
+        fetch_to(k0, qb, ph1, pg1);
+        fetch_to(k0 + 1, qc, ph2, pg2);
+        if (MODE != APPLY) load_b(k0 + 1, pbq, pbh, pdq);
+        step(k0 - 1, qc, qd, qa, qb);
+#pragma unroll
+        for (int v = 0; v < 5; v++) ph[v] = ph1[v];
+        pg = pg1;
+        step(k0, qd, qa, qb, qc);
+#pragma unroll
+        for (int v = 0; v < 5; v++) ph[v] = ph2[v];
+        pg = pg2;
+        step(k0 + 1, qa, qb, qc, qd);
+        return;
+    }
     fetch(k0 - 1, qa);
     for (int p = k0 - 1; p <= k1; p += 4) {
         step(p, qc, qd, qa, qb);

@@ -249,6 +249,19 @@ def main():
     best = min(conv, key=lambda k: results[k]["seconds"]) if conv else None
     val = results[best]["seconds"] if best else None
 
+    # the metric's comparison: the learned-preconditioned solve against the plain FP64 multigrid
+    # (the fastest converged stationary cycle on the GPU), both to the same 1e-10 FP64 true residual
+    plain = [k for k in conv if not k.startswith("fgmres-") and results[k]["precision"] == "f64"]
+    learned = [k for k in conv if k.startswith("fgmres-learned-")]
+    lvp = None
+    if plain and learned:
+        pk = min(plain, key=lambda k: results[k]["seconds"])
+        lk = min(learned, key=lambda k: results[k]["seconds"])
+        lvp = {"learned": lk, "learned_s": results[lk]["seconds"], "learned_iterations": results[lk]["iterations"],
+               "plain_mg": pk, "plain_mg_s": results[pk]["seconds"], "plain_mg_iterations": results[pk]["iterations"],
+               "speedup": results[pk]["seconds"] / results[lk]["seconds"],
+               "unconverged_plain": [k for k in results if not k.startswith("fgmres-") and k not in conv]}
+
     # the value method's solution, checked by the oracle's FP64 operator on the host
     oracle_res = None
     if best is not None and not a.skip_cpu:
@@ -290,6 +303,7 @@ def main():
                        "sweeps_per_call_finest": a.krylov_sweeps_finest or a.krylov_sweeps,
                        "smoother": a.krylov_smoother, "bottom_solver": a.krylov_bottom},
             "oracle_true_rel_res": oracle_res,
+            "learned_vs_plain_mg": lvp,
             "methods": results,
             "training": train_info,
             "fapply_gbs": fapply["gbs"] if fapply else None,
