@@ -209,3 +209,45 @@ def test_config_training_129_2000_pairs(cuda):
     print("2000 pairs: loss %.4f -> %.4f in %.1f s" % (first, last, time.time() - t0), flush=True)
     assert first > 0.9
     assert last < 0.2 * first
+
+
+def test_bench_field_257_true_residual(cuda, ora):
+    """The benchmark's workload: the 256^3-cell (257^3-node) two-phase 1:1000 field (correlation 8
+    cells, seed 256, erf interface w = 0.7), solved by mixed and FP64 FGMRES(24) with the benchmark's
+    l1 F-cycle (2 sweeps on the finest level, 3 below, coarsest level 4 cells, dense bottom) to 1e-10.
+    The oracle's FP64 operator evaluates the mixed solution's true residual on the host, and the two
+    solutions agree to 1e-8."""
+    import torch
+    from alamo_amd.operator import CYCLE_F
+    n = 256
+    print("generating the 257^3 two-phase field", flush=True)
+    pr = P.two_phase_problem(n=n, corr_cells=8, contrast=1000.0, seed=256, width=0.7)
+    print("interface", pr["interface"], flush=True)
+    N = (n + 1) ** 3
+    x, st = {}, {}
+    rhs = None
+    for prec in (PREC_MIXED, 0):
+        op = Elastic(pr["ncell"], pr["dx"], sym=P.ISO, bc=(pr["bct"], pr["bcv"]), precision=prec, cycle=CYCLE_F,
+                     pre_smooth=1, post_smooth=1, sweeps_per_smooth=3, sweeps_finest=2, smoother="l1",
+                     bottom_solver="dense", max_coarsening_level=6)
+        op.SetModel(pr["C"])
+        if rhs is None:
+            b = torch.zeros(3 * N, dtype=torch.float64, device=cuda)
+            op.BCInit(b)
+            rhs = torch.zeros_like(b)
+            op.NewtonRHS(torch.zeros_like(b), b, to_dev(pr["F0"], cuda), rhs)
+            del b
+        sol = torch.zeros_like(rhs)
+        st[prec] = Krylov(op, restart=24, max_iter=400).solve(sol, rhs, 1e-10, 0.0)
+        print("precision", prec, "iterations", st[prec]["iterations"], "seconds", st[prec]["seconds"], flush=True)
+        assert st[prec]["converged"] == 1
+        x[prec] = to_host(sol)
+        del op, sol
+        torch.cuda.empty_cache()
+    rh = to_host(rhs)
+    r = ora.residual(pr["ncell"], pr["dx"], P.ISO, pr["C"], pr["bct"], x[PREC_MIXED], rh)
+    rel = np.abs(r).max() / np.abs(rh).max()
+    err = _rel(x[PREC_MIXED], x[0])
+    print("oracle |b - A u|inf / |b|inf %.3e, rel L2 mixed vs FP64 %.3e" % (rel, err), flush=True)
+    assert rel <= 1e-9
+    assert err <= 1e-8
