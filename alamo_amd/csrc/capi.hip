@@ -105,6 +105,7 @@ struct LevelBuf {
     float *corf = nullptr, *resf = nullptr, *rescorf = nullptr, *tmpf = nullptr, *holdf = nullptr;
     double* dsm = nullptr;  // l1-Jacobi smoother diagonal (3 planes), allocated when that smoother is used
     float* dsmf = nullptr;
+    uint16_t* dsmr = nullptr;  // BF16 reciprocal of dsm for the FP32 row-kernel sweeps (mixed precision)
     double* S = nullptr;    // Galerkin stencil of a coarse level (GAL_NPLANE planes), coarse_op GALERKIN
     float* Sf = nullptr;
     double* psi = nullptr;  // Elastic::m_psi_mf of the level (cells), when set
@@ -712,6 +713,7 @@ static int prepare(alamo_mg* h) {
         else launch_node<double>(h->sym, DIAG, lb.g, h->bc, lb.C, nullptr, nullptr, lb.D, 0.0, h->stream);
         lb.g.dsm = nullptr;
         lb.g.dsmf = nullptr;
+        lb.g.dsmr = nullptr;
         if (l1) {
             // the smoother diagonal, from the level's coefficients and boundary table (or stencil)
             if (!lb.dsm) TRY(dalloc(h, &lb.dsm, 3 * lb.g.tot));
@@ -727,6 +729,9 @@ static int prepare(alamo_mg* h) {
                 k_convert<double, float><<<gs_blocks(3 * lb.g.tot), 256, 0, h->stream>>>(lb.dsm, lb.dsmf,
                                                                                          3 * lb.g.tot);
                 lb.g.dsmf = lb.dsmf;
+                if (!lb.dsmr) TRY(dalloc(h, &lb.dsmr, 3 * lb.g.tot));
+                k_recip_bf16<<<gs_blocks(3 * lb.g.tot), 256, 0, h->stream>>>(lb.dsm, lb.dsmr, 3 * lb.g.tot);
+                lb.g.dsmr = lb.dsmr;
             }
         }
         if (h->d.precision == ALAMO_PREC_MIXED) {

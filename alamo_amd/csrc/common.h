@@ -57,6 +57,9 @@ struct Geom {
     // so a sweep is x += (b - A x) / dsm
     const double* dsm;
     const float* dsmf;
+    // BF16 reciprocal of the l1 diagonal (3 planes of bfloat16 bit patterns) for the FP32 row-kernel
+    // sweeps, which multiply by it: 6 bytes per node instead of 12 and no division; null when absent
+    const uint16_t* dsmr;
     // MajorMinor levels after preparation: the node-local contraction of the stiffness gradient,
     // D_ikl = sum_j d_j C_ijkl (central differences; symmetric in k, l), 18 planes i * 6 + pair(k, l),
     // FP64 and FP32 copies; null when not prepared (the rows then difference C themselves)

@@ -495,6 +495,14 @@ __global__ void k_fill(T* __restrict__ x, int64_t n, T v) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         x[q] = v;
 }
+// bfloat16 bit pattern of 1 / d (round to nearest even), the FP32 sweeps' smoother reciprocal
+__global__ void k_recip_bf16(const double* __restrict__ d, uint16_t* __restrict__ out, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, (float)(1.0 / d[q]));
+        out[q] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+}
+
 template <typename Ti, typename To>
 __global__ void k_convert(const Ti* __restrict__ a, To* __restrict__ b, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)

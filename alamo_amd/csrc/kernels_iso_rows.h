@@ -210,8 +210,8 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
     T* __restrict__ O[3] = {out, out + g.tot, out + 2 * g.tot};
     const T idx = T(1.0 / g.dx[0]), idy = T(1.0 / g.dx[1]), idz = T(1.0 / g.dx[2]);
     const bool l1 = MODE == JACOBI && smoother_diag<T>(g) != nullptr;  // l1-Jacobi smoother level
-    // FP32 sweeps read the prepared l1 diagonal (12 more bytes per node, against the VALU work of
-    // forming it here: the 257^3 solve 0.396 -> 0.389 s); FP64 sweeps form it from the row below
+    // FP32 sweeps read the prepared l1 diagonal's BF16 reciprocal (6 more bytes per node, against
+    // the VALU work of forming it here); FP64 sweeps form it from the row below
     constexpr bool L1_READ = sizeof(T) == 4;
     const T hx = T(0.5) * idx, hy = T(0.5) * idy, hz = T(0.5) * idz;
     const T ix2 = idx * idx, iy2 = idy * idy, iz2 = idz * idz, hxy = hx * hy;
@@ -250,8 +250,15 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
             for (int v = 0; v < 3; v++) bq[v] = ldb(B[v], mcb);
         }
         if (L1_READ && l1 && out_i) {
+            if (g.dsmr) {
+                // the BF16 reciprocal: its bit pattern is the high half of an FP32 value
 #pragma unroll
-            for (int v = 0; v < 3; v++) dq[v] = ldb(smoother_diag<T>(g) + (int64_t)v * g.tot, mcb);
+                for (int v = 0; v < 3; v++)
+                    dq[v] = (T)__builtin_bit_cast(float, (uint32_t)ldb(g.dsmr + (int64_t)v * g.tot, mcb >> 1) << 16);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 3; v++) dq[v] = T(1) / ldb(smoother_diag<T>(g) + (int64_t)v * g.tot, mcb);
+            }
         }
         if (out_fh) {
 #pragma unroll
@@ -364,7 +371,7 @@ k_iso_rows64(Geom g, const T* __restrict__ C, const T* __restrict__ u, const T* 
                 else {
                     const T D = T(-2) * ((lam + T(2) * mu) * w2[v] + mu * (ix2 + iy2 + iz2 - w2[v]));
                     if (l1 && L1_READ) {
-                        stb(O[v], mcb, qc.u[v] + (bq[v] - f) / dq[v]);
+                        stb(O[v], mcb, qc.u[v] + (bq[v] - f) * dq[v]);
                     } else if (l1) {
                         const T io = SIH - ih[v];  // sum of 1/h_a over a != v
                         const T S = T(2) * fmax((lam + T(2) * mu) * w2[v], fabs(dl[v] + T(2) * dm[v]) * hh[v]) +
