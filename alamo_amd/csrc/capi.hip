@@ -23,6 +23,7 @@
 #include "kernels_generic.h"
 #include "kernels_iso.h"
 #include "kernels_iso_rows.h"
+#include "kernels_tail.h"
 #include "krylov.h"
 #include "amr.h"
 #include "post.h"
@@ -427,12 +428,37 @@ static void bottom(alamo_mg* h) {
         k_convert<double, float><<<gs_blocks(n), 256, 0, h->stream>>>(lb.cor, lb.corf, n);
 }
 
+// The FP32 l1 cycle's last smoothed level runs as two single-workgroup launches around the dense
+// bottom (k_cycle_tail) when it has at most TAIL_MAX nodes, all-Dirichlet isotropic rows and a
+// prepared l1 diagonal; ALAMO_CYCLE_TAIL=0 (read at every call) runs it as separate launches.
+template <typename T>
+static bool cycle_tail_ok(const alamo_mg* h) {
+    if constexpr (!std::is_same<T, float>::value) return false;
+    else {
+        const int bot = (int)h->L.size() - 1;
+        if (bot < 1 || h->d.bottom_solver != ALAMO_BOTTOM_DENSE || h->gal || h->sym != ISO || !all_dirichlet(h->bc))
+            return false;
+        const Geom& g = h->L[bot - 1].g;
+        const Geom& gc = h->L[bot].g;
+        if (g.psi || g.uniform || !g.dsmr || !g.dsmf || g.tot > TAIL_MAX || gc.tot > TAIL_MAXC || !h->dense_inv)
+            return false;
+        for (int d = 0; d < 3; d++)
+            if (g.n[d] < 2 || g.n[d] != 2 * gc.n[d] || g.sy != g.N[0] || g.sz != (int64_t)g.N[0] * g.N[1])
+                return false;
+        if (gc.sy != gc.N[0] || gc.sz != (int64_t)gc.N[0] * gc.N[1]) return false;
+        const char* e = getenv("ALAMO_CYCLE_TAIL");
+        return !(e && e[0] == '0');
+    }
+}
+
 // guess: the top level's cor already holds an initial guess (its pre-smoothing sweeps start from it)
 template <typename T>
 static void vcycle(alamo_mg* h, int top, bool first_done = false, bool guess = false) {
     const int bot = (int)h->L.size() - 1;
+    const bool tail = top <= bot - 1 && cycle_tail_ok<T>(h);
+    const int last = tail ? bot - 1 : bot;  // levels top .. last - 1 by separate launches
     bool done = first_done;
-    for (int l = top; l < bot; l++) {
+    for (int l = top; l < last; l++) {
         LevelBuf& lb = h->L[l];
         if (guess && l == top) sweeps<T>(h, l, h->d.pre_smooth * sps(h, l));
         else presmooth_from_zero<T>(h, l, done);
@@ -444,8 +470,30 @@ static void vcycle(alamo_mg* h, int top, bool first_done = false, bool guess = f
                                                                  done ? LB<T>::cor(lc) : nullptr, sweep_diag<T>(h, lc),
                                                                  T(sweep_omega(h)));
     }
-    bottom<T>(h);
-    for (int l = bot - 1; l >= top; l--) {
+    if (tail) {
+        if constexpr (std::is_same<T, float>::value) {
+            const int l = bot - 1;
+            LevelBuf& lb = h->L[l];
+            const int total = h->d.pre_smooth * sps(h, l);
+            int init = 0, npre = total;  // guess at the top level: sweeps from cor
+            if (!(guess && l == top)) {
+                if (total == 0) init = 2;
+                else {
+                    init = done ? 0 : 1;  // the first sweep from zero already written by the restriction, or here
+                    npre = total - 1;
+                }
+            }
+            LevelBuf& lc = h->L[bot];
+            k_cycle_tail<0><<<1, TAIL_NT, 0, h->stream>>>(lb.g, lc.g, lb.Cf, lb.dsmf, lb.dsmr, lb.resf, lb.corf,
+                                                          lc.resf, nullptr, init, npre);
+            bottom<T>(h);
+            k_cycle_tail<1><<<1, TAIL_NT, 0, h->stream>>>(lb.g, lc.g, lb.Cf, lb.dsmf, lb.dsmr, lb.resf, lb.corf,
+                                                          nullptr, lc.corf, 0, h->d.post_smooth * sps(h, l));
+        }
+    } else {
+        bottom<T>(h);
+    }
+    for (int l = last - 1; l >= top; l--) {
         LevelBuf& lb = h->L[l];
         LevelBuf& lc = h->L[l + 1];
         launch_interp<T>(lc.g, lb.g, LB<T>::cor(lc), LB<T>::cor(lb), h->stream);

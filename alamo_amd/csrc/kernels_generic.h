@@ -495,6 +495,9 @@ __global__ void k_fill(T* __restrict__ x, int64_t n, T v) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         x[q] = v;
 }
+// FP32 value of a bfloat16 bit pattern
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
 // bfloat16 bit pattern of 1 / d (round to nearest even), the FP32 sweeps' smoother reciprocal
 __global__ void k_recip_bf16(const double* __restrict__ d, uint16_t* __restrict__ out, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
