@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv3_fwd(Grid3 g, int Cin,
 }
 
 // ---------------------------------------------------------------------------
-// Forward convolution for 16 or 32 input channels with the input staged in LDS: a workgroup of
+// Forward convolution for 16, 32 or 64 input channels with the input staged in LDS: a workgroup of
 // CL_TY wavefronts owns a 64 x CL_TY voxel tile of an xy plane and marches CL_KZ planes in z;
 // a ring of three input planes (66 x (CL_TY + 2) voxels with their halo, zero outside the
 // domain) lives in LDS, and each wavefront reads its 27 shifted A fragments from there instead
@@ -194,11 +194,12 @@ __global__ void __launch_bounds__(64 * CL_TY) k_conv3_fwd_lds(Grid3 g, int xb, c
         // gfx950's 16x16x32 products (operand layout at k_conv3_wgrad_lds): K = 32 is two offsets of
         // the 16 channels (lanes 0-31 the even offset of a pair, lanes 32-63 the odd one; the 27th
         // offset pairs with zero weights), or the 32 channels of one offset
-        constexpr int NP = CIN == 16 ? 14 : 27;
+        // (64 channels: two products per offset, channels 0-31 and 32-63)
+        constexpr int NP = CIN == 16 ? 14 : 27 * (CIN / 32);
         const int half = lane >> 5;
-        const int c0 = CIN == 16 ? 8 * ((lane >> 4) & 1) : 8 * (lane >> 4);
         for (int pp = 0; pp < NP; pp++) {
-            const int o = CIN == 16 ? 2 * pp + half : pp;
+            const int o = CIN == 16 ? 2 * pp + half : (CIN == 32 ? pp : pp >> 1);
+            const int c0 = CIN == 16 ? 8 * ((lane >> 4) & 1) : (CIN == 32 ? 0 : 32 * (pp & 1)) + 8 * (lane >> 4);
             const bool live = o < 27;
             int dx, dy, dz;
             off3(live ? o : 26, dx, dy, dz);

@@ -132,14 +132,23 @@ static void conv_pass(alamo_net* n, const net::Grid3& g, int cin, int xb, int xe
                       const uint16_t* W, const float* bias, void* out, bool out_f32, int relu) {
     const int tx = (xe - xb + 16 * MT - 1) / (16 * MT);
     if (tx <= 0) return;
-    // LDS staging pays for 16 input channels and for 32 with at most 32 outputs (with 64 outputs
-    // its register footprint leaves one wavefront per SIMD)
-    if ((cin == 16 || (cin == 32 && NT <= 2)) && g_conv_lds) {
+    // LDS staging pays for 16 input channels and for 32 or 64 with at most 32 outputs (with 64
+    // outputs its register footprint leaves one wavefront per SIMD); 64 input channels take tiles
+    // of at most 32 voxels (a 64-voxel tile's three planes would fill the LDS of a CU)
+    const bool lds64 = cin == 64 && NT <= 2 && MT <= 2;
+    if ((cin == 16 || (cin == 32 && NT <= 2) || lds64) && g_conv_lds) {
         dim3 grid(tx, (g.N[1] + net::CL_TY - 1) / net::CL_TY, (g.N[2] + net::CL_KZ - 1) / net::CL_KZ);
         const unsigned blk = 64 * net::CL_TY;
         if (cin == 16) {
             if (out_f32) net::k_conv3_fwd_lds<16, NT, MT, true><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
             else net::k_conv3_fwd_lds<16, NT, MT, false><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+        } else if (cin == 64) {
+            if constexpr (MT <= 2 && NT <= 2) {
+                if (out_f32)
+                    net::k_conv3_fwd_lds<64, NT, MT, true><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+                else
+                    net::k_conv3_fwd_lds<64, NT, MT, false><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
+            }
         } else {
             if (out_f32) net::k_conv3_fwd_lds<32, NT, MT, true><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
             else net::k_conv3_fwd_lds<32, NT, MT, false><<<grid, blk, 0, n->s>>>(g, xb, in, W, bias, out, relu);
@@ -157,6 +166,12 @@ static void conv_pass(alamo_net* n, const net::Grid3& g, int cin, int xb, int xe
 template <int NT>
 static void conv_launch_nt(alamo_net* n, const net::Grid3& g, int cin, const uint16_t* in, const uint16_t* W,
                            const float* bias, void* out, bool out_f32, int relu) {
+    if (cin == 64 && NT <= 2 && g_conv_lds) {
+        const int xm = (g.N[0] / 32) * 32;  // 32-voxel tiles over [0, xm), 16-voxel tiles over [xm, N0)
+        conv_pass<NT, 2>(n, g, cin, 0, xm, in, W, bias, out, out_f32, relu);
+        conv_pass<NT, 1>(n, g, cin, xm, g.N[0], in, W, bias, out, out_f32, relu);
+        return;
+    }
     const int xm = (g.N[0] / 64) * 64;  // 64-voxel tiles over [0, xm), 16-voxel tiles over [xm, N0)
     conv_pass<NT, 4>(n, g, cin, 0, xm, in, W, bias, out, out_f32, relu);
     conv_pass<NT, 1>(n, g, cin, xm, g.N[0], in, W, bias, out, out_f32, relu);
