@@ -435,8 +435,10 @@ template <typename T>
 static bool cycle_tail_ok(const alamo_mg* h) {
     if constexpr (!std::is_same<T, float>::value) return false;
     else {
+        // level 0 is never the tail level: the preconditioner entry points alias its buffers to the
+        // Krylov vectors and count on one buffer swap per sweep there
         const int bot = (int)h->L.size() - 1;
-        if (bot < 1 || h->d.bottom_solver != ALAMO_BOTTOM_DENSE || h->gal || h->sym != ISO || !all_dirichlet(h->bc))
+        if (bot < 2 || h->d.bottom_solver != ALAMO_BOTTOM_DENSE || h->gal || h->sym != ISO || !all_dirichlet(h->bc))
             return false;
         const Geom& g = h->L[bot - 1].g;
         const Geom& gc = h->L[bot].g;
